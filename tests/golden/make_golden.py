@@ -1,0 +1,152 @@
+"""Writes the golden fixtures in tests/golden/ from the reference itself.
+
+Runs the reference Fortran/MPI library (built from the reference sources by
+``make -C oracle ref`` into oracle/_ref/, driven by our own
+oracle/ref_driver/*.f90) on seeded inputs and stores inputs and outputs as
+compressed .npz files. Run it in the build container (where the reference
+tree and MPICH are present):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+The fixtures are data only: inputs, expected outputs and residual histories.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import refrun  # noqa: E402
+from mps3d_amd import workloads as W  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def case_dict(c: W.Case, **kw):
+    d = dict(n=c.n, len=c.length, bc_codes=list(c.bc_codes), bc_vals=c.bc_vals,
+             dcoeff=c.dcoeff, velx=c.velx, vely=c.vely, velz=c.velz, reac=c.reac,
+             source=c.source, timederiv=c.timederiv, dt=c.dt)
+    d.update(kw)
+    return d
+
+
+def inputs_of(c: W.Case):
+    z = np.zeros(c.n, order="F")
+    d = {"n": np.array(c.n), "length": np.array(c.length),
+         "bc_codes": np.array(["%-4s" % s for s in c.bc_codes]),
+         "timederiv": np.array(int(c.timederiv)), "dt": np.array(c.dt)}
+    for i, f in enumerate(c.bc_vals):
+        d["bc%d" % i] = f
+    for k in ("dcoeff", "velx", "vely", "velz", "reac", "source"):
+        v = getattr(c, k)
+        d[k] = z if v is None else v
+    return d
+
+
+def save(name, **arrays):
+    path = os.path.join(OUT, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def main():
+    assert refrun.available(), "build the reference first: make -C oracle ref"
+    timings = {}
+
+    # --- operator, RHS and smoother on a mixed-BC random problem -------------
+    c = W.random_cdr((12, 10, 8), seed=11)
+    rng = np.random.default_rng(12)
+    u0 = rng.standard_normal(c.n)
+    st = rng.standard_normal(c.n)
+    o = refrun.run(case_dict(c, mode=refrun.MODE_APPLY, u0=u0, sterm=st))
+    s = refrun.run(case_dict(c, mode=refrun.MODE_SMOOTH, u0=u0, sterm=st, niter=2))
+    save("operator_mixed", u0=u0, sterm=st, AX=o["AX"], DIAG=o["DIAG"], OFFDIAG=o["OFFDIAG"],
+         BVEC=o["BVEC"], BVECNOBC=o["BVECNOBC"], SMOOTH_U=s["U"], SMOOTH_RES=s["RES"], **inputs_of(c))
+
+    # Poisson-only variant (no convection, reaction or time term)
+    c2 = W.random_cdr((9, 7, 6), seed=21, codes=("DIRC", "DIRC", "ZGRD", "FLUX", "DIRC", "ZGRD"),
+                      timederiv=False)
+    c2.velx = c2.vely = c2.velz = c2.reac = None
+    o = refrun.run(case_dict(c2, mode=refrun.MODE_APPLY, u0=u0[:9, :7, :6], sterm=st[:9, :7, :6]))
+    save("operator_poisson", u0=u0[:9, :7, :6], sterm=st[:9, :7, :6], AX=o["AX"], DIAG=o["DIAG"],
+         OFFDIAG=o["OFFDIAG"], BVEC=o["BVEC"], **inputs_of(c2))
+
+    # --- grid transfers and ghost rules -------------------------------------
+    ct = W.random_cdr((8, 6, 4), seed=31, codes=("DIRC", "ZGRD", "FLUX", "DIRC", "ZGRD", "DIRC"))
+    fine = rng.standard_normal(ct.n)
+    coarse_g = rng.standard_normal((4 + 2, 3 + 2, 2 + 2))
+    o = refrun.run(case_dict(ct, mode=refrun.MODE_TRANSFER, u0=fine, extra=coarse_g))
+    ghost_in = rng.standard_normal((8 + 2, 6 + 2, 4 + 2))
+    g = refrun.run(case_dict(ct, mode=refrun.MODE_GHOST, extra=ghost_in))
+    save("transfer", fine=fine, coarse_g=coarse_g, RESTR=o["RESTR"].reshape((4, 3, 2), order="F"),
+         PROLONG=o["PROLONG"], BCLEFT=o["BCLEFT"], BCRIGHT=o["BCRIGHT"], BCBOTTOM=o["BCBOTTOM"],
+         BCTOP=o["BCTOP"], BCBACK=o["BCBACK"], BCFRONT=o["BCFRONT"], ghost_in=ghost_in,
+         GHOSTED=g["GHOSTED"].reshape((10, 8, 6), order="F"), **inputs_of(ct))
+
+    # --- restarted GMRES with the GS preconditioner --------------------------
+    cg = W.random_cdr((4, 4, 4), seed=41)
+    cg.dcoeff = cg.dcoeff + 2.0  # keep the one-sweep GS preconditioner well defined
+    b = rng.standard_normal(cg.n)
+    x0 = rng.standard_normal(cg.n)
+    runs = {}
+    for tag, (m, nr, tol) in {"full": (64, 1, 1e-15), "restarted": (6, 8, 1e-12)}.items():
+        o = refrun.run(case_dict(cg, mode=refrun.MODE_GMRES, u0=x0, b=b, maxkspdim=m,
+                                 nrestarts=nr, tol=tol))
+        runs["X_" + tag] = o["X"]
+        runs["cfg_" + tag] = np.array([m, nr, tol])
+    save("gmres_cdr", b=b, x0=x0, **runs, **inputs_of(cg))
+
+    # --- V-cycle solve trajectories ------------------------------------------
+    def solve_fixture(name, c, p, ncyc, tol, full=True):
+        t = time.time()
+        o = refrun.run(case_dict(c, mode=refrun.MODE_SOLVE, p=p, nvcycles=ncyc, tol=tol), timeout=3600)
+        timings[name] = {"ranks": int(np.prod(p)), "solve_s": float(o["TIME"][0]),
+                         "wall_s": time.time() - t, "n": list(c.n),
+                         "cycles": int(np.sum(o["HIST"] >= 0))}
+        hist = o["HIST"][o["HIST"] >= 0]
+        U = o["U"]
+        nx, ny, nz = c.n
+        extra = {"U": U} if full else {
+            "line_x": U[:, ny // 2, nz // 2], "line_y": U[nx // 2, :, nz // 2], "line_z": U[nx // 2, ny // 2, :],
+            "stats": np.array([U.sum(), (U * U).sum(), U.min(), U.max()])}
+        params = {}
+        if "blobs" in c.params:
+            params["blobs"] = c.params["blobs"]
+        save(name, hist=hist, p=np.array(p), tol=np.array(tol), ncyc=np.array(ncyc),
+             kind=np.array(c.params.get("kind", "")), **params, **extra,
+             **({} if not full else inputs_of(c)))
+
+    tol_generic = 1e-11 * float(np.float32(0.001))  # generic_pde/pdedefs.f90:368
+    solve_fixture("solve_laplace16_p1", W.laplace_deck(16), (1, 1, 1), 20, tol_generic)
+    solve_fixture("solve_laplace16_p8", W.laplace_deck(16), (2, 2, 2), 20, tol_generic)
+    vc16 = W.varcoef_poisson(16)
+    solve_fixture("solve_varcoef16_p1", vc16, (1, 1, 1), 20, 0.0)
+    solve_fixture("solve_varcoef16_p8", vc16, (2, 2, 2), 20, 0.0)
+    solve_fixture("solve_mms20x12x8_p512", W.mms_cdr((20, 12, 8)), (5, 1, 2), 10, 0.0)
+    solve_fixture("solve_mms16x8x8_p1", W.mms_cdr((16, 8, 8)), (1, 1, 1), 10, 0.0)
+    solve_fixture("solve_laplace64_p1", W.laplace_deck(64), (1, 1, 1), 20, tol_generic, full=False)
+    solve_fixture("solve_laplace64_p8", W.laplace_deck(64), (2, 2, 2), 20, tol_generic, full=False)
+    solve_fixture("solve_varcoef64_p8", W.varcoef_poisson(64), (2, 2, 2), 20, 0.0, full=False)
+    solve_fixture("solve_varcoef128_p8", W.varcoef_poisson(128), (2, 2, 2), 20, 0.0, full=False)
+
+    # --- 1D Scharfetter-Gummel flux ------------------------------------------
+    rows = []
+    for cL in (-3.0, -1e-3, -1e-9, 0.0, 2e-11, 1e-6, 0.7, 50.0):
+        for cR in (-2.0, 0.0, 1e-3, 4.0):
+            for DL, DR in ((1.0, 1.0), (0.3, 2.0), (1e-12, 1e-12), (5.0, 1e-3)):
+                rows.append([cL, cR, DL, DR, 1.3, -0.4, 0.05])
+    rows = np.array(rows)
+    save("sg1d", rows=rows, out=refrun.run_sg1d(rows))
+
+    with open(os.path.join(OUT, "ref_timings.json"), "w") as fh:
+        json.dump({"host": os.uname().nodename, "nproc": os.cpu_count(), "runs": timings}, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()
