@@ -1,0 +1,184 @@
+/* mps3d_amd: MI355X (gfx950) implicit field-solve library -- C ABI.
+ *
+ * Drop-in for the multigrid call surface of mps3d's mgridlib. Each entry
+ * point names the reference routine it replaces (file:line in the
+ * reference tree). Plain pointers and sizes only; callable from Fortran via
+ * ISO_C_BINDING (mps3d_amd/fortran/mps3d_gpu.f90), from Python via ctypes
+ * (mps3d_amd/_lib.py) or from C/C++.
+ *
+ * Layouts. Solution and coefficient arrays are the reference's ghosted
+ * Fortran arrays, (0:nx+1, 0:ny+1, 0:nz+1) real*8, i fastest
+ * (mgridlib/discretepde.f90:75-104). Interior arrays (mgridsterm, AX, diag,
+ * offdiag, residual) are (nx,ny,nz). BC faces follow
+ * mgridlib/globalvars.f90:57-66: left/right (ny,nz), bottom/top (nx,nz),
+ * back/front (nx,ny). Any array pointer may be host or device memory; the
+ * library detects which (hipPointerGetAttributes) and works in place on
+ * device arrays.
+ *
+ * Status codes replace the reference's abort()/MPI_Abort paths
+ * (mgridlib/gmres.f90:396-400, mgridlib/mgridsteps.f90:357-360).
+ *
+ * Threading: a context owns one HIP stream and its device buffers; it is not
+ * re-entrant. Use one host thread per context.
+ */
+#ifndef MPS3D_H
+#define MPS3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define MPS_OK 0
+#define MPS_NOT_CONVERGED 1   /* tolerance not met within max_cycles      */
+#define MPS_ERR_NAN (-1)      /* NaN/Inf met (gmres.f90:113-120)          */
+#define MPS_ERR_ARG (-2)      /* bad arguments or unsupported grid        */
+#define MPS_ERR_HIP (-3)      /* HIP runtime error                         */
+#define MPS_ERR_STATE (-4)    /* call out of order (e.g. no coefficients)  */
+#define MPS_ERR_NOLIB (-5)    /* no usable gfx950 device                   */
+
+/* ---- BC codes: character(4) 'DIRC','ZGRD','FLUX' of the reference;
+ *      any other code is a no-flux face with no row term ---------------- */
+enum { MPS_BC_NONE = 0, MPS_BC_DIRC = 1, MPS_BC_ZGRD = 2, MPS_BC_FLUX = 3 };
+
+/* ---- solver forms ----------------------------------------------------- */
+enum {
+    /* the reference FAS V-cycle: 8-child mean restriction, prolongation1,
+     * rediscretised coarse operators, GMRES bottom solve on the coarse grid
+     * fixed by the process grid (mgridsteps.f90:384-579). Needs per-rank
+     * sizes nx/px.. that halve evenly down to 1. */
+    MPS_HIER_REF = 0,
+    /* cell-centred grids of m*2^k+1 cells: coarse cells sit on every other
+     * fine cell centre (65 -> 33 -> 17 -> 9 -> 5 -> 3), volume-weighted
+     * full-weighting restriction, trilinear prolongation, correction scheme,
+     * direct bottom solve. No reference counterpart (the reference cannot
+     * coarsen odd sizes, mgridsteps.f90:21-23). */
+    MPS_HIER_ALIGNED = 1
+};
+enum {
+    MPS_PREC_FP64 = 0,    /* every kernel in FP64 (reference arithmetic)   */
+    MPS_PREC_FP32_DC = 1  /* FP32 smoothing/restriction/prolongation inside
+                             an FP64 defect-correction loop; the residual is
+                             always evaluated in FP64                        */
+};
+enum {
+    MPS_FLUX_UPWIND = 0,  /* reference face fluxes: arithmetic-mean D,
+                             first-order upwind convection
+                             (discretepde.f90:620-670)                      */
+    MPS_FLUX_SG = 1       /* Scharfetter-Gummel exponential fluxes with the
+                             harmonic-mean D of mps1d/source/convdiff.f90:10-64,
+                             applied per face in 3D                         */
+};
+enum {
+    MPS_TOL_RMS = 0,      /* RMS of the absolute residual (globalvars.f90:136-159) */
+    MPS_TOL_REL_L2 = 1    /* ||b - A u||_2 / ||b||_2                              */
+};
+
+typedef struct {
+    int nx, ny, nz;             /* global cells (g_Nx, g_Ny, g_Nz)             */
+    double lenx, leny, lenz;    /* domain lengths (g_xlen, g_ylen, g_zlen)     */
+    int px, py, pz;             /* process grid (g_px..): fixes the coarsest
+                                   level of MPS_HIER_REF; 1,1,1 otherwise      */
+} mps_grid;
+
+typedef struct {
+    int hierarchy;              /* MPS_HIER_*                                  */
+    int precision;              /* MPS_PREC_*                                  */
+    int flux;                   /* MPS_FLUX_*                                  */
+    int nu1, nu2;               /* pre/post RB-GS sweeps (reference: 2, 2,
+                                   mgridsteps.f90:473)                         */
+    int max_cycles;             /* V-cycles (or defect-correction steps) per
+                                   mps_solve (reference loops: 20)             */
+    double tol;                 /* stopping tolerance for mps_solve            */
+    int tol_kind;               /* MPS_TOL_*                                   */
+} mps_opts;
+
+typedef struct mps_ctx mps_ctx;
+
+/* Fills reference defaults: REF hierarchy, FP64, upwind, nu1=nu2=2,
+ * max_cycles=20, tol=1e-11 RMS (globalvars.f90:29). */
+void mps_default_opts(mps_opts *opts);
+
+/* Allocates a context for one grid on the current HIP device. Replaces the
+ * setup of domaindecompose (par_decompose.f90:7-85): the whole domain lives
+ * on one GPU and the halo exchange becomes in-device ghost layers. */
+int mps_create(mps_ctx **ctx, const mps_grid *grid, const mps_opts *opts);
+int mps_destroy(mps_ctx *ctx);
+
+/* Boundary conditions: codes[6] (MPS_BC_*) and six face arrays (NULL = 0),
+ * as bc_codes/bcvals of perform_vcycle (mgridsteps.f90:392-393). */
+int mps_set_bc(mps_ctx *ctx, const int codes[6], const double *const faces[6]);
+/* Same, with the Fortran character(4) codes packed as 24 chars. */
+int mps_set_bc_str(mps_ctx *ctx, const char codes[24], const double *const faces[6]);
+
+/* Operator coefficients, ghosted layout, NULL = 0: dcoeff, vel (x,y,z),
+ * reac, source, and the time-derivative flag and step of perform_vcycle's
+ * argument list (mgridsteps.f90:384-415). Coarse-level coefficients are
+ * derived here once per call (the reference re-restricts them on every
+ * V-cycle, mgridsteps.f90:533-539). */
+int mps_set_coeffs(mps_ctx *ctx, const double *dcoeff, const double *velx, const double *vely,
+                   const double *velz, const double *reac, const double *source, int timederiv,
+                   double dt);
+
+/* perform_vcycle (mgridsteps.f90:384-579) on the finest level: one V-cycle
+ * on soln (ghosted, in/out) with multigrid source mgridsterm (interior,
+ * NULL = 0); resnorm gets the RMS residual after the cycle. Ghost values of
+ * soln are left as the caller set them, as in the reference. */
+int mps_vcycle(mps_ctx *ctx, double *soln, const double *mgridsterm, double *resnorm);
+
+/* The solve loop of the equation modules (poissoneq.f90:427-450,
+ * pdedefs.f90:378-402): up to opts.max_cycles V-cycles (or FP32
+ * defect-correction steps), stopping when the opts.tol_kind residual measure
+ * <= opts.tol. history (nullable, max_cycles entries) gets the measure after
+ * each cycle. Returns MPS_OK or MPS_NOT_CONVERGED. */
+int mps_solve(mps_ctx *ctx, double *soln, int *ncycles, double *resnorm, double *history);
+
+/* find_AX (discretepde.f90:672-1119): AX, diag and offdiag (interior arrays,
+ * each nullable) for the ghosted input x. */
+int mps_apply(mps_ctx *ctx, const double *x, double *ax, double *diag, double *offdiag);
+
+/* Transposed operator: ATX = A^T x (interior output) for the adjoint. */
+int mps_apply_transpose(mps_ctx *ctx, const double *x, double *atx);
+
+/* find_bvec (discretepde.f90:382-618): b = BC terms + mgridsterm + source
+ * (BC and source terms only when applybc != 0). */
+int mps_bvec(mps_ctx *ctx, const double *mgridsterm, int applybc, double *b);
+
+/* perform_gseidel_smoothing (discretepde.f90:132-236) on the finest level:
+ * niter red-black sweeps, then res = b - A soln (interior, nullable). */
+int mps_smooth(mps_ctx *ctx, double *soln, const double *mgridsterm, int niter, double *res);
+
+/* update_boundary_ghostvalues (discretepde.f90:1121-1267): DIRC ghost =
+ * 2 g - u, other codes ghost = u, on the six faces of soln. */
+int mps_fill_ghosts(mps_ctx *ctx, double *soln);
+
+/* performgmres (gmres.f90:197-434) on the finest level: restarted GMRES(m)
+ * for A x = b (b interior, x ghosted in/out), left-preconditioned by one
+ * V-cycle (precond = 1) or by one red-black sweep from zero as
+ * gseidel_precond (precond = 0). Converged when the preconditioned
+ * residual <= tol relative to ||M^-1 b||; the in-loop exit of
+ * mps1d/source/gmres.f90:343 is used. iters and relres are nullable. */
+int mps_gmres(mps_ctx *ctx, const double *b, double *x, int m, int max_restarts, double tol,
+              int precond, int *iters, double *relres);
+
+/* Adjoint solve: A^T lambda = g (g interior, lambda ghosted out) with the
+ * transposed operator and the same hierarchy; stops as mps_solve. */
+int mps_solve_transpose(mps_ctx *ctx, const double *g, double *lambda, int *ncycles,
+                        double *resnorm);
+
+/* Timing of the last mps_solve / mps_solve_transpose on the context's stream
+ * (HIP events): total milliseconds and per-cycle milliseconds. */
+int mps_last_timing(mps_ctx *ctx, double *total_ms, int *ncycles);
+
+/* The context's HIP stream (hipStream_t), for callers that order work
+ * against it. */
+void *mps_stream(mps_ctx *ctx);
+
+const char *mps_error_string(int status);
+/* Text of the most recent failure on this thread. */
+const char *mps_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPS3D_H */

@@ -1,0 +1,382 @@
+// Coarsest-level solve and Scharfetter-Gummel stencil assembly (gfx950).
+#pragma once
+
+#include "mps3d_kernels.h"
+
+namespace mps {
+
+// ------------------------------------------------------------------------
+// Bottom solve: performgmres (mgridlib/gmres.f90:197-434) with find_AX and
+// gseidel_precond (discretepde.f90:238-337, niter = 1) on the coarsest grid,
+// as bottomsolve calls it (mgridsteps.f90:342-355): m = n, one restart,
+// classical Gram-Schmidt, Givens rotations, lucky breakdown at 1e-25.
+//
+// One workgroup runs the whole solve. Vector updates are spread over the
+// workgroup; inner products and norms are summed by one lane in the
+// reference's memory order, so the coarse solve reproduces the reference's
+// rounding. The coarsest grid holds at most kBottomMax cells.
+// ------------------------------------------------------------------------
+constexpr int kBottomMax = 512;
+
+template <typename T>
+__host__ __device__ inline long long bottom_work_elems(int m, long long ng)
+{
+    return (long long)(m + 1) * ng + 8 * ng + (long long)(m + 1) * m + 4LL * (m + 1);
+}
+
+// performgmres with nrestarts restarts of GMRES(m); the coarse solve uses
+// m = n, nrestarts = 1 and tol = ERRTOLR*0.0001 (mgridsteps.f90:305, :342-343).
+template <typename T, int OP, bool VEL, bool REAC>
+__global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__restrict__ b, T *work, int m,
+                                                  int nrestarts, T tol, int *status)
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int n = nx * ny * nz;
+    const long long ng = (long long)(nx + 2) * (ny + 2) * (nz + 2);
+    T *V = work;
+    T *x = V + (long long)(m + 1) * ng;
+    T *x0 = x + ng;
+    T *r0 = x0 + ng;
+    T *r = r0 + ng;
+    T *av = r + ng;
+    T *w = av + ng;
+    T *wj = w + ng;
+    T *mb = wj + ng;
+    T *H = mb + ng;
+    T *cs = H + (long long)(m + 1) * m;
+    T *sn = cs + (m + 1);
+    T *be = sn + (m + 1);
+    T *y = be + (m + 1);
+    __shared__ T sh[2];
+    const int tid = threadIdx.x, nt = blockDim.x;
+
+    auto cidx = [&](int q, int &i, int &j, int &k) {
+        i = q % nx + 1;
+        j = (q / nx) % ny + 1;
+        k = q / (nx * ny) + 1;
+        return (long long)i + (long long)L.sy * j + L.sz * k;
+    };
+    auto zero = [&](T *v) {
+        for (long long q = tid; q < ng; q += nt) v[q] = T(0);
+        __syncthreads();
+    };
+    auto apply = [&](T *out, const T *in) {  // out interior = A in
+        for (int q = tid; q < n; q += nt) {
+            int i, j, k;
+            long long c = cidx(q, i, j, k);
+            T dg, of;
+            row<T, OP, VEL, REAC>(L, in, i, j, k, c, dg, of);
+            out[c] = dg * in[c] + of;
+        }
+        __syncthreads();
+    };
+    // gseidel_precond, niter = 1 (discretepde.f90:238-337)
+    auto precond = [&](T *out, const T *in) {
+        zero(out);
+        if (n == 1) {
+            if (tid == 0) {
+                int i, j, k;
+                long long c = cidx(0, i, j, k);
+                T dg, of;
+                row<T, OP, VEL, REAC>(L, out, i, j, k, c, dg, of);
+                out[c] = in[c] / dg;
+            }
+            __syncthreads();
+            return;
+        }
+        for (int red = 1; red >= 0; --red) {
+            for (int q = tid; q < n; q += nt) {
+                int i, j, k;
+                long long c = cidx(q, i, j, k);
+                if (((i + j + k) & 1) != red) continue;
+                T dg, of;
+                row<T, OP, VEL, REAC>(L, out, i, j, k, c, dg, of);
+                out[c] = (in[c] - of) / dg;
+            }
+            __syncthreads();
+        }
+    };
+    auto dot = [&](const T *a, const T *c2) {  // one lane, reference order
+        if (tid == 0) {
+            T s = T(0);
+            for (int q = 0; q < n; ++q) {
+                int i, j, k;
+                long long c = cidx(q, i, j, k);
+                s = s + a[c] * c2[c];
+            }
+            sh[0] = s;
+        }
+        __syncthreads();
+        T s = sh[0];
+        __syncthreads();
+        return s;
+    };
+#define HM(i_, j_) H[(long long)((i_) - 1) + (long long)(m + 1) * ((j_) - 1)]
+
+    if (tid == 0) *status = 0;
+    zero(x);
+    zero(r0);
+    for (int q = tid; q < n; q += nt) {
+        int i, j, k;
+        long long c = cidx(q, i, j, k);
+        x[c] = u[c];
+        r0[c] = b[c];
+    }
+    __syncthreads();
+    for (int rs = 1; rs <= nrestarts; ++rs) {
+        for (long long q = tid; q < (long long)(m + 1) * ng; q += nt) V[q] = T(0);
+        for (long long q = tid; q < (long long)(m + 1) * m; q += nt) H[q] = T(0);
+        for (int q = tid; q < m + 1; q += nt) {
+            cs[q] = T(0);
+            sn[q] = T(0);
+            be[q] = T(0);
+            y[q] = T(0);
+        }
+        __syncthreads();
+        // Minvb from b with zero ghosts (r0 is reused as the ghosted b)
+        zero(r0);
+        for (int q = tid; q < n; q += nt) {
+            int i, j, k;
+            long long c = cidx(q, i, j, k);
+            r0[c] = b[c];
+        }
+        __syncthreads();
+        precond(mb, r0);
+        const T bnorm = sqrt(dot(mb, mb));
+        if (bnorm == T(0)) {
+            zero(x);
+            break;
+        }
+        for (long long q = tid; q < ng; q += nt) x0[q] = x[q];
+        __syncthreads();
+        zero(av);
+        apply(av, x0);
+        zero(r0);
+        for (int q = tid; q < n; q += nt) {
+            int i, j, k;
+            long long c = cidx(q, i, j, k);
+            r0[c] = b[c] - av[c];
+        }
+        __syncthreads();
+        precond(r, r0);
+        const T beta = sqrt(dot(r, r));
+        if (tid == 0) be[0] = beta;
+        __syncthreads();
+        if (fabs((double)beta) < (double)tol) break;
+        for (long long q = tid; q < ng; q += nt) V[q] = r[q] / beta;
+        __syncthreads();
+        int optk = m;
+        int lucky = 0;
+        T residnorm = T(0);
+        for (int kd = 1; kd <= m; ++kd) {
+            const T *vj = V + ng * (long long)(kd - 1);
+            zero(av);
+            apply(av, vj);
+            precond(w, av);
+            for (long long q = tid; q < ng; q += nt) wj[q] = w[q];
+            __syncthreads();
+            for (int ii = 1; ii <= kd; ++ii) {  // classical Gram-Schmidt
+                const T *vi = V + ng * (long long)(ii - 1);
+                T h = dot(w, vi);
+                if (tid == 0) HM(ii, kd) = h;
+                for (long long q = tid; q < ng; q += nt) wj[q] = wj[q] - h * vi[q];
+                __syncthreads();
+            }
+            T hn = sqrt(dot(wj, wj));
+            int nanflag = 0;
+            if (hn > T(1e-25)) {
+                T *vn = V + ng * (long long)kd;
+                for (long long q = tid; q < ng; q += nt) vn[q] = wj[q] / hn;
+            } else if (!(hn != hn) && !((hn / hn != T(1)) && hn != T(0))) {
+                lucky = 1;
+            } else {
+                nanflag = 1;
+            }
+            __syncthreads();
+            if (nanflag) {
+                if (tid == 0) *status = -1;
+                return;
+            }
+            if (tid == 0) {  // triangularize (gmres.f90:159-195)
+                HM(kd + 1, kd) = hn;
+                for (int ii = 1; ii <= kd - 1; ++ii) {
+                    T t = cs[ii - 1] * HM(ii, kd) + sn[ii - 1] * HM(ii + 1, kd);
+                    HM(ii + 1, kd) = -sn[ii - 1] * HM(ii, kd) + cs[ii - 1] * HM(ii + 1, kd);
+                    HM(ii, kd) = t;
+                }
+                T rr = HM(kd, kd), ss = HM(kd + 1, kd), c, s;
+                if (rr == T(0)) {
+                    c = T(0);
+                    s = T(1);
+                } else {
+                    c = rr / sqrt(rr * rr + ss * ss);
+                    s = ss / sqrt(rr * rr + ss * ss);
+                }
+                HM(kd, kd) = c * HM(kd, kd) + s * HM(kd + 1, kd);
+                HM(kd + 1, kd) = T(0);
+                cs[kd - 1] = c;
+                sn[kd - 1] = s;
+                be[kd] = -s * be[kd - 1];
+                be[kd - 1] = c * be[kd - 1];
+                sh[1] = fabs(be[kd]);
+            }
+            __syncthreads();
+            residnorm = sh[1];
+            __syncthreads();
+            if (lucky) {
+                optk = kd;
+                break;
+            }
+        }
+        if (tid == 0) {  // uppertrisolve (gmres.f90:124-142)
+            for (int ii = optk; ii >= 1; --ii) {
+                y[ii - 1] = be[ii - 1];
+                for (int jj = ii + 1; jj <= optk; ++jj) y[ii - 1] = y[ii - 1] - HM(ii, jj) * y[jj - 1];
+                y[ii - 1] = y[ii - 1] / HM(ii, ii);
+            }
+        }
+        __syncthreads();
+        for (long long q = tid; q < ng; q += nt) {
+            T xv = x0[q];
+            for (int jj = 1; jj <= optk; ++jj) xv = xv + y[jj - 1] * V[ng * (long long)(jj - 1) + q];
+            x[q] = xv;
+        }
+        __syncthreads();
+        if (residnorm / bnorm <= tol || lucky) break;
+    }
+    for (int q = tid; q < n; q += nt) {
+        int i, j, k;
+        long long c = cidx(q, i, j, k);
+        u[c] = x[c];
+    }
+#undef HM
+}
+
+// ------------------------------------------------------------------------
+// Scharfetter-Gummel operator rows
+// ------------------------------------------------------------------------
+
+// Bernoulli function B(x) = x/(e^x - 1), evaluated in FP64 with expm1 and a
+// Taylor series near 0 (B = 1 - x/2 + x^2/12 - x^4/720 + ...) so that the
+// small-drift limit stays exact.
+__device__ __forceinline__ double bernoulli(double x)
+{
+    const double ax = fabs(x);
+    if (ax < 1e-3) {
+        const double x2 = x * x;
+        return 1.0 - 0.5 * x + x2 * (1.0 / 12.0 - x2 * (1.0 / 720.0));
+    }
+    if (x > 700.0) return x * exp(-x);
+    return x / expm1(x);
+}
+
+// SG weights of one face (wL multiplies the left value, wR the right) in
+// the form of find_sg_exponential_flux (mps1d/source/convdiff.f90:10-64):
+// c = mean velocity, D = harmonic mean with the reference's 1e-10 guard,
+// pure upwind for vanishing D, central diffusion for vanishing Peclet.
+__device__ __forceinline__ void sg_weights(double cL, double cR, double DL, double DR, double d, double &wL,
+                                           double &wR)
+{
+    const double eps = 1e-10;
+    const double ch = 0.5 * (cL + cR);
+    const double Dh = 2.0 * DL * DR / (DL + DR + eps);
+    if (fabs(Dh) < eps) {
+        const double sgn = ch <= 0.0 ? -1.0 : 1.0;
+        wL = cL * 0.5 * (1.0 + sgn);
+        wR = cR * 0.5 * (1.0 - sgn);
+        return;
+    }
+    const double Pe = ch * d / Dh;
+    const double g = Dh / d;
+    if (fabs(Pe) < eps) {
+        wL = g;
+        wR = -g;
+        return;
+    }
+    wL = g * bernoulli(-Pe);
+    wR = -g * bernoulli(Pe);
+}
+
+// Assembles the 7-point SG rows of one level: each interior face carries
+// the flux F = wL u_L + wR u_R (positive along +axis); row L gains F/w_L,
+// row R loses F/w_R. Physical faces keep the reference's boundary rows
+// (DIRC diffusion to the wall plus upwind outflow, ZGRD outflow). Reaction
+// and 1/dt sit on the diagonal. transpose != 0 stores A^T.
+template <int OP>
+__global__ void __launch_bounds__(kBlock) k_sg_stencil(Lev<double> L, double *st0, double *st1, double *st2,
+                                                      double *st3, double *st4, double *st5, double *st6,
+                                                      int transpose)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    double *st[ST_N] = {st0, st1, st2, st3, st4, st5, st6};
+    double a[ST_N];
+    for (int q = 0; q < ST_N; ++q) a[q] = 0.0;
+    double d = 0.0;
+    const double *D = L.D;
+    const double Dc = D[c];
+    const double *vel[3] = {L.vx, L.vy, L.vz};
+    const long long s[3] = {1, L.sy, L.sz};
+    const int idx[3] = {i, j, k};
+    const int nn[3] = {L.nx, L.ny, L.nz};
+    const double h[3] = {L.dx, L.dy, L.dz};
+    const double ai[3] = {L.ax, L.ay, L.az};
+    const double ab[3] = {L.axb, L.ayb, L.azb};
+    const double i0[3] = {L.ix0, L.iy0, L.iz0};
+    for (int dim = 0; dim < 3; ++dim) {
+        const bool edge = idx[dim] == 1 || idx[dim] == nn[dim];
+        const double abv = (OP == OP_ALIGNED && edge) ? ab[dim] : ai[dim];
+        const double vc = vel[dim] ? vel[dim][c] : 0.0;
+        if (idx[dim] > 1) {  // face (c-s | c): this cell is R
+            const long long m = c - s[dim];
+            double wL, wR;
+            sg_weights(vel[dim] ? vel[dim][m] : 0.0, vc, D[m], Dc, h[dim], wL, wR);
+            d = d - wR * abv;
+            a[1 + 2 * dim] = -wL * abv;
+        }
+        if (idx[dim] < nn[dim]) {  // face (c | c+s): this cell is L
+            const long long p = c + s[dim];
+            double wL, wR;
+            sg_weights(vc, vel[dim] ? vel[dim][p] : 0.0, Dc, D[p], h[dim], wL, wR);
+            d = d + wL * abv;
+            a[2 + 2 * dim] = wR * abv;
+        }
+        if (idx[dim] == 1) bc_row<double, OP, true>(L.bc[2 * dim], false, Dc, vc, h[dim], abv, i0[dim], d);
+        if (idx[dim] == nn[dim]) bc_row<double, OP, true>(L.bc[2 * dim + 1], true, Dc, vc, h[dim], abv, i0[dim], d);
+    }
+    if (L.reac) d = d - L.reac[c];
+    if (L.timederiv) d = d + 1.0 / L.dt;
+    a[ST_C] = d;
+    if (!transpose) {
+        for (int q = 0; q < ST_N; ++q) st[q][c] = a[q];
+        return;
+    }
+    st[ST_C][c] = a[ST_C];
+    if (i > 1) st[ST_XP][c - 1] = a[ST_XM];
+    if (i < L.nx) st[ST_XM][c + 1] = a[ST_XP];
+    if (j > 1) st[ST_YP][c - L.sy] = a[ST_YM];
+    if (j < L.ny) st[ST_YM][c + L.sy] = a[ST_YP];
+    if (k > 1) st[ST_ZP][c - L.sz] = a[ST_ZM];
+    if (k < L.nz) st[ST_ZM][c + L.sz] = a[ST_ZP];
+}
+
+// transposes an explicit stencil (st -> stT); stT must be zero-filled
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_transpose_stencil(Lev<T> L, const T *s0, const T *s1, const T *s2,
+                                                             const T *s3, const T *s4, const T *s5, const T *s6,
+                                                             T *t0, T *t1, T *t2, T *t3, T *t4, T *t5, T *t6)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    t0[c] = s0[c];
+    if (i > 1) t2[c - 1] = s1[c];
+    if (i < L.nx) t1[c + 1] = s2[c];
+    if (j > 1) t4[c - L.sy] = s3[c];
+    if (j < L.ny) t3[c + L.sy] = s4[c];
+    if (k > 1) t6[c - L.sz] = s5[c];
+    if (k < L.nz) t5[c + L.sz] = s6[c];
+}
+
+}  // namespace mps

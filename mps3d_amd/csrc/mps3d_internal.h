@@ -1,0 +1,60 @@
+// Internal declarations shared by the HIP kernels and the host driver.
+// Written for gfx950 (MI355X): 64-wide wavefronts, FP64 vector ALUs; every
+// kernel here is an HBM-bandwidth-bound stencil or reduction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace mps {
+
+enum Face { LEFT = 0, RIGHT = 1, BOTTOM = 2, TOP = 3, BACK = 4, FRONT = 5 };
+enum BcCode { BC_NONE = 0, BC_DIRC = 1, BC_ZGRD = 2, BC_FLUX = 3 };
+
+// Operator forms.
+//  OP_REF      reference face fluxes from cell coefficients, uniform
+//              spacing, the reference's exact evaluation order
+//              (mgridlib/discretepde.f90:620-1119).
+//  OP_ALIGNED  same fluxes on the vertex-aligned coarse grids of the
+//              m*2^k+1 hierarchy, whose boundary cells have width
+//              (H + h0)/2 and sit h0/2 from the wall.
+//  OP_STENCIL  explicit 7-point rows: c u + xm u(i-1) + xp u(i+1) + ...
+//              (Scharfetter-Gummel operators and transposed operators).
+enum OpKind { OP_REF = 0, OP_ALIGNED = 1, OP_STENCIL = 2 };
+
+// Stencil slots for OP_STENCIL.
+enum { ST_C = 0, ST_XM, ST_XP, ST_YM, ST_YP, ST_ZM, ST_ZP, ST_N };
+
+// One grid level as the kernels see it. Arrays share the ghosted layout
+// (nx+2, ny+2, nz+2), x fastest: cell (i,j,k), 0 <= i <= nx+1, sits at
+// i + sy*j + sz*k.
+template <typename T>
+struct Lev {
+    int nx, ny, nz;
+    int sy;
+    long long sz;
+    const T *D, *vx, *vy, *vz, *reac;   // cell coefficients (nullable)
+    const T *st[ST_N];                  // explicit stencil rows (OP_STENCIL)
+    T dx, dy, dz;                       // centre spacing
+    T ax, ay, az;                       // 1/width of interior cells
+    T axb, ayb, azb;                    // 1/width of boundary cells (OP_ALIGNED)
+    T ix0, iy0, iz0;                    // 1/(distance centre -> wall) (OP_ALIGNED)
+    int bc[6];
+    int timederiv;
+    T dt;
+};
+
+// Per-level, per-dimension restriction weights of the aligned hierarchy:
+// coarse cell I draws on fine cells 2I-2, 2I-1, 2I with w[3*(I-1)+0..2].
+struct AlignedW {
+    const double *wx, *wy, *wz;
+};
+
+struct Reduce {
+    double *partials;   // one per block
+    int nblocks;
+};
+
+}  // namespace mps

@@ -1,0 +1,759 @@
+// HIP kernels for the multigrid field solve (gfx950).
+//
+// All kernels are bandwidth-bound 7-point stencils, grid transfers or
+// reductions: no MFMA. One thread owns one cell (or one coarse cell for the
+// transfers); threads are laid out over flattened (i,j) planes with
+// blockIdx.y walking k, so consecutive lanes of a 64-wide wavefront touch
+// consecutive x addresses.
+//
+// Reference-arithmetic rule: for T = double and OP_REF the diagonal and
+// off-diagonal sums are accumulated in the order of the reference's find_AX
+// (mgridlib/discretepde.f90:754-1106) and findcellcontributions
+// (:620-670), and the library is built without FMA contraction, so single
+// cells match the reference bit for bit.
+#pragma once
+
+#include "mps3d_internal.h"
+
+namespace mps {
+
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------------
+// cell addressing
+// ------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ bool cell_of(const Lev<T> &L, int &i, int &j, int &k, long long &c)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.nx * L.ny) return false;
+    j = t / L.nx;
+    i = t - j * L.nx + 1;
+    j += 1;
+    k = blockIdx.y + 1;
+    c = (long long)i + (long long)L.sy * j + L.sz * k;
+    return true;
+}
+
+// ------------------------------------------------------------------------
+// reference face contributions (findcellcontributions)
+// ------------------------------------------------------------------------
+// contribution of the face (m|c) to cell c (c is the right-hand cell)
+template <typename T, bool VEL>
+__device__ __forceinline__ void face_right(T ul, T kl, T kr, T vl, T vr, T abv, T d, T &dg, T &of)
+{
+    const T zero = T(0);
+    T kh = T(0.5) * (kl + kr);
+    T dr = zero + (kh / d) * abv;
+    T orr = zero - ul * (kh / d) * abv;
+    if (VEL) {
+        T vh = T(0.5) * (vl + vr);
+        if (vh > zero) {
+            orr = orr - vh * ul * abv;
+        } else {
+            dr = dr - vh * abv;
+        }
+    }
+    dg = dg + dr;
+    of = of + orr;
+}
+
+// contribution of the face (c|p) to cell c (c is the left-hand cell)
+template <typename T, bool VEL>
+__device__ __forceinline__ void face_left(T ur, T kl, T kr, T vl, T vr, T abv, T d, T &dg, T &of)
+{
+    const T zero = T(0);
+    T kh = T(0.5) * (kl + kr);
+    T dl = zero + (kh / d) * abv;
+    T ol = zero - ur * (kh / d) * abv;
+    if (VEL) {
+        T vh = T(0.5) * (vl + vr);
+        if (vh > zero) {
+            dl = dl + vh * abv;
+        } else {
+            ol = ol + vh * ur * abv;
+        }
+    }
+    dg = dg + dl;
+    of = of + ol;
+}
+
+// boundary row terms of find_AX (discretepde.f90:798-865 and the y/z
+// copies): 'lo' is the x-/y-/z- face, 'hi' the + face.
+template <typename T, int OP, bool VEL>
+__device__ __forceinline__ void bc_row(int code, bool hi, T Dc, T vb, T h, T abv, T idist, T &dg)
+{
+    if (code == BC_DIRC) {
+        if (OP == OP_REF) {
+            dg = dg + T(2) * Dc / (h * h);
+            if (VEL) {
+                if (hi ? (vb > T(0)) : (vb < T(0))) dg = hi ? dg + vb / h : dg - vb / h;
+            }
+        } else {
+            dg = dg + Dc * idist * abv;
+            if (VEL) {
+                if (hi ? (vb > T(0)) : (vb < T(0))) dg = hi ? dg + vb * abv : dg - vb * abv;
+            }
+        }
+    } else if (code == BC_ZGRD) {
+        if (VEL) {
+            if (OP == OP_REF) dg = hi ? dg + vb / h : dg - vb / h;
+            else dg = hi ? dg + vb * abv : dg - vb * abv;
+        }
+    }
+}
+
+// diag and offdiag of cell (i,j,k) at address c for the current u.
+template <typename T, int OP, bool VEL, bool REAC>
+__device__ __forceinline__ void row(const Lev<T> &L, const T *__restrict__ u, int i, int j, int k,
+                                    long long c, T &dg, T &of)
+{
+    if (OP == OP_STENCIL) {
+        const long long sy = L.sy, sz = L.sz;
+        T o = T(0);
+        o = o + L.st[ST_XM][c] * u[c - 1];
+        o = o + L.st[ST_XP][c] * u[c + 1];
+        o = o + L.st[ST_YM][c] * u[c - sy];
+        o = o + L.st[ST_YP][c] * u[c + sy];
+        o = o + L.st[ST_ZM][c] * u[c - sz];
+        o = o + L.st[ST_ZP][c] * u[c + sz];
+        dg = L.st[ST_C][c];
+        of = o;
+        return;
+    }
+    const T *__restrict__ D = L.D;
+    T d = T(0), o = T(0);
+    const T Dc = D[c];
+    // ---- x ----
+    {
+        const bool edge = (i == 1) || (i == L.nx);
+        const T abv = (OP == OP_ALIGNED && edge) ? L.axb : L.ax;
+        const T vc = VEL ? L.vx[c] : T(0);
+        if (i > 1) face_right<T, VEL>(u[c - 1], D[c - 1], Dc, VEL ? L.vx[c - 1] : T(0), vc, abv, L.dx, d, o);
+        if (i < L.nx) face_left<T, VEL>(u[c + 1], Dc, D[c + 1], vc, VEL ? L.vx[c + 1] : T(0), abv, L.dx, d, o);
+        if (i == 1) bc_row<T, OP, VEL>(L.bc[LEFT], false, Dc, vc, L.dx, abv, L.ix0, d);
+        if (i == L.nx) bc_row<T, OP, VEL>(L.bc[RIGHT], true, Dc, vc, L.dx, abv, L.ix0, d);
+    }
+    // ---- y ----
+    {
+        const long long s = L.sy;
+        const bool edge = (j == 1) || (j == L.ny);
+        const T abv = (OP == OP_ALIGNED && edge) ? L.ayb : L.ay;
+        const T vc = VEL ? L.vy[c] : T(0);
+        if (j > 1) face_right<T, VEL>(u[c - s], D[c - s], Dc, VEL ? L.vy[c - s] : T(0), vc, abv, L.dy, d, o);
+        if (j < L.ny) face_left<T, VEL>(u[c + s], Dc, D[c + s], vc, VEL ? L.vy[c + s] : T(0), abv, L.dy, d, o);
+        if (j == 1) bc_row<T, OP, VEL>(L.bc[BOTTOM], false, Dc, vc, L.dy, abv, L.iy0, d);
+        if (j == L.ny) bc_row<T, OP, VEL>(L.bc[TOP], true, Dc, vc, L.dy, abv, L.iy0, d);
+    }
+    // ---- z ----
+    {
+        const long long s = L.sz;
+        const bool edge = (k == 1) || (k == L.nz);
+        const T abv = (OP == OP_ALIGNED && edge) ? L.azb : L.az;
+        const T vc = VEL ? L.vz[c] : T(0);
+        if (k > 1) face_right<T, VEL>(u[c - s], D[c - s], Dc, VEL ? L.vz[c - s] : T(0), vc, abv, L.dz, d, o);
+        if (k < L.nz) face_left<T, VEL>(u[c + s], Dc, D[c + s], vc, VEL ? L.vz[c + s] : T(0), abv, L.dz, d, o);
+        if (k == 1) bc_row<T, OP, VEL>(L.bc[BACK], false, Dc, vc, L.dz, abv, L.iz0, d);
+        if (k == L.nz) bc_row<T, OP, VEL>(L.bc[FRONT], true, Dc, vc, L.dz, abv, L.iz0, d);
+    }
+    if (REAC) d = d - L.reac[c];
+    if (L.timederiv) d = d + T(1) / L.dt;
+    dg = d;
+    of = o;
+}
+
+// The seven row coefficients (centre and the six neighbours) of an
+// OP_REF / OP_ALIGNED row, for building explicit (e.g. transposed) stencils.
+template <typename T, int OP, bool VEL, bool REAC>
+__device__ void row_coeffs(const Lev<T> &L, int i, int j, int k, long long c, T a[ST_N])
+{
+    // the off-diagonal sum is linear in the neighbour values: probe it with
+    // unit values, one neighbour at a time, through the same face formulas
+    T dg, of;
+    const T *D = L.D;
+    const T Dc = D[c];
+    for (int q = 0; q < ST_N; ++q) a[q] = T(0);
+    T d = T(0), o;
+    {
+        const bool edge = (i == 1) || (i == L.nx);
+        const T abv = (OP == OP_ALIGNED && edge) ? L.axb : L.ax;
+        const T vc = VEL ? L.vx[c] : T(0);
+        if (i > 1) { o = T(0); face_right<T, VEL>(T(1), D[c - 1], Dc, VEL ? L.vx[c - 1] : T(0), vc, abv, L.dx, d, o); a[ST_XM] = o; }
+        if (i < L.nx) { o = T(0); face_left<T, VEL>(T(1), Dc, D[c + 1], vc, VEL ? L.vx[c + 1] : T(0), abv, L.dx, d, o); a[ST_XP] = o; }
+        if (i == 1) bc_row<T, OP, VEL>(L.bc[LEFT], false, Dc, vc, L.dx, abv, L.ix0, d);
+        if (i == L.nx) bc_row<T, OP, VEL>(L.bc[RIGHT], true, Dc, vc, L.dx, abv, L.ix0, d);
+    }
+    {
+        const long long s = L.sy;
+        const bool edge = (j == 1) || (j == L.ny);
+        const T abv = (OP == OP_ALIGNED && edge) ? L.ayb : L.ay;
+        const T vc = VEL ? L.vy[c] : T(0);
+        if (j > 1) { o = T(0); face_right<T, VEL>(T(1), D[c - s], Dc, VEL ? L.vy[c - s] : T(0), vc, abv, L.dy, d, o); a[ST_YM] = o; }
+        if (j < L.ny) { o = T(0); face_left<T, VEL>(T(1), Dc, D[c + s], vc, VEL ? L.vy[c + s] : T(0), abv, L.dy, d, o); a[ST_YP] = o; }
+        if (j == 1) bc_row<T, OP, VEL>(L.bc[BOTTOM], false, Dc, vc, L.dy, abv, L.iy0, d);
+        if (j == L.ny) bc_row<T, OP, VEL>(L.bc[TOP], true, Dc, vc, L.dy, abv, L.iy0, d);
+    }
+    {
+        const long long s = L.sz;
+        const bool edge = (k == 1) || (k == L.nz);
+        const T abv = (OP == OP_ALIGNED && edge) ? L.azb : L.az;
+        const T vc = VEL ? L.vz[c] : T(0);
+        if (k > 1) { o = T(0); face_right<T, VEL>(T(1), D[c - s], Dc, VEL ? L.vz[c - s] : T(0), vc, abv, L.dz, d, o); a[ST_ZM] = o; }
+        if (k < L.nz) { o = T(0); face_left<T, VEL>(T(1), Dc, D[c + s], vc, VEL ? L.vz[c + s] : T(0), abv, L.dz, d, o); a[ST_ZP] = o; }
+        if (k == 1) bc_row<T, OP, VEL>(L.bc[BACK], false, Dc, vc, L.dz, abv, L.iz0, d);
+        if (k == L.nz) bc_row<T, OP, VEL>(L.bc[FRONT], true, Dc, vc, L.dz, abv, L.iz0, d);
+    }
+    if (REAC) d = d - L.reac[c];
+    if (L.timederiv) d = d + T(1) / L.dt;
+    a[ST_C] = d;
+    (void)dg;
+    (void)of;
+}
+
+// ------------------------------------------------------------------------
+// block reduction (deterministic: fixed tree inside a block)
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double red[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+    }
+    return s;  // valid on thread 0
+}
+
+// ------------------------------------------------------------------------
+// kernels: operator, smoother, residual
+// ------------------------------------------------------------------------
+
+// find_AX: AX = diag*u + off (interior arrays, each nullable)
+template <typename T, int OP, bool VEL, bool REAC>
+__global__ void __launch_bounds__(kBlock) k_apply(Lev<T> L, const T *__restrict__ u, T *ax, T *diag, T *off)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    T dg, of;
+    row<T, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
+    long long n = (long long)(i - 1) + (long long)L.nx * ((j - 1) + (long long)L.ny * (k - 1));
+    if (ax) ax[n] = dg * u[c] + of;
+    if (diag) diag[n] = dg;
+    if (off) off[n] = of;
+}
+
+// one red (red=1) or black (red=0) pass: u = (b - off)/diag on one colour;
+// red = odd (i+j+k), the reference's colouring (discretepde.f90:49-57)
+template <typename T, int OP, bool VEL, bool REAC>
+__global__ void __launch_bounds__(kBlock) k_colour(Lev<T> L, T *__restrict__ u, const T *__restrict__ b, int red)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    if (((i + j + k) & 1) != red) return;
+    T dg, of;
+    row<T, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
+    u[c] = (b[c] - of) / dg;
+}
+
+// r = b - (diag*u + off); writes r (type R, ghosted layout, nullable) and the
+// block's partial sum of r^2 (nullable)
+template <typename T, typename R, int OP, bool VEL, bool REAC>
+__global__ void __launch_bounds__(kBlock) k_residual(Lev<T> L, const T *__restrict__ u, const T *__restrict__ b,
+                                                    R *r, double *partials)
+{
+    int i, j, k;
+    long long c;
+    double sq = 0.0;
+    if (cell_of(L, i, j, k, c)) {
+        T dg, of;
+        row<T, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
+        T rv = b[c] - (dg * u[c] + of);
+        if (r) r[c] = (R)rv;
+        sq = (double)rv * (double)rv;
+    }
+    if (partials) {
+        double s = block_sum(sq);
+        if (threadIdx.x == 0) partials[blockIdx.x + (long long)gridDim.x * blockIdx.y] = s;
+    }
+}
+
+// sums n partials in a fixed order with one block; out[0] = sum
+__global__ void __launch_bounds__(kBlock) k_reduce_final(const double *partials, int n, double *out)
+{
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) s += partials[q];
+    s = block_sum(s);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// ------------------------------------------------------------------------
+// right-hand side (find_bvec) on the finest level, FP64
+// ------------------------------------------------------------------------
+struct Faces {
+    const double *f[6];
+};
+
+__global__ void __launch_bounds__(kBlock) k_bvec(Lev<double> L, Faces F, const double *sterm /* interior, nullable */,
+                                                const double *source /* ghosted, nullable */, int applybc, double *b)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    double v = 0.0;
+    const double Dc = L.D[c];
+    if (applybc) {
+        const double vx = L.vx ? L.vx[c] : 0.0, vy = L.vy ? L.vy[c] : 0.0, vz = L.vz ? L.vz[c] : 0.0;
+        const double dx = L.dx, dy = L.dy, dz = L.dz;
+        if (i == 1) {
+            double g = F.f[LEFT][(j - 1) + (long long)L.ny * (k - 1)];
+            if (L.bc[LEFT] == BC_DIRC) {
+                v = v + 2.0 * Dc * g / (dx * dx);
+                if (vx > 0.0) v = v + vx * g / dx;
+            } else if (L.bc[LEFT] == BC_FLUX) {
+                v = v - g / dx;
+            }
+        }
+        if (i == L.nx) {
+            double g = F.f[RIGHT][(j - 1) + (long long)L.ny * (k - 1)];
+            if (L.bc[RIGHT] == BC_DIRC) {
+                v = v + 2.0 * Dc * g / (dx * dx);
+                if (vx < 0.0) v = v - vx * g / dx;
+            } else if (L.bc[RIGHT] == BC_FLUX) {
+                v = v - g / dx;
+            }
+        }
+        if (j == 1) {
+            double g = F.f[BOTTOM][(i - 1) + (long long)L.nx * (k - 1)];
+            if (L.bc[BOTTOM] == BC_DIRC) {
+                v = v + 2.0 * Dc * g / (dy * dy);
+                if (vy > 0.0) v = v + vy * g / dy;
+            } else if (L.bc[BOTTOM] == BC_FLUX) {
+                v = v - g / dy;
+            }
+        }
+        if (j == L.ny) {
+            double g = F.f[TOP][(i - 1) + (long long)L.nx * (k - 1)];
+            if (L.bc[TOP] == BC_DIRC) {
+                v = v + 2.0 * Dc * g / (dy * dy);
+                if (vy < 0.0) v = v - vy * g / dy;
+            } else if (L.bc[TOP] == BC_FLUX) {
+                v = v - g / dy;
+            }
+        }
+        if (k == 1) {
+            double g = F.f[BACK][(i - 1) + (long long)L.nx * (j - 1)];
+            if (L.bc[BACK] == BC_DIRC) {
+                v = v + 2.0 * Dc * g / (dz * dz);
+                if (vz > 0.0) v = v + vz * g / dz;
+            } else if (L.bc[BACK] == BC_FLUX) {
+                v = v - g / dz;
+            }
+        }
+        if (k == L.nz) {
+            double g = F.f[FRONT][(i - 1) + (long long)L.nx * (j - 1)];
+            if (L.bc[FRONT] == BC_DIRC) {
+                v = v + 2.0 * Dc * g / (dz * dz);
+                if (vz < 0.0) v = v - vz * g / dz;
+            } else if (L.bc[FRONT] == BC_FLUX) {
+                v = v - g / dz;
+            }
+        }
+    }
+    long long n = (long long)(i - 1) + (long long)L.nx * ((j - 1) + (long long)L.ny * (k - 1));
+    v = v + (sterm ? sterm[n] : 0.0);
+    if (applybc && source) v = v + source[c];
+    b[c] = v;
+}
+
+// ------------------------------------------------------------------------
+// reference (FAS) transfers
+// ------------------------------------------------------------------------
+
+// restriction (mgridsteps.f90:11-53) of u and r: coarse cell sums its eight
+// children in the reference order. Writes uold and u_c (= R u) and b_c
+// (= R r; tau is added by k_tau).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_restrict_fas(Lev<T> F, Lev<T> C, const T *__restrict__ uf,
+                                                        const T *__restrict__ rf, T *uold, T *uc, T *bc)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(C, i, j, k, c)) return;
+    const long long sy = F.sy, sz = F.sz;
+    const long long f0 = (long long)(2 * i - 1) + sy * (2 * j - 1) + sz * (2 * k - 1);
+    const long long o[8] = {0, 1, sy, sy + 1, sz, sz + 1, sz + sy, sz + sy + 1};
+    T s = T(0), q = T(0);
+    s = s + uf[f0 + o[0]] + uf[f0 + o[1]];
+    s = s + uf[f0 + o[2]] + uf[f0 + o[3]];
+    s = s + uf[f0 + o[4]] + uf[f0 + o[5]];
+    s = s + uf[f0 + o[6]] + uf[f0 + o[7]];
+    q = q + rf[f0 + o[0]] + rf[f0 + o[1]];
+    q = q + rf[f0 + o[2]] + rf[f0 + o[3]];
+    q = q + rf[f0 + o[4]] + rf[f0 + o[5]];
+    q = q + rf[f0 + o[6]] + rf[f0 + o[7]];
+    const T eighth = T(1.0 / 8.0);
+    s = eighth * s;
+    uold[c] = s;
+    uc[c] = s;
+    bc[c] = eighth * q;
+}
+
+// 8-child mean of a coefficient array (same rule; once per solve)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_restrict_mean(Lev<T> F, Lev<T> C, const T *__restrict__ f, T *cf)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(C, i, j, k, c)) return;
+    const long long sy = F.sy, sz = F.sz;
+    const long long f0 = (long long)(2 * i - 1) + sy * (2 * j - 1) + sz * (2 * k - 1);
+    T s = T(0);
+    s = s + f[f0] + f[f0 + 1];
+    s = s + f[f0 + sy] + f[f0 + sy + 1];
+    s = s + f[f0 + sz] + f[f0 + sz + 1];
+    s = s + f[f0 + sz + sy] + f[f0 + sz + sy + 1];
+    cf[c] = T(1.0 / 8.0) * s;
+}
+
+// FAS right-hand side on the coarse level: b_c = A_c(R u) + R r
+// (mgridsteps.f90:544-550); b_c holds R r on entry.
+template <typename T, int OP, bool VEL, bool REAC>
+__global__ void __launch_bounds__(kBlock) k_tau(Lev<T> C, const T *__restrict__ uold, T *bc)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(C, i, j, k, c)) return;
+    T dg, of;
+    row<T, OP, VEL, REAC>(C, uold, i, j, k, c, dg, of);
+    bc[c] = (dg * uold[c] + of) + bc[c];
+}
+
+template <typename T>
+struct FacesT {
+    const T *f[6];  // coarse BC faces for the ghost rule (nullable = 0)
+};
+
+// value of error_2h = unew - uold at a coarse index that may be a ghost,
+// with update_boundary_ghostvalues applied to both (discretepde.f90:1144-1263)
+template <typename T>
+__device__ __forceinline__ T err_at(const Lev<T> &C, const T *unew, const T *uold, const FacesT<T> &G, int i, int j,
+                                    int k)
+{
+    int ii = i, jj = j, kk = k, face = -1;
+    long long fidx = 0;
+    if (i == 0) { ii = 1; face = LEFT; fidx = (j - 1) + (long long)C.ny * (k - 1); }
+    else if (i == C.nx + 1) { ii = C.nx; face = RIGHT; fidx = (j - 1) + (long long)C.ny * (k - 1); }
+    else if (j == 0) { jj = 1; face = BOTTOM; fidx = (i - 1) + (long long)C.nx * (k - 1); }
+    else if (j == C.ny + 1) { jj = C.ny; face = TOP; fidx = (i - 1) + (long long)C.nx * (k - 1); }
+    else if (k == 0) { kk = 1; face = BACK; fidx = (i - 1) + (long long)C.nx * (j - 1); }
+    else if (k == C.nz + 1) { kk = C.nz; face = FRONT; fidx = (i - 1) + (long long)C.nx * (j - 1); }
+    const long long c = (long long)ii + (long long)C.sy * jj + C.sz * kk;
+    if (face < 0) return unew[c] - uold[c];
+    if (C.bc[face] == BC_DIRC) {
+        const T g2 = G.f[face] ? T(2) * G.f[face][fidx] : T(0);
+        return (g2 - unew[c]) - (g2 - uold[c]);
+    }
+    return unew[c] - uold[c];
+}
+
+// prolongation1 + correction (mgridsteps.f90:170-216, :562-567)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_prolong1_correct(Lev<T> F, Lev<T> C, T *uf, const T *__restrict__ unew,
+                                                            const T *__restrict__ uold, FacesT<T> G)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(C, i, j, k, c)) return;
+    const T e = unew[c] - uold[c];
+    const T sx = T(0.5) * (err_at(C, unew, uold, G, i + 1, j, k) - err_at(C, unew, uold, G, i - 1, j, k));
+    const T sy_ = T(0.5) * (err_at(C, unew, uold, G, i, j + 1, k) - err_at(C, unew, uold, G, i, j - 1, k));
+    const T sz_ = T(0.5) * (err_at(C, unew, uold, G, i, j, k + 1) - err_at(C, unew, uold, G, i, j, k - 1));
+    const long long sy = F.sy, sz = F.sz;
+    const long long f0 = (long long)(2 * i - 1) + sy * (2 * j - 1) + sz * (2 * k - 1);
+    const T q = T(0.25);
+    uf[f0] = uf[f0] + (e + q * (-sx - sy_ - sz_));
+    uf[f0 + 1] = uf[f0 + 1] + (e + q * (+sx - sy_ - sz_));
+    uf[f0 + sy] = uf[f0 + sy] + (e + q * (-sx + sy_ - sz_));
+    uf[f0 + sy + 1] = uf[f0 + sy + 1] + (e + q * (+sx + sy_ - sz_));
+    uf[f0 + sz] = uf[f0 + sz] + (e + q * (-sx - sy_ + sz_));
+    uf[f0 + sz + 1] = uf[f0 + sz + 1] + (e + q * (+sx - sy_ + sz_));
+    uf[f0 + sz + sy] = uf[f0 + sz + sy] + (e + q * (-sx + sy_ + sz_));
+    uf[f0 + sz + sy + 1] = uf[f0 + sz + sy + 1] + (e + q * (+sx + sy_ + sz_));
+}
+
+// ------------------------------------------------------------------------
+// aligned (m*2^k+1) transfers
+// ------------------------------------------------------------------------
+
+// volume-weighted full weighting: coarse I draws on fine 2I-2, 2I-1, 2I
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_restrict_fw(Lev<T> F, Lev<T> C, AlignedW W, const T *__restrict__ f,
+                                                       T *cf)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(C, i, j, k, c)) return;
+    double acc = 0.0;
+    for (int dk = -1; dk <= 1; ++dk) {
+        const int fk = 2 * k - 1 + dk;
+        const double wk = W.wz[3 * (k - 1) + dk + 1];
+        if (wk == 0.0) continue;
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int fj = 2 * j - 1 + dj;
+            const double wjk = wk * W.wy[3 * (j - 1) + dj + 1];
+            if (wjk == 0.0) continue;
+            const long long base = (long long)F.sy * fj + F.sz * fk;
+            double row = 0.0;
+            for (int di = -1; di <= 1; ++di) {
+                const double wi = W.wx[3 * (i - 1) + di + 1];
+                if (wi != 0.0) row += wi * (double)f[base + 2 * i - 1 + di];
+            }
+            acc += wjk * row;
+        }
+    }
+    cf[c] = (T)acc;
+}
+
+// trilinear prolongation + correction on the aligned hierarchy: fine cell f
+// takes coarse (f+1)/2 when f is odd, the mean of f/2 and f/2+1 when even
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_prolong_tri_correct(Lev<T> F, Lev<T> C, T *uf, const T *__restrict__ uc)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(F, i, j, k, c)) return;
+    const int ia = (i + 1) >> 1, ja = (j + 1) >> 1, ka = (k + 1) >> 1;
+    const int ib = (i & 1) ? ia : ia + 1, jb = (j & 1) ? ja : ja + 1, kb = (k & 1) ? ka : ka + 1;
+    const long long sy = C.sy, sz = C.sz;
+    auto at = [&](int a, int b_, int d) { return uc[(long long)a + sy * b_ + sz * d]; };
+    T e;
+    if (i & 1) {
+        if (j & 1) {
+            e = (k & 1) ? at(ia, ja, ka) : T(0.5) * (at(ia, ja, ka) + at(ia, ja, kb));
+        } else {
+            T a0 = T(0.5) * (at(ia, ja, ka) + at(ia, jb, ka));
+            e = (k & 1) ? a0 : T(0.5) * (a0 + T(0.5) * (at(ia, ja, kb) + at(ia, jb, kb)));
+        }
+    } else {
+        T r00 = T(0.5) * (at(ia, ja, ka) + at(ib, ja, ka));
+        if (j & 1) {
+            e = (k & 1) ? r00 : T(0.5) * (r00 + T(0.5) * (at(ia, ja, kb) + at(ib, ja, kb)));
+        } else {
+            T r10 = T(0.5) * (at(ia, jb, ka) + at(ib, jb, ka));
+            T p0 = T(0.5) * (r00 + r10);
+            if (k & 1) {
+                e = p0;
+            } else {
+                T r01 = T(0.5) * (at(ia, ja, kb) + at(ib, ja, kb));
+                T r11 = T(0.5) * (at(ia, jb, kb) + at(ib, jb, kb));
+                e = T(0.5) * (p0 + T(0.5) * (r01 + r11));
+            }
+        }
+    }
+    uf[c] = uf[c] + e;
+}
+
+// ------------------------------------------------------------------------
+// element-wise helpers (graph-safe replacements for memset/copies)
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fill(T *a, long long n, T v)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) a[t] = v;
+}
+
+template <typename Ti, typename To>
+__global__ void __launch_bounds__(kBlock) k_convert(const Ti *in, To *out, long long n)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) out[t] = (To)in[t];
+}
+
+// interior correction u += e (defect correction: FP64 u, FP32 e)
+template <typename T, typename E>
+__global__ void __launch_bounds__(kBlock) k_add_interior(Lev<T> L, T *u, const E *e)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    u[c] = u[c] + (T)e[c];
+}
+
+// interior copy between a caller's interior array (nx,ny,nz) and the
+// ghosted layout, either direction
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_interior_in(Lev<T> L, const T *src, T *dst_ghosted)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    long long n = (long long)(i - 1) + (long long)L.nx * ((j - 1) + (long long)L.ny * (k - 1));
+    dst_ghosted[c] = src[n];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_interior_out(Lev<T> L, const T *src_ghosted, T *dst)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    long long n = (long long)(i - 1) + (long long)L.nx * ((j - 1) + (long long)L.ny * (k - 1));
+    dst[n] = src_ghosted[c];
+}
+
+// update_boundary_ghostvalues (discretepde.f90:1121-1267) on one level;
+// one thread per face element, faces in the reference order
+__global__ void __launch_bounds__(kBlock) k_ghosts(Lev<double> L, double *u, Faces G)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nyz = (long long)L.ny * L.nz, nxz = (long long)L.nx * L.nz, nxy = (long long)L.nx * L.ny;
+    const long long sy = L.sy, sz = L.sz;
+    if (t < nyz) {
+        int j = (int)(t % L.ny) + 1, k = (int)(t / L.ny) + 1;
+        long long c0 = sy * j + sz * k;
+        double g = G.f[LEFT] ? G.f[LEFT][t] : 0.0;
+        u[c0] = L.bc[LEFT] == BC_DIRC ? 2.0 * g - u[c0 + 1] : u[c0 + 1];
+        g = G.f[RIGHT] ? G.f[RIGHT][t] : 0.0;
+        u[c0 + L.nx + 1] = L.bc[RIGHT] == BC_DIRC ? 2.0 * g - u[c0 + L.nx] : u[c0 + L.nx];
+        return;
+    }
+    t -= nyz;
+    if (t < nxz) {
+        int i = (int)(t % L.nx) + 1, k = (int)(t / L.nx) + 1;
+        long long c0 = i + sz * k;
+        double g = G.f[BOTTOM] ? G.f[BOTTOM][t] : 0.0;
+        u[c0] = L.bc[BOTTOM] == BC_DIRC ? 2.0 * g - u[c0 + sy] : u[c0 + sy];
+        g = G.f[TOP] ? G.f[TOP][t] : 0.0;
+        u[c0 + sy * (L.ny + 1)] = L.bc[TOP] == BC_DIRC ? 2.0 * g - u[c0 + sy * L.ny] : u[c0 + sy * L.ny];
+        return;
+    }
+    t -= nxz;
+    if (t < nxy) {
+        int i = (int)(t % L.nx) + 1, j = (int)(t / L.nx) + 1;
+        long long c0 = i + sy * j;
+        double g = G.f[BACK] ? G.f[BACK][t] : 0.0;
+        u[c0] = L.bc[BACK] == BC_DIRC ? 2.0 * g - u[c0 + sz] : u[c0 + sz];
+        g = G.f[FRONT] ? G.f[FRONT][t] : 0.0;
+        u[c0 + sz * (L.nz + 1)] = L.bc[FRONT] == BC_DIRC ? 2.0 * g - u[c0 + sz * L.nz] : u[c0 + sz * L.nz];
+    }
+}
+
+// explicit stencil of an OP_REF / OP_ALIGNED operator; transpose != 0 writes
+// the transposed rows (A^T: the coefficient of u(i-1) in row i becomes the
+// coefficient of u(i+1) in row i-1, and so on)
+template <typename T, int OP, bool VEL, bool REAC>
+__global__ void __launch_bounds__(kBlock) k_build_stencil(Lev<T> L, T *st0, T *st1, T *st2, T *st3, T *st4, T *st5,
+                                                         T *st6, int transpose)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    T *st[ST_N] = {st0, st1, st2, st3, st4, st5, st6};
+    T a[ST_N];
+    row_coeffs<T, OP, VEL, REAC>(L, i, j, k, c, a);
+    if (!transpose) {
+        for (int q = 0; q < ST_N; ++q) st[q][c] = a[q];
+        return;
+    }
+    // row c of A couples to c-1 with a[XM]: in A^T that is row c-1's
+    // coefficient of c, i.e. slot XP of row c-1, and so on
+    st[ST_C][c] = a[ST_C];
+    if (i > 1) st[ST_XP][c - 1] = a[ST_XM];
+    if (i < L.nx) st[ST_XM][c + 1] = a[ST_XP];
+    if (j > 1) st[ST_YP][c - L.sy] = a[ST_YM];
+    if (j < L.ny) st[ST_YM][c + L.sy] = a[ST_YP];
+    if (k > 1) st[ST_ZP][c - L.sz] = a[ST_ZM];
+    if (k < L.nz) st[ST_ZM][c + L.sz] = a[ST_ZP];
+}
+
+}  // namespace mps
+
+namespace mps {
+
+// ------------------------------------------------------------------------
+// Krylov vector kernels (FP64): fused multi-dot and multi-axpy over the
+// interior of ghosted-layout vectors, with block partial sums reduced in a
+// fixed order by k_reduce_cols.
+// ------------------------------------------------------------------------
+constexpr int kMaxVec = 64;
+
+struct VecList {
+    const double *v[kMaxVec];
+};
+
+// partials[q*nblocks + block] = sum over the block's cells of w*v_q
+__global__ void __launch_bounds__(kBlock) k_multidot(Lev<double> L, const double *__restrict__ w, VecList V, int nv,
+                                                    double *partials)
+{
+    int i, j, k;
+    long long c;
+    const bool ok = cell_of(L, i, j, k, c);
+    const double wc = ok ? w[c] : 0.0;
+    const int nblocks = gridDim.x * gridDim.y;
+    const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+    for (int q = 0; q < nv; ++q) {
+        double s = block_sum(ok ? wc * V.v[q][c] : 0.0);
+        if (threadIdx.x == 0) partials[(long long)q * nblocks + bid] = s;
+        __syncthreads();
+    }
+}
+
+// out[q] = sum_b partials[q*nblocks + b]; one block per column
+__global__ void __launch_bounds__(kBlock) k_reduce_cols(const double *partials, int nblocks, double *out)
+{
+    const int q = blockIdx.x;
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nblocks; t += blockDim.x) s += partials[(long long)q * nblocks + t];
+    s = block_sum(s);
+    if (threadIdx.x == 0) out[q] = s;
+}
+
+// w = w - sum_q h[q] v_q over the interior; block partial of |w|^2
+__global__ void __launch_bounds__(kBlock) k_multiaxpy(Lev<double> L, double *w, VecList V, const double *h, int nv,
+                                                     double *partials)
+{
+    int i, j, k;
+    long long c;
+    double sq = 0.0;
+    if (cell_of(L, i, j, k, c)) {
+        double x = w[c];
+        for (int q = 0; q < nv; ++q) x = x - h[q] * V.v[q][c];
+        w[c] = x;
+        sq = x * x;
+    }
+    if (partials) {
+        double s = block_sum(sq);
+        if (threadIdx.x == 0) partials[blockIdx.x + (long long)gridDim.x * blockIdx.y] = s;
+    }
+}
+
+// out = A u over the interior (ghosted output)
+template <typename T, int OP, bool VEL, bool REAC>
+__global__ void __launch_bounds__(kBlock) k_matvec(Lev<T> L, const T *__restrict__ u, T *out)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    T dg, of;
+    row<T, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
+    out[c] = dg * u[c] + of;
+}
+
+// v = w * (1/s[0]) over the interior (ghosts of v stay zero)
+__global__ void __launch_bounds__(kBlock) k_scale(Lev<double> L, const double *w, double *v, const double *s)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    v[c] = w[c] / s[0];
+}
+
+}  // namespace mps

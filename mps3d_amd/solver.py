@@ -1,0 +1,273 @@
+"""Host-side mirror of the reference multigrid call surface over the C ABI.
+
+``FieldSolver`` keeps the names and argument meaning of the reference
+routines it stands in for:
+
+=====================  ====================================================
+``perform_vcycle``      mgridlib/mgridsteps.f90:384 (perform_vcycle)
+``solve``               the equation modules' V-cycle loop
+                        (plasma_pde/poissoneq.f90:427-450,
+                        generic_pde/pdedefs.f90:378-402)
+``find_AX``             mgridlib/discretepde.f90:672 (find_AX)
+``find_bvec``           mgridlib/discretepde.f90:382 (find_bvec)
+``smooth``              mgridlib/discretepde.f90:132 (perform_gseidel_smoothing)
+``fill_ghosts``         mgridlib/discretepde.f90:1121 (update_boundary_ghostvalues)
+``gmres``               mgridlib/gmres.f90:197 (performgmres)
+=====================  ====================================================
+
+Arrays are the reference's: ghosted Fortran-order float64 arrays of shape
+(nx+2, ny+2, nz+2) for solutions and coefficients, interior (nx, ny, nz)
+arrays for mgridsterm and operator outputs. A numpy array is host memory;
+a torch tensor on a HIP device is used in place (its memory must be laid
+out as the Fortran array, i.e. a C-contiguous tensor of shape
+(nz+2, ny+2, nx+2) -- see :func:`to_device` / :func:`from_device`).
+
+Every compute call goes to the HIP library; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as M
+
+_NAMES = {"ref": M.HIER_REF, "aligned": M.HIER_ALIGNED}
+_PREC = {"fp64": M.PREC_FP64, "fp32dc": M.PREC_FP32_DC, "fp32_dc": M.PREC_FP32_DC}
+_FLUX = {"upwind": M.FLUX_UPWIND, "sg": M.FLUX_SG}
+_TOL = {"rms": M.TOL_RMS, "rel_l2": M.TOL_REL_L2, "rel": M.TOL_REL_L2}
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def ghosted(interior):
+    """Zero-ghost Fortran array of shape (nx+2, ny+2, nz+2) around an interior array."""
+    a = np.asarray(interior, dtype=np.float64)
+    g = np.zeros((a.shape[0] + 2, a.shape[1] + 2, a.shape[2] + 2), order="F")
+    g[1:-1, 1:-1, 1:-1] = a
+    return g
+
+
+def to_device(a, device="cuda"):
+    """Fortran-order numpy array -> C-contiguous torch tensor with the same memory layout."""
+    import torch
+    a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+    return torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).to(device)
+
+
+def from_device(t):
+    """Inverse of :func:`to_device`."""
+    return np.asfortranarray(t.detach().cpu().numpy().transpose(2, 1, 0))
+
+
+class FieldSolver:
+    """One grid on one GPU (an ``mps_ctx``)."""
+
+    def __init__(self, n, length=(1.0, 1.0, 1.0), p=(1, 1, 1), hierarchy="ref", precision="fp64",
+                 flux="upwind", nu1=2, nu2=2, max_cycles=20, tol=1e-11, tol_kind="rms"):
+        self.lib = M.load()
+        self.n = tuple(int(v) for v in (n if not np.isscalar(n) else (n, n, n)))
+        self.length = tuple(float(v) for v in (length if not np.isscalar(length) else (length,) * 3))
+        g = M.Grid(self.n[0], self.n[1], self.n[2], self.length[0], self.length[1], self.length[2],
+                   int(p[0]), int(p[1]), int(p[2]))
+        o = M.Opts()
+        self.lib.mps_default_opts(C.byref(o))
+        o.hierarchy = _NAMES[hierarchy] if isinstance(hierarchy, str) else int(hierarchy)
+        o.precision = _PREC[precision] if isinstance(precision, str) else int(precision)
+        o.flux = _FLUX[flux] if isinstance(flux, str) else int(flux)
+        o.nu1, o.nu2, o.max_cycles = int(nu1), int(nu2), int(max_cycles)
+        o.tol = float(tol)
+        o.tol_kind = _TOL[tol_kind] if isinstance(tol_kind, str) else int(tol_kind)
+        self.opts = o
+        self.max_cycles = int(max_cycles)
+        h = C.c_void_p()
+        M.check(self.lib.mps_create(C.byref(h), C.byref(g), C.byref(o)))
+        self._h = h
+        self._keep = []
+
+    # ------------------------------------------------------------ plumbing
+    @property
+    def shape(self):
+        return tuple(v + 2 for v in self.n)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.mps_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ptr(self, a, kind, writable=False):
+        """Pointer to a caller array. kind: 'g' ghosted, 'i' interior, 'f' face."""
+        if a is None:
+            return None
+        if _is_torch(a):
+            import torch
+            if not a.is_cuda or a.dtype != torch.float64 or not a.is_contiguous():
+                raise ValueError("device arrays must be contiguous float64 tensors on the GPU")
+            want = {"g": int(np.prod(self.shape)), "i": int(np.prod(self.n))}.get(kind)
+            if want is not None and a.numel() != want:
+                raise ValueError("device array has %d elements, expected %d" % (a.numel(), want))
+            return C.c_void_p(a.data_ptr())
+        arr = np.asarray(a, dtype=np.float64)
+        if kind == "g":
+            if arr.shape == self.n and not writable:
+                arr = ghosted(arr)
+            if arr.shape != self.shape:
+                raise ValueError("expected ghosted shape %s, got %s" % (self.shape, arr.shape))
+        elif kind == "i" and arr.shape != self.n:
+            raise ValueError("expected interior shape %s, got %s" % (self.n, arr.shape))
+        if writable:
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.f_contiguous):
+                raise ValueError("in/out arrays must be Fortran-ordered float64 numpy arrays (or device tensors)")
+            arr = a
+        else:
+            arr = np.asfortranarray(arr)
+        self._keep.append(arr)
+        return C.c_void_p(arr.ctypes.data)
+
+    def _release(self):
+        self._keep = []
+
+    # ------------------------------------------------------------ set-up
+    def set_bc(self, codes, faces=(0.0,) * 6):
+        """bc_codes / bcvals of perform_vcycle: codes 'DIRC','ZGRD','FLUX' (others
+        act as no-flux), faces left/right (ny,nz), bottom/top (nx,nz), back/front (nx,ny)."""
+        nx, ny, nz = self.n
+        shapes = [(ny, nz), (ny, nz), (nx, nz), (nx, nz), (nx, ny), (nx, ny)]
+        ks = (C.c_int * 6)(*[M.BC_BY_NAME.get(str(c).strip().upper(), M.BC_NONE) if isinstance(c, str) else int(c)
+                             for c in codes])
+        ptrs = (C.c_void_p * 6)()
+        for f in range(6):
+            v = faces[f]
+            if _is_torch(v):
+                ptrs[f] = self._ptr(v, "f")
+            else:
+                arr = np.asfortranarray(np.broadcast_to(np.asarray(v, dtype=np.float64), shapes[f]))
+                self._keep.append(arr)
+                ptrs[f] = arr.ctypes.data
+        try:
+            M.check(self.lib.mps_set_bc(self._h, ks, ptrs))
+        finally:
+            self._release()
+
+    def set_coeffs(self, dcoeff=None, velx=None, vely=None, velz=None, reac=None, source=None,
+                   timederiv=False, dt=1.0):
+        """Operator coefficients (ghosted or interior arrays; None = 0)."""
+        try:
+            args = [self._ptr(a, "g") for a in (dcoeff, velx, vely, velz, reac, source)]
+            M.check(self.lib.mps_set_coeffs(self._h, *args, int(bool(timederiv)), float(dt)))
+        finally:
+            self._release()
+
+    def set_case(self, case):
+        """Loads a :class:`mps3d_amd.workloads.Case`."""
+        self.set_bc(case.bc_codes, case.bc_vals)
+        self.set_coeffs(case.dcoeff, case.velx, case.vely, case.velz, case.reac, case.source,
+                        case.timederiv, case.dt)
+
+    # ------------------------------------------------------------ compute
+    def perform_vcycle(self, soln, mgridsterm=None):
+        r = C.c_double()
+        try:
+            M.check(self.lib.mps_vcycle(self._h, self._ptr(soln, "g", True), self._ptr(mgridsterm, "i"),
+                                        C.byref(r)))
+        finally:
+            self._release()
+        return r.value
+
+    def solve(self, soln, raise_on_failure=False):
+        """V-cycle loop to the configured tolerance. Returns (ncycles, resnorm, history, status)."""
+        n = C.c_int()
+        r = C.c_double()
+        hist = np.zeros(self.max_cycles)
+        try:
+            st = self.lib.mps_solve(self._h, self._ptr(soln, "g", True), C.byref(n), C.byref(r),
+                                    C.c_void_p(hist.ctypes.data))
+        finally:
+            self._release()
+        M.check(st, allow=(M.OK,) if raise_on_failure else (M.OK, M.NOT_CONVERGED))
+        return n.value, r.value, hist[:n.value].copy(), st
+
+    def last_timing(self):
+        ms = C.c_double()
+        n = C.c_int()
+        M.check(self.lib.mps_last_timing(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def find_AX(self, x):
+        ax, dg, of = (np.zeros(self.n, order="F") for _ in range(3))
+        try:
+            M.check(self.lib.mps_apply(self._h, self._ptr(x, "g"), self._ptr(ax, "i", True),
+                                       self._ptr(dg, "i", True), self._ptr(of, "i", True)))
+        finally:
+            self._release()
+        return ax, dg, of
+
+    def apply_transpose(self, x):
+        out = np.zeros(self.n, order="F")
+        try:
+            M.check(self.lib.mps_apply_transpose(self._h, self._ptr(x, "g"), self._ptr(out, "i", True)))
+        finally:
+            self._release()
+        return out
+
+    def find_bvec(self, mgridsterm=None, applybc=True):
+        b = np.zeros(self.n, order="F")
+        try:
+            M.check(self.lib.mps_bvec(self._h, self._ptr(mgridsterm, "i"), int(bool(applybc)),
+                                      self._ptr(b, "i", True)))
+        finally:
+            self._release()
+        return b
+
+    def smooth(self, soln, niter=2, mgridsterm=None):
+        res = np.zeros(self.n, order="F")
+        try:
+            M.check(self.lib.mps_smooth(self._h, self._ptr(soln, "g", True), self._ptr(mgridsterm, "i"),
+                                        int(niter), self._ptr(res, "i", True)))
+        finally:
+            self._release()
+        return res
+
+    def fill_ghosts(self, soln):
+        try:
+            M.check(self.lib.mps_fill_ghosts(self._h, self._ptr(soln, "g", True)))
+        finally:
+            self._release()
+
+    def gmres(self, b, x, m=30, restarts=20, tol=1e-10, precond=1, raise_on_failure=False):
+        """precond 0: the reference's performgmres (left, one RB sweep, small grids);
+        precond 1: FGMRES(m) right-preconditioned by one V-cycle. Returns (iters, relres, status)."""
+        it = C.c_int()
+        rr = C.c_double()
+        try:
+            st = self.lib.mps_gmres(self._h, self._ptr(b, "i"), self._ptr(x, "g", True), int(m), int(restarts),
+                                    float(tol), int(precond), C.byref(it), C.byref(rr))
+        finally:
+            self._release()
+        M.check(st, allow=(M.OK,) if raise_on_failure else (M.OK, M.NOT_CONVERGED))
+        return it.value, rr.value, st
+
+    def solve_transpose(self, g, lam):
+        n = C.c_int()
+        r = C.c_double()
+        try:
+            st = self.lib.mps_solve_transpose(self._h, self._ptr(g, "i"), self._ptr(lam, "g", True), C.byref(n),
+                                              C.byref(r))
+        finally:
+            self._release()
+        M.check(st, allow=(M.OK, M.NOT_CONVERGED))
+        return n.value, r.value, st
