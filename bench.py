@@ -1,0 +1,195 @@
+"""Benchmark of the implicit field-solve hot path on one MI355X.
+
+Workload: the variable-coefficient Poisson problem with space charge (eps_r = 4
+slab under z = 0.5 cm, eight seeded Gaussian charge blobs, 0 V / 1 kV
+Dirichlet plates) on a 257^3 cell-centred grid over a 2 cm cube, solved from a
+zero initial guess to a relative residual ||b - A u||_2/||b||_2 <= 1e-10 by
+V(2,2) red-black Gauss-Seidel multigrid on the 257 -> 129 -> ... -> 3
+hierarchy.
+
+One step = one complete solve in the headline form: FP32 smoothing,
+restriction and prolongation under an FP64 defect-correction loop (the
+residual is evaluated in FP64 every step). The same solve in plain FP64 and
+the adjoint (transposed-operator) solve are timed beside it. All fields are
+resident in HBM before the timed region.
+
+Prints one JSON line (see the keys below). Also times the reference CPU path
+(the reference Fortran/MPI library built from the reference sources into
+oracle/_ref/, falling back to the oracle's C restatement) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+PEAK_HBM_GBPS = 8000.0
+# algorithmic HBM bytes per fine cell per V-cycle (Poisson, c = 1 coefficient
+# field), summed over the level hierarchy
+BYTES_PER_CELL_CYCLE = {"fp64": 302.9, "fp32dc": 199.4}
+
+
+def gpu_solves(n, steps, warmup, length=0.02, tol=1e-10):
+    import torch
+
+    from mps3d_amd import FieldSolver, from_device, to_device
+    from mps3d_amd import workloads as W
+
+    case = W.varcoef_poisson(n, length=length)
+    hier = "aligned" if n % 2 == 1 else "ref"
+    out = {}
+    sols = {}
+    for prec in ("fp32dc", "fp64"):
+        s = FieldSolver(case.n, case.length, hierarchy=hier, precision=prec, max_cycles=80, tol=tol,
+                        tol_kind="rel")
+        s.set_case(case)
+        u = to_device(np.zeros(s.shape))
+        for _ in range(warmup):
+            u.zero_()
+            s.solve(u)
+        torch.cuda.synchronize()
+        times, cycles, res = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            u.zero_()
+            k, r, _, st = s.solve(u, raise_on_failure=True)
+            ms, _ = s.last_timing()
+            times.append(ms)
+            cycles.append(k)
+            res.append(r)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+        ent = {"ms_per_solve": wall, "event_ms": float(np.mean(times)), "cycles": int(np.max(cycles)),
+               "relres": float(np.max(res))}
+        ent["unknowns_per_s"] = float(np.prod(case.n)) * ent["cycles"] / (wall * 1e-3)
+        ent["solve_GBps"] = BYTES_PER_CELL_CYCLE[prec] * float(np.prod(case.n)) * ent["cycles"] / (wall * 1e-3) / 1e9
+        # hot kernels, timed live with HIP events on the library's stream
+        kern = {}
+        for kid, label in ((0, "colour"), (1, "residual"), (2, "restrict"), (3, "prolong")):
+            fp32 = prec == "fp32dc" and kid != 1
+            ms, by, name = s.time_kernel(kid, fp32=fp32, reps=50)
+            kern[label] = {"name": name, "fp32": fp32, "avg_us": ms * 1e3, "alg_bytes": by,
+                           "GBps": by / (ms * 1e-3) / 1e9}
+        ent["kernels"] = kern
+        if prec == "fp64":
+            # adjoint: A^T lambda = g with the transposed operator, same tolerance
+            g = np.random.default_rng(7).standard_normal(case.n)
+            lam = to_device(np.zeros(s.shape))
+            gd = torch.from_numpy(np.ascontiguousarray(g.transpose(2, 1, 0))).cuda()
+            s.solve_transpose(gd, lam)  # warm (builds A^T)
+            ta = []
+            for _ in range(max(1, steps // 2)):
+                lam.zero_()
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                ka, ra, _ = s.solve_transpose(gd, lam)
+                torch.cuda.synchronize()
+                ta.append((time.perf_counter() - t) * 1e3)
+            ent["adjoint_ms"] = float(np.mean(ta))
+            ent["adjoint_cycles"] = int(ka)
+        sols[prec] = from_device(u)[1:-1, 1:-1, 1:-1]
+        out[prec] = ent
+        s.close()
+        del u
+        torch.cuda.empty_cache()
+    err = float(np.linalg.norm(sols["fp32dc"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
+    return case, hier, out, err
+
+
+def cpu_baseline(n_gpu, cycles_gpu):
+    """Reference CPU path on the host cores, on a bounded sample, scaled to
+    milliseconds per n_gpu^3 solve of cycles_gpu V-cycles."""
+    from mps3d_amd import workloads as W
+    from oracle import refrun
+
+    ncpu = os.cpu_count() or 1
+    if refrun.available():
+        ranks = (2, 2, 4) if ncpu >= 16 else (2, 2, 2)
+        n, ncyc = 128, 40
+        case = W.varcoef_poisson(n)
+        d = dict(mode=refrun.MODE_SOLVE, n=case.n, len=case.length, p=ranks, bc_codes=list(case.bc_codes),
+                 bc_vals=case.bc_vals, dcoeff=case.dcoeff, source=case.source, nvcycles=ncyc, tol=0.0)
+        try:
+            o = refrun.run(d, timeout=600)
+            t = float(o["TIME"][0])
+            per_cell_cycle = t / (n ** 3 * ncyc)
+            return {"value": per_cell_cycle * n_gpu ** 3 * cycles_gpu * 1e3, "unit": "ms per solve (scaled)",
+                    "cores": int(np.prod(ranks)), "kind": "reference",
+                    "sample": "reference Fortran/MPI (amdflang -O3, MPICH) on the 128^3 twin of the same "
+                              "var-coef Poisson, %d ranks, %d V-cycles in %.2f s; scaled per cell-cycle to "
+                              "%d^3 x %d cycles" % (int(np.prod(ranks)), ncyc, t, n_gpu, cycles_gpu)}
+        except Exception as e:  # fall through to the port
+            err = str(e)[:200]
+    else:
+        err = "reference binary or mpiexec missing"
+    from oracle import oracle as O
+    n, ncyc = 32, 10
+    case = W.varcoef_poisson(n)
+    P = O.Problem(case.n, case.length, (1, 1, 1), case.bc_codes, case.bc_vals, case.dcoeff, source=case.source)
+    t = time.perf_counter()
+    O.solve(P, P.zeros(), ncyc, 0.0)
+    t = time.perf_counter() - t
+    return {"value": t / (n ** 3 * ncyc) * n_gpu ** 3 * cycles_gpu * 1e3, "unit": "ms per solve (scaled)",
+            "cores": 1, "kind": "port",
+            "sample": "oracle C restatement, 32^3 twin, 10 V-cycles in %.2f s, scaled (%s)" % (t, err)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=257)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; the whole 257^3 domain fits one MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(3)
+    torch.cuda.set_device(0)
+    case, hier, res, err = gpu_solves(args.n, args.steps, args.warmup)
+    dc, f64 = res["fp32dc"], res["fp64"]
+    kc = dc["kernels"]["colour"]
+    line = {
+        "metric": "257^3 MG solve time to 1e-10: FP32+refine vs FP64, GB/s; adjoint/fwd ratio",
+        "value": round(dc["ms_per_solve"], 3),
+        "unit": "ms per %d^3 solve to rel. residual 1e-10 (FP32 smoothing + FP64 defect correction)" % args.n,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dc["ms_per_solve"], 3),
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": "varcoef_poisson_%d" % args.n, "n": args.n, "domain_m": 0.02,
+                   "hierarchy": hier, "smoother": "RB-GS V(2,2)", "tol_rel_l2": 1e-10, "seed": 20261202},
+        "fp64_ms": round(f64["ms_per_solve"], 3),
+        "speedup_fp32dc_vs_fp64": round(f64["ms_per_solve"] / dc["ms_per_solve"], 3),
+        "cycles": {"fp32dc": dc["cycles"], "fp64": f64["cycles"]},
+        "relres": {"fp32dc": dc["relres"], "fp64": f64["relres"]},
+        "fp32dc_vs_fp64_rel_l2": err,
+        "unknowns_per_s": {"fp32dc": dc["unknowns_per_s"], "fp64": f64["unknowns_per_s"]},
+        "solve_GBps_alg": {"fp32dc": round(dc["solve_GBps"], 1), "fp64": round(f64["solve_GBps"], 1)},
+        "adjoint_ms": round(f64["adjoint_ms"], 3),
+        "adjoint_fwd_ratio": round(f64["adjoint_ms"] / f64["ms_per_solve"], 3),
+        "roofline": {"bound": "hbm", "kernel": kc["name"] + " (fp32, finest level)",
+                     "achieved": round(kc["GBps"], 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": round(kc["GBps"] / PEAK_HBM_GBPS, 4), "traffic": None,
+                     "avg_us": round(kc["avg_us"], 2), "alg_bytes": kc["alg_bytes"]},
+        "kernels": {"fp32dc": dc["kernels"], "fp64": f64["kernels"]},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.n, f64["cycles"])
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -170,6 +170,20 @@ int mps_solve_transpose(mps_ctx *ctx, const double *g, double *lambda, int *ncyc
  * (HIP events): total milliseconds and per-cycle milliseconds. */
 int mps_last_timing(mps_ctx *ctx, double *total_ms, int *ncycles);
 
+/* Measurement hook (no reference counterpart): times `reps` back-to-back
+ * launches of one hot kernel of the current hierarchy with HIP events on
+ * the context's stream, after the context has solved once.
+ *   kernel 0  red-black colour pass on the finest level
+ *   kernel 1  residual b - A u on the finest level
+ *   kernel 2  restriction of the finest residual to level 1
+ *   kernel 3  prolongation + correction from level 1 to the finest level
+ * fp32 selects the FP32 inner hierarchy of MPS_PREC_FP32_DC. avg_ms gets
+ * the mean launch time, alg_bytes the algorithmic HBM bytes of one launch
+ * (each input and output element counted once), name (nullable, name_len
+ * bytes) the kernel's symbol for matching profiler output. */
+int mps_time_kernel(mps_ctx *ctx, int kernel, int fp32, int reps, double *avg_ms, double *alg_bytes,
+                    char *name, int name_len);
+
 /* The context's HIP stream (hipStream_t), for callers that order work
  * against it. */
 void *mps_stream(mps_ctx *ctx);

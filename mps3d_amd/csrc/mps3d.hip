@@ -6,6 +6,7 @@
 #include "../../include/mps3d.h"
 #include "mps3d_bottom.h"
 #include "mps3d_kernels.h"
+#include "mps3d_sweep.h"
 
 #include <hip/hip_runtime.h>
 
@@ -14,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -119,6 +121,8 @@ struct LevelBuf {
     T *st[ST_N] = {};
     T *faces[6] = {};
     double *w[3] = {};
+    T *u2 = nullptr;              // ping-pong partner of u for fused sweeps
+    const double *inv = nullptr;  // dense inverse of the coarsest operator
     bool own_u = true;
     int op = -1;  // operator form override (>= 0), e.g. transposed stencils
 };
@@ -155,6 +159,9 @@ struct mps_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_cycles = 0;
+    double *dinv = nullptr;              // dense inverse of the coarsest operator
+    double *dinvT = nullptr;             // and of its transpose
+    double *gbuf = nullptr;              // ghosted right-hand side of adjoint solves
     bool mg_ok = true;                   // a multigrid hierarchy exists
     std::string mg_error;                // why not, when it does not
 
@@ -164,6 +171,11 @@ struct mps_ctx {
 namespace {
 
 dim3 cells_grid(const Geom &g) { return dim3((unsigned)((g.nx * g.ny + kBlock - 1) / kBlock), (unsigned)g.nz); }
+// reductions: blocks loop over k so a grid has few partial sums
+dim3 red_grid(const Geom &g)
+{
+    return dim3((unsigned)((g.nx * g.ny + kBlock - 1) / kBlock), (unsigned)std::min(g.nz, 16));
+}
 dim3 block() { return dim3(kBlock); }
 
 template <typename T>
@@ -373,6 +385,7 @@ void alloc_levels(mps_ctx *c, std::vector<LevelBuf<T>> &L, bool with_coeffs)
         const long long ng = g.ng();
         LevelBuf<T> &B = L[l];
         B.u = dalloc<T>(ng);
+        B.u2 = dalloc<T>(ng);
         B.b = dalloc<T>(ng);
         B.r = dalloc<T>(ng);
         if (l > 0) B.uold = dalloc<T>(ng);
@@ -390,6 +403,7 @@ void free_levels(std::vector<LevelBuf<T>> &L)
 {
     for (auto &B : L) {
         if (B.own_u) dfree(B.u);
+        dfree(B.u2);
         dfree(B.b);
         dfree(B.r);
         dfree(B.uold);
@@ -451,13 +465,14 @@ void launch_colour(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int red, int 
 
 template <typename T, typename R_>
 void launch_residual(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *u, const T *b, R_ *r, double *partials,
-                     int op)
+                     int op, bool reduce_grid = false)
 {
     Lev<T> v = view(c, L, l);
     const Geom &g = c->geom[l];
     with_op(op, c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
         hipLaunchKernelGGL((k_residual<T, R_, decltype(O)::value, decltype(V)::value, decltype(R)::value>),
-                           cells_grid(g), block(), 0, c->stream, v, u, b, r, partials);
+                           (reduce_grid || partials) ? red_grid(g) : cells_grid(g), block(), 0, c->stream, v, u, b,
+                           r, partials);
     });
 }
 
@@ -466,8 +481,8 @@ template <typename T>
 void residual_sumsq(mps_ctx *c, std::vector<LevelBuf<T>> &L, const T *u, const T *b, int op, T *r_out = nullptr)
 {
     const Geom &g = c->geom[0];
-    dim3 gr = cells_grid(g);
-    launch_residual<T, T>(c, L, 0, u, b, r_out, c->partials, op);
+    dim3 gr = red_grid(g);
+    launch_residual<T, T>(c, L, 0, u, b, r_out, c->partials, op, true);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials,
                        (int)(gr.x * gr.y), c->d_scalar);
 }
@@ -490,6 +505,10 @@ template <typename T>
 void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *work)
 {
     Lev<T> v = view(c, L, l);
+    if (L[l].inv) {  // dense inverse (aligned hierarchy)
+        hipLaunchKernelGGL(k_bottom_dense<T>, dim3(1), block(), 0, c->stream, v, L[l].u, (const T *)L[l].b, L[l].inv);
+        return;
+    }
     // err_tol = ERRTOLR*0.0001 with a default-real literal (mgridsteps.f90:305)
     const T tol = (T)(1e-11 * (double)0.0001f);
     const int m = (int)c->geom[l].n();
@@ -497,6 +516,47 @@ void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *wo
         hipLaunchKernelGGL((k_bottom<T, decltype(O)::value, decltype(V)::value, decltype(R)::value>), dim3(1),
                            block(), 0, c->stream, v, L[l].u, (const T *)L[l].b, work, m, 1, tol, c->d_status);
     });
+}
+
+// fused red-black sweep u -> u2 (see mps3d_sweep.h)
+constexpr int kSweepTX = 64, kSweepTY = 4, kSweepKZ = 32;
+
+template <typename T>
+void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, T *uout, int op)
+{
+    Lev<T> v = view(c, L, l);
+    const Geom &g = c->geom[l];
+    dim3 gr((unsigned)((g.nx + kSweepTX - 1) / kSweepTX), (unsigned)((g.ny + kSweepTY - 1) / kSweepTY),
+            (unsigned)((g.nz + kSweepKZ - 1) / kSweepKZ));
+    auto go = [&](auto O, auto R) {
+        hipLaunchKernelGGL((k_sweep<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
+                           dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kSweepKZ);
+    };
+    if (op == OP_REF) {
+        if (c->has_reac) go(IntC<OP_REF>{}, BoolC<true>{});
+        else go(IntC<OP_REF>{}, BoolC<false>{});
+    } else {
+        if (c->has_reac) go(IntC<OP_ALIGNED>{}, BoolC<true>{});
+        else go(IntC<OP_ALIGNED>{}, BoolC<false>{});
+    }
+}
+
+// nu red-black sweeps on level l: pairs of fused sweeps u -> u2 -> u where
+// the operator has no convection, in-place colour passes otherwise
+template <typename T>
+void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op)
+{
+    int s = 0;
+    if (op != OP_STENCIL && !c->has_vel && L[l].u2) {
+        for (; s + 2 <= nu; s += 2) {
+            launch_sweep<T>(c, L, l, L[l].u, L[l].u2, op);
+            launch_sweep<T>(c, L, l, L[l].u2, L[l].u, op);
+        }
+    }
+    for (; s < nu; ++s) {
+        launch_colour<T>(c, L, l, 1, op);
+        launch_colour<T>(c, L, l, 0, op);
+    }
 }
 
 // --------------------------------------------------------------- V-cycle
@@ -511,10 +571,7 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
         launch_bottom<T>(c, L, l, op, bwork);
         return;
     }
-    for (int s = 0; s < c->opts.nu1; ++s) {
-        launch_colour<T>(c, L, l, 1, op);
-        launch_colour<T>(c, L, l, 0, op);
-    }
+    smooth_level<T>(c, L, l, c->opts.nu1, op);
     launch_residual<T, T>(c, L, l, L[l].u, L[l].b, L[l].r, nullptr, op);
     const Geom &gc = c->geom[l + 1];
     Lev<T> vf = view(c, L, l), vc = view(c, L, l + 1);
@@ -540,10 +597,7 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
         hipLaunchKernelGGL(k_prolong_tri_correct<T>, cells_grid(g), block(), 0, c->stream, vf, vc, L[l].u,
                            (const T *)L[l + 1].u);
     }
-    for (int s = 0; s < c->opts.nu2; ++s) {
-        launch_colour<T>(c, L, l, 1, op);
-        launch_colour<T>(c, L, l, 0, op);
-    }
+    smooth_level<T>(c, L, l, c->opts.nu2, op);
 }
 
 // ---------------------------------------------------------- coefficients
@@ -684,17 +738,18 @@ const double *stage_interior(mps_ctx *c, const double *a)
     return c->user_tmp2;
 }
 
-// one FP64 V-cycle on u (level 0 bound to u) with b0 as right-hand side;
-// returns the sum of squares of the final residual in d_scalar
-void fp64_cycle(mps_ctx *c, double *u)
+// one FP64 V-cycle on u (level 0 bound to u) with right-hand side b;
+// leaves the sum of squares of the final residual in d_scalar
+void fp64_cycle(mps_ctx *c, double *u, double *b, bool homog)
 {
     auto &L = c->L64;
     L[0].u = u;
     L[0].own_u = false;
-    L[0].b = c->b0;
-    vcycle<double>(c, L, 0, c->bottom64, false);
-    if (!c->geom[0].bottom) residual_sumsq<double>(c, L, u, c->b0, level_op(c, 0));
+    L[0].b = b;
+    vcycle<double>(c, L, 0, c->bottom64, homog);
+    if (!c->geom[0].bottom) residual_sumsq<double>(c, L, u, b, level_op(c, 0));
 }
+void fp64_cycle(mps_ctx *c, double *u) { fp64_cycle(c, u, c->b0, false); }
 
 
 }  // namespace
@@ -705,15 +760,17 @@ template <int OP, bool VEL, bool REAC>
 __global__ void __launch_bounds__(kBlock) k_defect32(Lev<double> L, const double *__restrict__ u,
                                                     const double *__restrict__ b, float *d32, double *partials)
 {
-    int i, j, k;
-    long long c;
+    int i, j;
     double sq = 0.0;
-    if (cell_of(L, i, j, k, c)) {
-        double dg, of;
-        row<double, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
-        double rv = b[c] - (dg * u[c] + of);
-        d32[c] = (float)rv;
-        sq = rv * rv;
+    if (cell_ij(L, i, j)) {
+        for (int k = blockIdx.y + 1; k <= L.nz; k += gridDim.y) {
+            const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
+            double dg, of;
+            row<double, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
+            double rv = b[c] - (dg * u[c] + of);
+            d32[c] = (float)rv;
+            sq += rv * rv;
+        }
     }
     double s = block_sum(sq);
     if (threadIdx.x == 0) partials[blockIdx.x + (long long)gridDim.x * blockIdx.y] = s;
@@ -722,16 +779,17 @@ __global__ void __launch_bounds__(kBlock) k_defect32(Lev<double> L, const double
 
 namespace {
 
-void defect_step(mps_ctx *c, double *u, bool do_cycle)
+void defect_step(mps_ctx *c, double *u, bool do_cycle, const double *b = nullptr)
 {
     auto &F = c->L32;
     const Geom &g = c->geom[0];
     Lev<double> v = view(c, c->L64, 0);
+    if (!b) b = c->b0;
     with_op(level_op(c, 0), c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
-        hipLaunchKernelGGL((k_defect32<decltype(O)::value, decltype(V)::value, decltype(R)::value>), cells_grid(g),
-                           block(), 0, c->stream, v, (const double *)u, (const double *)c->b0, F[0].b, c->partials);
+        hipLaunchKernelGGL((k_defect32<decltype(O)::value, decltype(V)::value, decltype(R)::value>), red_grid(g),
+                           block(), 0, c->stream, v, (const double *)u, b, F[0].b, c->partials);
     });
-    dim3 gr = cells_grid(g);
+    dim3 gr = red_grid(g);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials,
                        (int)(gr.x * gr.y), c->d_scalar);
     if (!do_cycle) return;
@@ -749,14 +807,151 @@ double measure(const mps_ctx *c, double sumsq, double bnorm2)
     return std::sqrt(sumsq / N);
 }
 
-// squared 2-norm of b0 into d_scalar
-void b0_sumsq(mps_ctx *c)
+// squared 2-norm of b into d_scalar
+void b_sumsq(mps_ctx *c, const double *b)
 {
-    // residual of u = 0 against b0 is b0 itself: reuse the residual kernel
+    // residual of u = 0 against b is b itself: reuse the residual kernel
     // with a zero solution held in the level-0 residual buffer
     auto &L = c->L64;
     fill<double>(c, L[0].r, c->geom[0].ng(), 0.0);
-    residual_sumsq<double>(c, L, L[0].r, c->b0, level_op(c, 0));
+    residual_sumsq<double>(c, L, L[0].r, b, level_op(c, 0));
+}
+
+// The solve loop shared by the forward and the self-adjoint solves: V-cycles
+// (FP64) or FP32 defect-correction steps on u for right-hand side b until
+// the opts.tol_kind measure <= opts.tol. Returns the cycles run.
+int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, double &meas)
+{
+    double bn2 = 0.0;
+    if (c->opts.tol_kind == MPS_TOL_REL_L2) {
+        b_sumsq(c, b);
+        bn2 = read_scalar(c);
+    }
+    const bool dc = c->opts.precision == MPS_PREC_FP32_DC;
+    int it = 0;
+    meas = 0.0;
+    if (dc) {
+        // defect correction: residual of the current iterate first
+        defect_step(c, u, false, b);
+        meas = measure(c, read_scalar(c), bn2);
+        if (!(meas <= c->opts.tol)) {
+            for (it = 0; it < c->opts.max_cycles;) {
+                // the FP32 cycle on the defect, the FP64 correction, then the
+                // FP64 residual of the corrected iterate (the next defect)
+                fill<float>(c, c->L32[0].u, c->geom[0].ng(), 0.0f);
+                vcycle<float>(c, c->L32, 0, c->bottom32, true);
+                Lev<double> v64 = view(c, c->L64, 0);
+                hipLaunchKernelGGL((k_add_interior<double, float>), cells_grid(c->geom[0]), block(), 0, c->stream,
+                                   v64, u, (const float *)c->L32[0].u);
+                defect_step(c, u, false, b);
+                meas = measure(c, read_scalar(c), bn2);
+                if (history) history[it] = meas;
+                ++it;
+                if (!std::isfinite(meas)) break;
+                if (meas <= c->opts.tol) break;
+            }
+        }
+    } else {
+        for (it = 0; it < c->opts.max_cycles;) {
+            fp64_cycle(c, u, b, homog);
+            meas = c->geom[0].bottom ? 0.0 : measure(c, read_scalar(c), bn2);
+            if (history) history[it] = meas;
+            ++it;
+            if (!std::isfinite(meas)) break;
+            if (meas <= c->opts.tol) break;
+        }
+    }
+    return it;
+}
+
+// coarsest-level dense inverse for the aligned hierarchy: assembles the
+// coarsest operator from its 7-point rows and inverts it (Gauss-Jordan with
+// partial pivoting, FP64) once per set of coefficients
+void build_bottom_inverse(mps_ctx *c)
+{
+    for (auto &B : c->L64) B.inv = nullptr;
+    for (auto &B : c->L32) B.inv = nullptr;
+    if (!c->mg_ok || c->opts.hierarchy != MPS_HIER_ALIGNED) return;
+    const int lb = c->nlev() - 1;
+    const Geom &g = c->geom[lb];
+    const int n = (int)g.n();
+    if (n > kBottomMaxDense) throw HipError{hipErrorInvalidValue, "coarsest grid too large for a dense solve"};
+    const long long ng = g.ng();
+    std::vector<std::vector<double>> st(ST_N, std::vector<double>(ng, 0.0));
+    auto &B = c->L64[lb];
+    if (level_op(c, lb) == OP_STENCIL) {
+        for (int q = 0; q < ST_N; ++q)
+            HIPCHK(hipMemcpyAsync(st[q].data(), B.st[q], ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    } else {
+        double *tmp[ST_N];
+        for (int q = 0; q < ST_N; ++q) {
+            tmp[q] = dalloc<double>(ng);
+            fill<double>(c, tmp[q], ng, 0.0);
+        }
+        Lev<double> v = view(c, c->L64, lb);
+        with_op(level_op(c, lb), c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
+            hipLaunchKernelGGL((k_build_stencil<double, decltype(O)::value, decltype(V)::value, decltype(R)::value>),
+                               cells_grid(g), block(), 0, c->stream, v, tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5],
+                               tmp[6], 0);
+        });
+        for (int q = 0; q < ST_N; ++q)
+            HIPCHK(hipMemcpyAsync(st[q].data(), tmp[q], ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int q = 0; q < ST_N; ++q) dfree(tmp[q]);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    std::vector<double> A((size_t)n * n, 0.0), I((size_t)n * n, 0.0);
+    auto gi = [&](int i, int j, int k) { return (long long)i + (long long)(nx + 2) * (j + (long long)(ny + 2) * k); };
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const int r = (i - 1) + nx * ((j - 1) + ny * (k - 1));
+                const long long cc = gi(i, j, k);
+                A[(size_t)r * n + r] = st[ST_C][cc];
+                if (i > 1) A[(size_t)r * n + r - 1] = st[ST_XM][cc];
+                if (i < nx) A[(size_t)r * n + r + 1] = st[ST_XP][cc];
+                if (j > 1) A[(size_t)r * n + r - nx] = st[ST_YM][cc];
+                if (j < ny) A[(size_t)r * n + r + nx] = st[ST_YP][cc];
+                if (k > 1) A[(size_t)r * n + r - nx * ny] = st[ST_ZM][cc];
+                if (k < nz) A[(size_t)r * n + r + nx * ny] = st[ST_ZP][cc];
+            }
+    for (int r = 0; r < n; ++r) I[(size_t)r * n + r] = 1.0;
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (std::fabs(A[(size_t)r * n + col]) > std::fabs(A[(size_t)piv * n + col])) piv = r;
+        if (!(std::fabs(A[(size_t)piv * n + col]) > 0.0))
+            throw HipError{hipErrorInvalidValue, "singular coarsest-level operator"};
+        if (piv != col)
+            for (int q = 0; q < n; ++q) {
+                std::swap(A[(size_t)piv * n + q], A[(size_t)col * n + q]);
+                std::swap(I[(size_t)piv * n + q], I[(size_t)col * n + q]);
+            }
+        const double d = 1.0 / A[(size_t)col * n + col];
+        for (int q = 0; q < n; ++q) {
+            A[(size_t)col * n + q] *= d;
+            I[(size_t)col * n + q] *= d;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == col) continue;
+            const double f = A[(size_t)r * n + col];
+            if (f == 0.0) continue;
+            for (int q = 0; q < n; ++q) {
+                A[(size_t)r * n + q] -= f * A[(size_t)col * n + q];
+                I[(size_t)r * n + q] -= f * I[(size_t)col * n + q];
+            }
+        }
+    }
+    std::vector<double> IT((size_t)n * n);
+    for (int r = 0; r < n; ++r)
+        for (int q = 0; q < n; ++q) IT[(size_t)q * n + r] = I[(size_t)r * n + q];
+    if (!c->dinv) c->dinv = dalloc<double>((size_t)kBottomMaxDense * kBottomMaxDense);
+    if (!c->dinvT) c->dinvT = dalloc<double>((size_t)kBottomMaxDense * kBottomMaxDense);
+    HIPCHK(hipMemcpy(c->dinv, I.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->dinvT, IT.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+    c->L64[lb].inv = c->dinv;
+    if (!c->L32.empty()) c->L32[lb].inv = c->dinv;
 }
 
 }  // namespace
@@ -817,6 +1012,9 @@ int mps_destroy(mps_ctx *c)
     dfree(c->bottomT);
     dfree(c->user_tmp);
     dfree(c->user_tmp2);
+    dfree(c->dinv);
+    dfree(c->dinvT);
+    dfree(c->gbuf);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -861,7 +1059,7 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
         c->b0 = dalloc<double>(g0.ng());
         c->user_tmp = dalloc<double>(g0.ng());
         c->user_tmp2 = dalloc<double>(g0.ng());
-        dim3 gr = cells_grid(g0);
+        dim3 gr = red_grid(g0);
         c->max_partials = (int)(gr.x * gr.y);
         c->partials = dalloc<double>(c->max_partials);
         c->d_scalar = dalloc<double>(4);
@@ -1031,6 +1229,7 @@ int mps_set_coeffs(mps_ctx *c, const double *dcoeff, const double *velx, const d
             }
         }
         if (c->opts.precision == MPS_PREC_FP32_DC) build_fp32(c);
+        build_bottom_inverse(c);
         HIPCHK(hipStreamSynchronize(c->stream));
         c->coeffs_set = true;
         c->transpose_ready = false;
@@ -1118,10 +1317,7 @@ int mps_smooth(mps_ctx *c, double *soln, const double *sterm, int niter, double 
         L[0].u = u;
         L[0].b = c->b0;
         const int op = level_op(c, 0);
-        for (int s = 0; s < niter; ++s) {
-            launch_colour<double>(c, L, 0, 1, op);
-            launch_colour<double>(c, L, 0, 0, op);
-        }
+        smooth_level<double>(c, L, 0, niter, op);
         if (res) {
             launch_residual<double, double>(c, L, 0, u, c->b0, L[0].r, nullptr, op);
             Lev<double> v = view(c, L, 0);
@@ -1198,45 +1394,8 @@ int mps_solve(mps_ctx *c, double *soln, int *ncycles, double *resnorm, double *h
         double *u = bind_solution(c, soln, staged);
         HIPCHK(hipEventRecord(c->ev0, c->stream));
         compute_b0(c, nullptr);
-        double bn2 = 0.0;
-        if (c->opts.tol_kind == MPS_TOL_REL_L2) {
-            b0_sumsq(c);
-            bn2 = read_scalar(c);
-        }
-        const bool dc = c->opts.precision == MPS_PREC_FP32_DC;
-        int it = 0;
         double meas = 0.0;
-        if (dc) {
-            // defect correction: residual of the current iterate first
-            defect_step(c, u, false);
-            meas = measure(c, read_scalar(c), bn2);
-            if (!(meas <= c->opts.tol)) {
-                for (it = 0; it < c->opts.max_cycles;) {
-                    // the cycle, then the FP64 residual of the corrected
-                    // iterate (which the next step reuses as its defect)
-                    fill<float>(c, c->L32[0].u, c->geom[0].ng(), 0.0f);
-                    vcycle<float>(c, c->L32, 0, c->bottom32, true);
-                    Lev<double> v64 = view(c, c->L64, 0);
-                    hipLaunchKernelGGL((k_add_interior<double, float>), cells_grid(c->geom[0]), block(), 0,
-                                       c->stream, v64, u, (const float *)c->L32[0].u);
-                    defect_step(c, u, false);
-                    meas = measure(c, read_scalar(c), bn2);
-                    if (history) history[it] = meas;
-                    ++it;
-                    if (!std::isfinite(meas)) break;
-                    if (meas <= c->opts.tol) break;
-                }
-            }
-        } else {
-            for (it = 0; it < c->opts.max_cycles;) {
-                fp64_cycle(c, u);
-                meas = c->geom[0].bottom ? 0.0 : measure(c, read_scalar(c), bn2);
-                if (history) history[it] = meas;
-                ++it;
-                if (!std::isfinite(meas)) break;
-                if (meas <= c->opts.tol) break;
-            }
-        }
+        const int it = solve_core(c, u, c->b0, false, history, meas);
         HIPCHK(hipEventRecord(c->ev1, c->stream));
         HIPCHK(hipMemcpyAsync(c->h_scalar + 1, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         unbind_solution(c, soln, staged);
@@ -1309,6 +1468,7 @@ void ensure_transpose(mps_ctx *c)
         c->LT[0].own_u = false;
     }
     build_stencils(c, c->LT, true);
+    c->LT.back().inv = c->opts.hierarchy == MPS_HIER_ALIGNED ? c->dinvT : nullptr;
     HIPCHK(hipStreamSynchronize(c->stream));
     c->transpose_ready = true;
 }
@@ -1364,7 +1524,7 @@ void multidot(mps_ctx *c, GmresBufs &G, const double *w, double *const *vs, int 
     Lev<double> v = view(c, c->L64, 0);
     VecList VL{};
     for (int q = 0; q < nv; ++q) VL.v[q] = vs[q];
-    hipLaunchKernelGGL(k_multidot, cells_grid(c->geom[0]), block(), 0, c->stream, v, w, VL, nv, G.dpart);
+    hipLaunchKernelGGL(k_multidot, red_grid(c->geom[0]), block(), 0, c->stream, v, w, VL, nv, G.dpart);
     hipLaunchKernelGGL(k_reduce_cols, dim3(nv), block(), 0, c->stream, (const double *)G.dpart, G.nblocks, G.dh);
     HIPCHK(hipMemcpyAsync(G.hh, G.dh, nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1379,9 +1539,9 @@ double multiaxpy(mps_ctx *c, GmresBufs &G, double *w, double *const *vs, int nv,
     for (int q = 0; q < nv; ++q) VL.v[q] = vs[q];
     std::memcpy(G.hh, h, nv * sizeof(double));
     HIPCHK(hipMemcpyAsync(G.dh, G.hh, nv * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_multiaxpy, cells_grid(c->geom[0]), block(), 0, c->stream, v, w, VL, (const double *)G.dh,
+    hipLaunchKernelGGL(k_multiaxpy, red_grid(c->geom[0]), block(), 0, c->stream, v, w, VL, (const double *)G.dh,
                        nv, c->partials);
-    dim3 gr = cells_grid(c->geom[0]);
+    dim3 gr = red_grid(c->geom[0]);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials,
                        (int)(gr.x * gr.y), c->d_scalar);
     return read_scalar(c);
@@ -1442,7 +1602,7 @@ int mps_gmres(mps_ctx *c, const double *b, double *x, int m, int max_restarts, d
         if (!c->mg_ok) return fail(MPS_ERR_ARG, "no multigrid hierarchy for this grid: %s", c->mg_error.c_str());
         GmresBufs G;
         G.m = m;
-        dim3 gr = cells_grid(g);
+        dim3 gr = red_grid(g);
         G.nblocks = (int)(gr.x * gr.y);
         for (int q = 0; q <= m; ++q) G.V.push_back(dalloc<double>(g.ng()));
         for (int q = 0; q < m; ++q) G.Z.push_back(dalloc<double>(g.ng()));
@@ -1595,6 +1755,40 @@ int mps_solve_transpose(mps_ctx *c, const double *gsrc, double *lambda, int *ncy
     if (st) return st;
     if (!gsrc || !lambda) return fail(MPS_ERR_ARG, "null argument");
     int status = MPS_NOT_CONVERGED;
+    // A = A^T for the reference diffusion operator without convection on a
+    // uniform finest grid: the adjoint solve is the forward solve with the
+    // adjoint source (the V-cycle is then self-adjoint as well)
+    const bool symmetric = c->opts.flux == MPS_FLUX_UPWIND && !c->has_vel && level_op(c, 0) == OP_REF;
+    if (symmetric) {
+        try {
+            const Geom &g = c->geom[0];
+            bool staged;
+            double *u = bind_solution(c, lambda, staged);
+            if (!c->gbuf) {
+                c->gbuf = dalloc<double>(g.ng());
+                fill<double>(c, c->gbuf, g.ng(), 0.0);
+            }
+            const double *gd = stage_interior(c, gsrc);
+            Lev<double> v = view(c, c->L64, 0);
+            hipLaunchKernelGGL(k_interior_in<double>, cells_grid(g), block(), 0, c->stream, v, gd, c->gbuf);
+            HIPCHK(hipEventRecord(c->ev0, c->stream));
+            double meas = 0.0;
+            const int it = solve_core(c, u, c->gbuf, true, nullptr, meas);
+            HIPCHK(hipEventRecord(c->ev1, c->stream));
+            unbind_solution(c, lambda, staged);
+            HIPCHK(hipStreamSynchronize(c->stream));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->last_ms = ms;
+            c->last_cycles = it;
+            if (ncycles) *ncycles = it;
+            if (resnorm) *resnorm = meas;
+            if (!std::isfinite(meas)) return fail(MPS_ERR_NAN, "residual is not finite");
+            return meas <= c->opts.tol ? MPS_OK : MPS_NOT_CONVERGED;
+        } catch (const HipError &e) {
+            return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+        }
+    }
     try {
         ensure_transpose(c);
         borrow_weights(c);
@@ -1642,3 +1836,113 @@ int mps_solve_transpose(mps_ctx *c, const double *gsrc, double *lambda, int *ncy
 }
 
 }  // extern "C"
+
+// ============================================================ measurement
+namespace {
+
+template <typename T>
+int time_kernel_t(mps_ctx *c, std::vector<LevelBuf<T>> &L, int kernel, int reps, double *avg_ms, double *bytes,
+                  std::string &name)
+{
+    const Geom &g = c->geom[0];
+    const double w = sizeof(T);
+    const double N = (double)g.n();
+    const int op = level_op(c, L, 0);
+    // coefficient fields read by the operator: D (+ 3 velocities + reac),
+    // or the 7 stencil rows
+    double cf = op == OP_STENCIL ? 7.0 : 1.0 + (c->has_vel ? 3.0 : 0.0) + (c->has_reac ? 1.0 : 0.0);
+    T *u = L[0].u, *b = L[0].b;
+    const bool bind64 = std::is_same<T, double>::value;
+    if (bind64) {
+        L[0].u = (T *)c->user_tmp;
+        L[0].b = (T *)c->b0;
+    }
+    std::function<void()> launch;
+    if (kernel == 0) {
+        launch = [&]() { launch_colour<T>(c, L, 0, 1, op); };
+        *bytes = (1.5 + cf) * w * N;  // other colour u (1/2), own b (1/2), coefficients; own u write (1/2)
+        name = "k_colour";
+    } else if (kernel == 1) {
+        launch = [&]() { launch_residual<T, T>(c, L, 0, L[0].u, L[0].b, L[0].r, nullptr, op); };
+        *bytes = (cf + 3.0) * w * N;
+        name = "k_residual";
+    } else if (kernel == 2) {
+        if (c->nlev() < 2) return fail(MPS_ERR_ARG, "no coarse level");
+        const Geom &gc = c->geom[1];
+        Lev<T> vf = view(c, L, 0), vc = view(c, L, 1);
+        if (c->opts.hierarchy == MPS_HIER_REF) {
+            launch = [&, vf, vc]() {
+                hipLaunchKernelGGL(k_restrict_fas<T>, cells_grid(gc), block(), 0, c->stream, vf, vc,
+                                   (const T *)L[0].u, (const T *)L[0].r, L[1].uold, L[1].u, L[1].b);
+            };
+            *bytes = (2.0 * N + 3.0 * (double)gc.n()) * w;
+            name = "k_restrict_fas";
+        } else {
+            AlignedW W{L[1].w[0], L[1].w[1], L[1].w[2]};
+            launch = [&, vf, vc, W]() {
+                hipLaunchKernelGGL(k_restrict_fw<T>, cells_grid(gc), block(), 0, c->stream, vf, vc, W,
+                                   (const T *)L[0].r, L[1].b);
+            };
+            *bytes = (N + (double)gc.n()) * w;
+            name = "k_restrict_fw";
+        }
+    } else if (kernel == 3) {
+        if (c->nlev() < 2) return fail(MPS_ERR_ARG, "no coarse level");
+        const Geom &gc = c->geom[1];
+        Lev<T> vf = view(c, L, 0), vc = view(c, L, 1);
+        if (c->opts.hierarchy == MPS_HIER_REF) {
+            FacesT<T> G{};
+            launch = [&, vf, vc, G]() {
+                hipLaunchKernelGGL(k_prolong1_correct<T>, cells_grid(gc), block(), 0, c->stream, vf, vc, L[0].u,
+                                   (const T *)L[1].u, (const T *)L[1].uold, G);
+            };
+            *bytes = (2.0 * N + 2.0 * (double)gc.n()) * w;
+            name = "k_prolong1_correct";
+        } else {
+            launch = [&, vf, vc]() {
+                hipLaunchKernelGGL(k_prolong_tri_correct<T>, cells_grid(g), block(), 0, c->stream, vf, vc, L[0].u,
+                                   (const T *)L[1].u);
+            };
+            *bytes = (2.0 * N + (double)gc.n()) * w;
+            name = "k_prolong_tri_correct";
+        }
+    } else {
+        return fail(MPS_ERR_ARG, "unknown kernel %d", kernel);
+    }
+    launch();  // warm
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int q = 0; q < reps; ++q) launch();
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / reps;
+    if (bind64) {
+        L[0].u = u;
+        L[0].b = b;
+    }
+    return MPS_OK;
+}
+
+}  // namespace
+
+extern "C" int mps_time_kernel(mps_ctx *c, int kernel, int fp32, int reps, double *avg_ms, double *alg_bytes,
+                               char *name, int name_len)
+{
+    int st = guard_mg(c);
+    if (st) return st;
+    if (reps < 1 || !avg_ms || !alg_bytes) return fail(MPS_ERR_ARG, "bad arguments");
+    if (fp32 && c->opts.precision != MPS_PREC_FP32_DC) return fail(MPS_ERR_ARG, "no FP32 hierarchy");
+    try {
+        std::string nm;
+        st = fp32 ? time_kernel_t<float>(c, c->L32, kernel, reps, avg_ms, alg_bytes, nm)
+                  : time_kernel_t<double>(c, c->L64, kernel, reps, avg_ms, alg_bytes, nm);
+        if (name && name_len > 0) {
+            std::strncpy(name, nm.c_str(), name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        return st;
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+}

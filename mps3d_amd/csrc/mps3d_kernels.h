@@ -18,6 +18,7 @@
 namespace mps {
 
 constexpr int kBlock = 256;
+constexpr int kBottomMaxDense = 512;  // largest coarsest grid solved by a dense inverse
 
 // ------------------------------------------------------------------------
 // cell addressing
@@ -32,6 +33,18 @@ __device__ __forceinline__ bool cell_of(const Lev<T> &L, int &i, int &j, int &k,
     j += 1;
     k = blockIdx.y + 1;
     c = (long long)i + (long long)L.sy * j + L.sz * k;
+    return true;
+}
+
+// (i,j) of this thread for kernels that loop over k themselves
+template <typename T>
+__device__ __forceinline__ bool cell_ij(const Lev<T> &L, int &i, int &j)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.nx * L.ny) return false;
+    j = t / L.nx;
+    i = t - j * L.nx + 1;
+    j += 1;
     return true;
 }
 
@@ -103,11 +116,57 @@ __device__ __forceinline__ void bc_row(int code, bool hi, T Dc, T vb, T h, T abv
     }
 }
 
+// diag and offdiag of a cell without convection, from the six neighbour
+// values and the seven diffusion coefficients (find_AX order; the neighbour
+// values of faces that do not exist are ignored).
+template <typename T, int OP, bool REAC>
+__device__ __forceinline__ void row7(const Lev<T> &L, int i, int j, int k, T uxm, T uxp, T uym, T uyp, T uzm,
+                                     T uzp, T Dc, T Dxm, T Dxp, T Dym, T Dyp, T Dzm, T Dzp, T rc, T &dg, T &of)
+{
+    T d = T(0), o = T(0);
+    {
+        const bool edge = (i == 1) || (i == L.nx);
+        const T abv = (OP == OP_ALIGNED && edge) ? L.axb : L.ax;
+        if (i > 1) face_right<T, false>(uxm, Dxm, Dc, T(0), T(0), abv, L.dx, d, o);
+        if (i < L.nx) face_left<T, false>(uxp, Dc, Dxp, T(0), T(0), abv, L.dx, d, o);
+        if (i == 1) bc_row<T, OP, false>(L.bc[LEFT], false, Dc, T(0), L.dx, abv, L.ix0, d);
+        if (i == L.nx) bc_row<T, OP, false>(L.bc[RIGHT], true, Dc, T(0), L.dx, abv, L.ix0, d);
+    }
+    {
+        const bool edge = (j == 1) || (j == L.ny);
+        const T abv = (OP == OP_ALIGNED && edge) ? L.ayb : L.ay;
+        if (j > 1) face_right<T, false>(uym, Dym, Dc, T(0), T(0), abv, L.dy, d, o);
+        if (j < L.ny) face_left<T, false>(uyp, Dc, Dyp, T(0), T(0), abv, L.dy, d, o);
+        if (j == 1) bc_row<T, OP, false>(L.bc[BOTTOM], false, Dc, T(0), L.dy, abv, L.iy0, d);
+        if (j == L.ny) bc_row<T, OP, false>(L.bc[TOP], true, Dc, T(0), L.dy, abv, L.iy0, d);
+    }
+    {
+        const bool edge = (k == 1) || (k == L.nz);
+        const T abv = (OP == OP_ALIGNED && edge) ? L.azb : L.az;
+        if (k > 1) face_right<T, false>(uzm, Dzm, Dc, T(0), T(0), abv, L.dz, d, o);
+        if (k < L.nz) face_left<T, false>(uzp, Dc, Dzp, T(0), T(0), abv, L.dz, d, o);
+        if (k == 1) bc_row<T, OP, false>(L.bc[BACK], false, Dc, T(0), L.dz, abv, L.iz0, d);
+        if (k == L.nz) bc_row<T, OP, false>(L.bc[FRONT], true, Dc, T(0), L.dz, abv, L.iz0, d);
+    }
+    if (REAC) d = d - rc;
+    if (L.timederiv) d = d + T(1) / L.dt;
+    dg = d;
+    of = o;
+}
+
 // diag and offdiag of cell (i,j,k) at address c for the current u.
 template <typename T, int OP, bool VEL, bool REAC>
 __device__ __forceinline__ void row(const Lev<T> &L, const T *__restrict__ u, int i, int j, int k,
                                     long long c, T &dg, T &of)
 {
+    if (OP != OP_STENCIL && !VEL) {
+        const long long sy = L.sy, sz = L.sz;
+        const T *__restrict__ D = L.D;
+        row7<T, OP, REAC>(L, i, j, k, u[c - 1], u[c + 1], u[c - sy], u[c + sy], u[c - sz], u[c + sz], D[c],
+                          D[c - 1], D[c + 1], D[c - sy], D[c + sy], D[c - sz], D[c + sz],
+                          REAC ? L.reac[c] : T(0), dg, of);
+        return;
+    }
     if (OP == OP_STENCIL) {
         const long long sy = L.sy, sz = L.sz;
         T o = T(0);
@@ -266,15 +325,17 @@ template <typename T, typename R, int OP, bool VEL, bool REAC>
 __global__ void __launch_bounds__(kBlock) k_residual(Lev<T> L, const T *__restrict__ u, const T *__restrict__ b,
                                                     R *r, double *partials)
 {
-    int i, j, k;
-    long long c;
+    int i, j;
     double sq = 0.0;
-    if (cell_of(L, i, j, k, c)) {
-        T dg, of;
-        row<T, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
-        T rv = b[c] - (dg * u[c] + of);
-        if (r) r[c] = (R)rv;
-        sq = (double)rv * (double)rv;
+    if (cell_ij(L, i, j)) {
+        for (int k = blockIdx.y + 1; k <= L.nz; k += gridDim.y) {
+            const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
+            T dg, of;
+            row<T, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
+            T rv = b[c] - (dg * u[c] + of);
+            if (r) r[c] = (R)rv;
+            sq += (double)rv * (double)rv;
+        }
     }
     if (partials) {
         double s = block_sum(sq);
@@ -693,14 +754,19 @@ struct VecList {
 __global__ void __launch_bounds__(kBlock) k_multidot(Lev<double> L, const double *__restrict__ w, VecList V, int nv,
                                                     double *partials)
 {
-    int i, j, k;
-    long long c;
-    const bool ok = cell_of(L, i, j, k, c);
-    const double wc = ok ? w[c] : 0.0;
+    int i, j;
+    const bool ok = cell_ij(L, i, j);
     const int nblocks = gridDim.x * gridDim.y;
     const int bid = blockIdx.x + gridDim.x * blockIdx.y;
     for (int q = 0; q < nv; ++q) {
-        double s = block_sum(ok ? wc * V.v[q][c] : 0.0);
+        double acc = 0.0;
+        if (ok) {
+            for (int k = blockIdx.y + 1; k <= L.nz; k += gridDim.y) {
+                const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
+                acc += w[c] * V.v[q][c];
+            }
+        }
+        double s = block_sum(acc);
         if (threadIdx.x == 0) partials[(long long)q * nblocks + bid] = s;
         __syncthreads();
     }
@@ -720,14 +786,16 @@ __global__ void __launch_bounds__(kBlock) k_reduce_cols(const double *partials, 
 __global__ void __launch_bounds__(kBlock) k_multiaxpy(Lev<double> L, double *w, VecList V, const double *h, int nv,
                                                      double *partials)
 {
-    int i, j, k;
-    long long c;
+    int i, j;
     double sq = 0.0;
-    if (cell_of(L, i, j, k, c)) {
-        double x = w[c];
-        for (int q = 0; q < nv; ++q) x = x - h[q] * V.v[q][c];
-        w[c] = x;
-        sq = x * x;
+    if (cell_ij(L, i, j)) {
+        for (int k = blockIdx.y + 1; k <= L.nz; k += gridDim.y) {
+            const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
+            double x = w[c];
+            for (int q = 0; q < nv; ++q) x = x - h[q] * V.v[q][c];
+            w[c] = x;
+            sq += x * x;
+        }
     }
     if (partials) {
         double s = block_sum(sq);

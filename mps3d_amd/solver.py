@@ -207,6 +207,15 @@ class FieldSolver:
         M.check(self.lib.mps_last_timing(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def time_kernel(self, kernel=0, fp32=False, reps=20):
+        """Mean launch time (ms), algorithmic bytes per launch and symbol name of
+        one hot kernel (0 colour pass, 1 residual, 2 restriction, 3 prolongation)."""
+        ms, by = C.c_double(), C.c_double()
+        name = C.create_string_buffer(64)
+        M.check(self.lib.mps_time_kernel(self._h, int(kernel), int(bool(fp32)), int(reps), C.byref(ms),
+                                         C.byref(by), name, 64))
+        return ms.value, by.value, name.value.decode()
+
     def find_AX(self, x):
         ax, dg, of = (np.zeros(self.n, order="F") for _ in range(3))
         try:

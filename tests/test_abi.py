@@ -1,0 +1,66 @@
+"""The C ABI library loads and exports every entry point include/mps3d.h declares.
+
+No compute call is made here (no GPU in this environment); only host-side
+paths: option defaults, error strings and the missing-device status.
+"""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+from mps3d_amd import _lib as M
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "mps3d.h")) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"\b(mps_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert sorted(M.EXPORTS) == declared_symbols()
+
+
+def test_library_exports_every_declared_symbol():
+    lib = M.load()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_default_options_are_the_reference_constants():
+    lib = M.load()
+    o = M.Opts()
+    lib.mps_default_opts(C.byref(o))
+    # nsmoothsteps = 2 (mgridsteps.f90:473), 20 V-cycles, ERRTOLR = 1e-11
+    assert (o.nu1, o.nu2, o.max_cycles, o.tol) == (2, 2, 20, 1e-11)
+    assert (o.hierarchy, o.precision, o.flux, o.tol_kind) == (M.HIER_REF, M.PREC_FP64, M.FLUX_UPWIND, M.TOL_RMS)
+
+
+def test_error_strings():
+    lib = M.load()
+    for code in (M.OK, M.NOT_CONVERGED, M.ERR_NAN, M.ERR_ARG, M.ERR_HIP, M.ERR_STATE, M.ERR_NOLIB):
+        assert lib.mps_error_string(code).decode()
+
+
+def test_create_rejects_bad_grid_before_touching_a_device():
+    lib = M.load()
+    h = C.c_void_p()
+    g = M.Grid(0, 4, 4, 1.0, 1.0, 1.0, 1, 1, 1)
+    assert lib.mps_create(C.byref(h), C.byref(g), None) == M.ERR_ARG
+    assert not h.value
+
+
+def test_null_context_is_an_argument_error():
+    lib = M.load()
+    assert lib.mps_solve(None, None, None, None, None) == M.ERR_ARG
+    assert lib.mps_destroy(None) == M.OK
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU device node is present")
+def test_create_without_device_reports_no_gpu():
+    lib = M.load()
+    h = C.c_void_p()
+    g = M.Grid(4, 4, 4, 1.0, 1.0, 1.0, 1, 1, 1)
+    assert lib.mps_create(C.byref(h), C.byref(g), None) == M.ERR_NOLIB

@@ -1,0 +1,185 @@
+"""The solver forms with no reference counterpart, checked against what the
+mathematics fixes: the aligned (m*2^k+1) hierarchy, FP32 smoothing under FP64
+defect correction, FGMRES with a V-cycle preconditioner, the 3D
+Scharfetter-Gummel operator and the transposed (adjoint) operator and solve.
+
+Dense checks assemble the operator column by column through mps_apply and
+solve it with numpy/scipy on small grids.
+"""
+import numpy as np
+import pytest
+
+from mps3d_amd import FieldSolver, ghosted
+from mps3d_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def assemble(s, n):
+    """Dense A from unit vectors through mps_apply (A e_j = column j)."""
+    N = int(np.prod(n))
+    A = np.zeros((N, N))
+    e = np.zeros(n, order="F")
+    for q in range(N):
+        idx = np.unravel_index(q, n, order="F")
+        e[idx] = 1.0
+        ax, _, _ = s.find_AX(ghosted(e))
+        A[:, q] = ax.ravel(order="F")
+        e[idx] = 0.0
+    return A
+
+
+@pytest.mark.parametrize("n", [17, 33, 65])
+def test_aligned_laplace_reproduces_linear_solution(gpu, n):
+    """Laplace deck on 2^k+1 cells: phi = x at cell centres solves the
+    discrete system exactly (tests/Laplace_test/README:1-4)."""
+    c = W.laplace_deck(n)
+    with FieldSolver(c.n, c.length, hierarchy="aligned", max_cycles=30, tol=1e-12, tol_kind="rel") as s:
+        s.set_case(c)
+        u = np.zeros(s.shape, order="F")
+        k, r, h, st = s.solve(u, raise_on_failure=True)
+    x = (np.arange(n) + 0.5) / n
+    assert np.abs(u[1:-1, 1:-1, 1:-1] - x[:, None, None]).max() < 1e-10
+    assert k <= 12  # contraction < 0.1 per cycle
+
+
+def test_aligned_solution_matches_dense_solve(gpu):
+    """The aligned hierarchy converges to the solution of the finest
+    (reference) operator: dense solve of the assembled system."""
+    c = W.varcoef_poisson(9)
+    with FieldSolver(c.n, c.length, hierarchy="aligned", max_cycles=60, tol=1e-13, tol_kind="rel") as s:
+        s.set_case(c)
+        A = assemble(s, c.n)
+        b = s.find_bvec(None, True).ravel(order="F")
+        u = np.zeros(s.shape, order="F")
+        s.solve(u, raise_on_failure=True)
+    ref = np.linalg.solve(A, b)
+    got = u[1:-1, 1:-1, 1:-1].ravel(order="F")
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-11
+
+
+def test_finest_operator_is_the_reference_operator_in_aligned_mode(gpu):
+    c = W.random_cdr((9, 9, 5), seed=8)
+    rng = np.random.default_rng(1)
+    u0 = rng.standard_normal(c.n)
+    P = O.Problem(c.n, c.length, (1, 1, 1), c.bc_codes, c.bc_vals, c.dcoeff, c.velx, c.vely, c.velz, c.reac,
+                  c.source, c.timederiv, c.dt)
+    ax_ref, _, _ = O.apply(P, O.ghosted(u0))
+    with FieldSolver(c.n, c.length, hierarchy="aligned") as s:
+        s.set_case(c)
+        ax, _, _ = s.find_AX(ghosted(u0))
+    assert np.array_equal(ax, ax_ref)
+
+
+@pytest.mark.parametrize("hier,n", [("aligned", 65), ("ref", 64)])
+def test_fp32_defect_correction_matches_fp64(gpu, hier, n):
+    """FP32 smoothing under FP64 defect correction reaches the same 1e-10
+    relative residual and the same solution as the plain FP64 solve."""
+    c = W.varcoef_poisson(n)
+    sol = {}
+    for prec in ("fp64", "fp32dc"):
+        with FieldSolver(c.n, c.length, hierarchy=hier, precision=prec, max_cycles=60, tol=1e-10,
+                         tol_kind="rel") as s:
+            s.set_case(c)
+            u = np.zeros(s.shape, order="F")
+            k, r, h, st = s.solve(u, raise_on_failure=True)
+            assert r <= 1e-10
+            ax, _, _ = s.find_AX(u)
+            b = s.find_bvec(None, True)
+            assert np.linalg.norm(b - ax) / np.linalg.norm(b) <= 1e-10  # the FP64 residual, recomputed
+            sol[prec] = (u[1:-1, 1:-1, 1:-1].copy(), k)
+    a, b = sol["fp32dc"][0], sol["fp64"][0]
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-9
+    assert abs(sol["fp32dc"][1] - sol["fp64"][1]) <= 2
+
+
+@pytest.mark.parametrize("flux", ["upwind", "sg"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32dc"])
+def test_fgmres_with_vcycle_preconditioner(gpu, flux, prec):
+    c = W.mms_cdr((32, 32, 32), (1.0, 1.0, 1.0), dt=0.1)
+    with FieldSolver(c.n, c.length, flux=flux, precision=prec) as s:
+        s.set_case(c)
+        b = s.find_bvec(None, True)
+        x = np.zeros(s.shape, order="F")
+        it, rr, st = s.gmres(b, x, m=30, restarts=10, tol=1e-10, precond=1, raise_on_failure=True)
+        ax, _, _ = s.find_AX(x)
+    assert np.linalg.norm(b - ax) / np.linalg.norm(b) <= 1e-10
+    assert it <= 30
+
+
+def test_sg_operator_rows_follow_the_1d_flux(gpu):
+    """A field varying only along x: the SG rows equal the 1D
+    find_sg_exponential_flux weights (mps1d/source/convdiff.f90:10-64) of each
+    face, divided by h, with the reference boundary rows at the ends."""
+    n = (12, 3, 3)
+    L = (1.0, 0.3, 0.3)
+    rng = np.random.default_rng(4)
+    vx = np.broadcast_to(rng.uniform(-30, 30, n[0])[:, None, None], n).copy(order="F")
+    D = np.broadcast_to(rng.uniform(0.5, 2.0, n[0])[:, None, None], n).copy(order="F")
+    u = np.broadcast_to(rng.standard_normal(n[0])[:, None, None], n).copy(order="F")
+    with FieldSolver(n, L, flux="sg") as s:
+        s.set_bc(("ZGRD",) * 6)
+        s.set_coeffs(dcoeff=D, velx=vx, vely=np.zeros(n), velz=np.zeros(n))
+        ax, _, _ = s.find_AX(ghosted(u))
+    h = L[0] / n[0]
+    ref = np.zeros(n[0])
+    for i in range(n[0] - 1):
+        wl, wr, f = O.sg_flux(vx[i, 0, 0], vx[i + 1, 0, 0], D[i, 0, 0], D[i + 1, 0, 0], u[i, 0, 0], u[i + 1, 0, 0], h)
+        ref[i] += f / h
+        ref[i + 1] -= f / h
+    # ZGRD boundary rows: convective outflow with the cell's own velocity
+    ref[0] += -vx[0, 0, 0] * u[0, 0, 0] / h
+    ref[-1] += vx[-1, 0, 0] * u[-1, 0, 0] / h
+    got = ax[:, 1, 1]
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_sg_is_exact_for_the_1d_boundary_layer(gpu):
+    """Steady 1D convection-diffusion with constant (c, D): the SG flux is
+    nodally exact, so the interior rows vanish on the exact exponential
+    profile (mps1d/source/pde_driver/blayer/blayer.f90:72-132)."""
+    n = (32, 3, 3)
+    L = (1.0, 0.1, 0.1)
+    cvel, D = 40.0, 1.0
+    h = L[0] / n[0]
+    x = (np.arange(n[0]) + 0.5) * h
+    prof = (np.exp(cvel * x / D) - 1.0) / (np.exp(cvel / D) - 1.0)
+    u = np.broadcast_to(prof[:, None, None], n).copy(order="F")
+    with FieldSolver(n, L, flux="sg") as s:
+        s.set_bc(("ZGRD",) * 6)
+        s.set_coeffs(dcoeff=np.full(n, D), velx=np.full(n, cvel), vely=np.zeros(n), velz=np.zeros(n))
+        ax, _, _ = s.find_AX(ghosted(u))
+    interior = ax[1:-1, 1, 1]
+    scale = cvel * prof.max() / h
+    assert np.abs(interior).max() < 1e-10 * scale
+
+
+@pytest.mark.parametrize("flux", ["upwind", "sg"])
+def test_transpose_is_the_adjoint(gpu, flux):
+    c = W.random_cdr((8, 8, 4), seed=3)
+    rng = np.random.default_rng(0)
+    with FieldSolver(c.n, c.length, flux=flux) as s:
+        s.set_case(c)
+        A = assemble(s, c.n)
+        y = rng.standard_normal(c.n)
+        aty = s.apply_transpose(ghosted(y))
+    assert np.allclose(aty.ravel(order="F"), A.T @ y.ravel(order="F"), rtol=1e-12, atol=1e-12 * np.abs(A).max())
+
+
+@pytest.mark.parametrize("case,hier", [("varcoef", "ref"), ("cdr", "ref"), ("varcoef_odd", "aligned")])
+def test_adjoint_solve(gpu, case, hier):
+    if case == "varcoef":
+        c = W.varcoef_poisson(32)
+    elif case == "varcoef_odd":
+        c = W.varcoef_poisson(33)
+    else:
+        c = W.mms_cdr((32, 32, 32), (1.0, 1.0, 1.0), dt=0.1)
+    g = np.random.default_rng(2).standard_normal(c.n)
+    with FieldSolver(c.n, c.length, hierarchy=hier, max_cycles=60, tol=1e-10, tol_kind="rel") as s:
+        s.set_case(c)
+        lam = np.zeros(s.shape, order="F")
+        k, r, st = s.solve_transpose(g, lam)
+        atl = s.apply_transpose(lam)
+    assert st == 0
+    assert np.linalg.norm(atl - g) / np.linalg.norm(g) <= 1e-10
