@@ -70,6 +70,15 @@ enum {
                              applied per face in 3D                         */
 };
 enum {
+    MPS_ARITH_EXACT = 0,  /* rows evaluated in the reference's order and
+                             rounding (mgridlib/discretepde.f90:620-1119)   */
+    MPS_ARITH_FAST = 1    /* the same operator with precomputed face factors
+                             and one reciprocal per row, for rows without
+                             convection; rows with convection stay exact.
+                             find_AX/find_bvec/smoothing entry points are
+                             always exact                                   */
+};
+enum {
     MPS_TOL_RMS = 0,      /* RMS of the absolute residual (globalvars.f90:136-159) */
     MPS_TOL_REL_L2 = 1    /* ||b - A u||_2 / ||b||_2                              */
 };
@@ -91,12 +100,13 @@ typedef struct {
                                    mps_solve (reference loops: 20)             */
     double tol;                 /* stopping tolerance for mps_solve            */
     int tol_kind;               /* MPS_TOL_*                                   */
+    int arith;                  /* MPS_ARITH_* for the solver's own rows       */
 } mps_opts;
 
 typedef struct mps_ctx mps_ctx;
 
 /* Fills reference defaults: REF hierarchy, FP64, upwind, nu1=nu2=2,
- * max_cycles=20, tol=1e-11 RMS (globalvars.f90:29). */
+ * max_cycles=20, tol=1e-11 RMS (globalvars.f90:29), exact arithmetic. */
 void mps_default_opts(mps_opts *opts);
 
 /* Allocates a context for one grid on the current HIP device. Replaces the
@@ -173,7 +183,9 @@ int mps_last_timing(mps_ctx *ctx, double *total_ms, int *ncycles);
 /* Measurement hook (no reference counterpart): times `reps` back-to-back
  * launches of one hot kernel of the current hierarchy with HIP events on
  * the context's stream, after the context has solved once.
- *   kernel 0  red-black colour pass on the finest level
+ *   kernel 0  one smoothing sweep on the finest level: the fused red-black
+ *             sweep where the operator has no convection term, else one
+ *             colour pass
  *   kernel 1  residual b - A u on the finest level
  *   kernel 2  restriction of the finest residual to level 1
  *   kernel 3  prolongation + correction from level 1 to the finest level

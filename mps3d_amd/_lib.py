@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libmps3d_hip.so")
+LIB_PATH = os.environ.get("MPS3D_AMD_LIB") or os.path.join(HERE, "lib", "libmps3d_hip.so")
 
 # status codes
 OK, NOT_CONVERGED, ERR_NAN, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOLIB = 0, 1, -1, -2, -3, -4, -5
@@ -21,6 +21,7 @@ HIER_REF, HIER_ALIGNED = 0, 1
 PREC_FP64, PREC_FP32_DC = 0, 1
 FLUX_UPWIND, FLUX_SG = 0, 1
 TOL_RMS, TOL_REL_L2 = 0, 1
+ARITH_EXACT, ARITH_FAST = 0, 1
 
 # every entry point declared in include/mps3d.h
 EXPORTS = (
@@ -40,7 +41,7 @@ class Grid(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("hierarchy", C.c_int), ("precision", C.c_int), ("flux", C.c_int),
                 ("nu1", C.c_int), ("nu2", C.c_int), ("max_cycles", C.c_int),
-                ("tol", C.c_double), ("tol_kind", C.c_int)]
+                ("tol", C.c_double), ("tol_kind", C.c_int), ("arith", C.c_int)]
 
 
 _dp = C.POINTER(C.c_double)

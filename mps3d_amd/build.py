@@ -35,18 +35,21 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
-    os.makedirs(OUT_DIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, SRC]
+def build(verbose=False, out=None, defines=()):
+    """Compiles the library; `out` and `defines` (e.g. "MPS_SWEEP_TY=8") build
+    a variant elsewhere for tuning."""
+    target = out or LIB
+    os.makedirs(os.path.dirname(target), exist_ok=True)
+    tmp = target + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-D" + d for d in defines] + ["-o", tmp, SRC]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stderr)
         raise RuntimeError("hipcc failed building %s" % SRC)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, target)
+    return target
 
 
 if __name__ == "__main__":

@@ -162,6 +162,7 @@ struct mps_ctx {
     double *dinv = nullptr;              // dense inverse of the coarsest operator
     double *dinvT = nullptr;             // and of its transpose
     double *gbuf = nullptr;              // ghosted right-hand side of adjoint solves
+    bool exact_now = false;              // reference-mirroring call in progress
     bool mg_ok = true;                   // a multigrid hierarchy exists
     std::string mg_error;                // why not, when it does not
 
@@ -210,7 +211,22 @@ Lev<T> view(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op_o
     for (int f = 0; f < 6; ++f) v.bc[f] = c->bc[f];
     v.timederiv = c->timederiv ? 1 : 0;
     v.dt = (T)c->dt;
-    (void)op_override;
+    // fast arithmetic: same operator with precomputed factors
+    v.fast = (c->opts.arith == MPS_ARITH_FAST && !c->has_vel && !c->exact_now) ? 1 : 0;
+    v.hx = (T)(0.5 * g.ax / g.dx);
+    v.hy = (T)(0.5 * g.ay / g.dy);
+    v.hz = (T)(0.5 * g.az / g.dz);
+    v.hxb = (T)(0.5 * g.axb / g.dx);
+    v.hyb = (T)(0.5 * g.ayb / g.dy);
+    v.hzb = (T)(0.5 * g.azb / g.dz);
+    const bool ref_geom = c->geom.size() == 1 || c->opts.hierarchy == MPS_HIER_REF || l == 0;
+    v.gx = (T)(ref_geom ? 2.0 / (g.dx * g.dx) : g.ix0 * g.ax);
+    v.gy = (T)(ref_geom ? 2.0 / (g.dy * g.dy) : g.iy0 * g.ay);
+    v.gz = (T)(ref_geom ? 2.0 / (g.dz * g.dz) : g.iz0 * g.az);
+    v.gxb = (T)(g.ix0 * g.axb);
+    v.gyb = (T)(g.iy0 * g.ayb);
+    v.gzb = (T)(g.iz0 * g.azb);
+    v.dtinv = c->timederiv ? (T)(1.0 / c->dt) : T(0);
     return v;
 }
 
@@ -519,7 +535,16 @@ void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *wo
 }
 
 // fused red-black sweep u -> u2 (see mps3d_sweep.h)
-constexpr int kSweepTX = 64, kSweepTY = 4, kSweepKZ = 32;
+#ifndef MPS_SWEEP_TX
+#define MPS_SWEEP_TX 32
+#endif
+#ifndef MPS_SWEEP_TY
+#define MPS_SWEEP_TY 8
+#endif
+#ifndef MPS_SWEEP_KZ
+#define MPS_SWEEP_KZ 16
+#endif
+constexpr int kSweepTX = MPS_SWEEP_TX, kSweepTY = MPS_SWEEP_TY, kSweepKZ = MPS_SWEEP_KZ;
 
 template <typename T>
 void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, T *uout, int op)
@@ -529,8 +554,12 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
     dim3 gr((unsigned)((g.nx + kSweepTX - 1) / kSweepTX), (unsigned)((g.ny + kSweepTY - 1) / kSweepTY),
             (unsigned)((g.nz + kSweepKZ - 1) / kSweepKZ));
     auto go = [&](auto O, auto R) {
-        hipLaunchKernelGGL((k_sweep<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
-                           dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kSweepKZ);
+        if (v.fast)
+            hipLaunchKernelGGL((k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
+                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kSweepKZ);
+        else
+            hipLaunchKernelGGL((k_sweep<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
+                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kSweepKZ);
     };
     if (op == OP_REF) {
         if (c->has_reac) go(IntC<OP_REF>{}, BoolC<true>{});
@@ -541,13 +570,19 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
     }
 }
 
+template <typename T>
+bool fused_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op)
+{
+    return op != OP_STENCIL && !c->has_vel && L[l].u2;
+}
+
 // nu red-black sweeps on level l: pairs of fused sweeps u -> u2 -> u where
 // the operator has no convection, in-place colour passes otherwise
 template <typename T>
 void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op)
 {
     int s = 0;
-    if (op != OP_STENCIL && !c->has_vel && L[l].u2) {
+    if (fused_ok(c, L, l, op)) {
         for (; s + 2 <= nu; s += 2) {
             launch_sweep<T>(c, L, l, L[l].u, L[l].u2, op);
             launch_sweep<T>(c, L, l, L[l].u2, L[l].u, op);
@@ -679,6 +714,14 @@ void build_fp32(mps_ctx *c)
         }
     }
 }
+
+// reference-mirroring entry points (find_AX, find_bvec, smoothing) always
+// evaluate rows in the reference's arithmetic
+struct ExactScope {
+    mps_ctx *c;
+    explicit ExactScope(mps_ctx *c_) : c(c_) { c->exact_now = true; }
+    ~ExactScope() { c->exact_now = false; }
+};
 
 // --------------------------------------------------------------- helpers
 int guard(mps_ctx *c, bool need_coeffs = true)
@@ -970,6 +1013,7 @@ void mps_default_opts(mps_opts *o)
     o->max_cycles = 20;
     o->tol = 1e-11;
     o->tol_kind = MPS_TOL_RMS;
+    o->arith = MPS_ARITH_EXACT;
 }
 
 const char *mps_error_string(int s)
@@ -1065,6 +1109,7 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
         c->d_scalar = dalloc<double>(4);
         HIPCHK(hipHostMalloc((void **)&c->h_scalar, 4 * sizeof(double), hipHostMallocDefault));
         c->d_status = dalloc<int>(1);
+        fill<int>(c, c->d_status, 1, 0);
         if (c->mg_ok) {
             const Geom &gb = c->geom.back();
             c->bottom64 = dalloc<double>(bottom_work_elems<double>((int)gb.n(), gb.ng()));
@@ -1243,6 +1288,7 @@ int mps_apply(mps_ctx *c, const double *x, double *ax, double *diag, double *off
 {
     int st = guard(c);
     if (st) return st;
+    ExactScope exact(c);
     if (!x) return fail(MPS_ERR_ARG, "null x");
     try {
         const Geom &g = c->geom[0];
@@ -1282,6 +1328,7 @@ int mps_bvec(mps_ctx *c, const double *sterm, int applybc, double *b)
 {
     int st = guard(c);
     if (st) return st;
+    ExactScope exact(c);
     if (!b) return fail(MPS_ERR_ARG, "null b");
     try {
         const Geom &g = c->geom[0];
@@ -1307,6 +1354,7 @@ int mps_smooth(mps_ctx *c, double *soln, const double *sterm, int niter, double 
 {
     int st = guard(c);
     if (st) return st;
+    ExactScope exact(c);
     if (!soln || niter < 0) return fail(MPS_ERR_ARG, "bad arguments");
     try {
         const Geom &g = c->geom[0];
@@ -1366,6 +1414,7 @@ int mps_vcycle(mps_ctx *c, double *soln, const double *sterm, double *resnorm)
     try {
         bool staged;
         double *u = bind_solution(c, soln, staged);
+        fill<int>(c, c->d_status, 1, 0);
         compute_b0(c, stage_interior(c, sterm));
         fp64_cycle(c, u);
         double ss = c->geom[0].bottom ? 0.0 : read_scalar(c);
@@ -1393,6 +1442,7 @@ int mps_solve(mps_ctx *c, double *soln, int *ncycles, double *resnorm, double *h
         bool staged;
         double *u = bind_solution(c, soln, staged);
         HIPCHK(hipEventRecord(c->ev0, c->stream));
+        fill<int>(c, c->d_status, 1, 0);
         compute_b0(c, nullptr);
         double meas = 0.0;
         const int it = solve_core(c, u, c->b0, false, history, meas);
@@ -1858,7 +1908,12 @@ int time_kernel_t(mps_ctx *c, std::vector<LevelBuf<T>> &L, int kernel, int reps,
         L[0].b = (T *)c->b0;
     }
     std::function<void()> launch;
-    if (kernel == 0) {
+    if (kernel == 0 && fused_ok(c, L, 0, op)) {
+        // one fused red+black sweep u -> u2 (both colours: (3 + 2c) w per cell)
+        launch = [&]() { launch_sweep<T>(c, L, 0, L[0].u, L[0].u2, op); };
+        *bytes = (3.0 + 2.0 * cf) * w * N;
+        name = view(c, L, 0).fast ? "k_sweep_fast" : "k_sweep";
+    } else if (kernel == 0) {
         launch = [&]() { launch_colour<T>(c, L, 0, 1, op); };
         *bytes = (1.5 + cf) * w * N;  // other colour u (1/2), own b (1/2), coefficients; own u write (1/2)
         name = "k_colour";

@@ -44,6 +44,12 @@ struct Lev {
     int bc[6];
     int timederiv;
     T dt;
+    // fast arithmetic (no convection): face weight = (D_L + D_R) * h*,
+    // h* = abv/(2 d); DIRC boundary diagonal = D_c * g*; 1/dt precomputed.
+    int fast;
+    T hx, hy, hz, hxb, hyb, hzb;        // interior / boundary-cell face factors
+    T gx, gy, gz, gxb, gyb, gzb;        // DIRC wall factors (interior unused)
+    T dtinv;
 };
 
 // Per-level, per-dimension restriction weights of the aligned hierarchy:

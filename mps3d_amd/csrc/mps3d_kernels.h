@@ -123,6 +123,40 @@ template <typename T, int OP, bool REAC>
 __device__ __forceinline__ void row7(const Lev<T> &L, int i, int j, int k, T uxm, T uxp, T uym, T uyp, T uzm,
                                      T uzp, T Dc, T Dxm, T Dxp, T Dym, T Dyp, T Dzm, T Dzp, T rc, T &dg, T &of)
 {
+    if (L.fast) {
+#pragma clang fp contract(fast)
+        // same operator, precomputed factors: w = (D_L + D_R) * abv / (2 d)
+        T d = T(0), o = T(0);
+        {
+            const bool edge = (OP == OP_ALIGNED) && ((i == 1) || (i == L.nx));
+            const T h = edge ? L.hxb : L.hx;
+            if (i > 1) { const T w = (Dxm + Dc) * h; d += w; o -= w * uxm; }
+            if (i < L.nx) { const T w = (Dc + Dxp) * h; d += w; o -= w * uxp; }
+            if (i == 1 && L.bc[LEFT] == BC_DIRC) d += Dc * (edge ? L.gxb : L.gx);
+            if (i == L.nx && L.bc[RIGHT] == BC_DIRC) d += Dc * (edge ? L.gxb : L.gx);
+        }
+        {
+            const bool edge = (OP == OP_ALIGNED) && ((j == 1) || (j == L.ny));
+            const T h = edge ? L.hyb : L.hy;
+            if (j > 1) { const T w = (Dym + Dc) * h; d += w; o -= w * uym; }
+            if (j < L.ny) { const T w = (Dc + Dyp) * h; d += w; o -= w * uyp; }
+            if (j == 1 && L.bc[BOTTOM] == BC_DIRC) d += Dc * (edge ? L.gyb : L.gy);
+            if (j == L.ny && L.bc[TOP] == BC_DIRC) d += Dc * (edge ? L.gyb : L.gy);
+        }
+        {
+            const bool edge = (OP == OP_ALIGNED) && ((k == 1) || (k == L.nz));
+            const T h = edge ? L.hzb : L.hz;
+            if (k > 1) { const T w = (Dzm + Dc) * h; d += w; o -= w * uzm; }
+            if (k < L.nz) { const T w = (Dc + Dzp) * h; d += w; o -= w * uzp; }
+            if (k == 1 && L.bc[BACK] == BC_DIRC) d += Dc * (edge ? L.gzb : L.gz);
+            if (k == L.nz && L.bc[FRONT] == BC_DIRC) d += Dc * (edge ? L.gzb : L.gz);
+        }
+        if (REAC) d -= rc;
+        d += L.dtinv;
+        dg = d;
+        of = o;
+        return;
+    }
     T d = T(0), o = T(0);
     {
         const bool edge = (i == 1) || (i == L.nx);
@@ -152,6 +186,17 @@ __device__ __forceinline__ void row7(const Lev<T> &L, int i, int j, int k, T uxm
     if (L.timederiv) d = d + T(1) / L.dt;
     dg = d;
     of = o;
+}
+
+// Gauss-Seidel update (b - off)/diag; the fast arithmetic mode uses the
+// hardware reciprocal in FP32.
+template <typename T>
+__device__ __forceinline__ T gs_update(const Lev<T> &L, T b, T of, T dg)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (L.fast) return (b - of) * __builtin_amdgcn_rcpf(dg);
+    }
+    return (b - of) / dg;
 }
 
 // diag and offdiag of cell (i,j,k) at address c for the current u.
@@ -316,7 +361,7 @@ __global__ void __launch_bounds__(kBlock) k_colour(Lev<T> L, T *__restrict__ u, 
     if (((i + j + k) & 1) != red) return;
     T dg, of;
     row<T, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
-    u[c] = (b[c] - of) / dg;
+    u[c] = gs_update(L, b[c], of, dg);
 }
 
 // r = b - (diag*u + off); writes r (type R, ghosted layout, nullable) and the

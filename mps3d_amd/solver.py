@@ -66,7 +66,7 @@ class FieldSolver:
     """One grid on one GPU (an ``mps_ctx``)."""
 
     def __init__(self, n, length=(1.0, 1.0, 1.0), p=(1, 1, 1), hierarchy="ref", precision="fp64",
-                 flux="upwind", nu1=2, nu2=2, max_cycles=20, tol=1e-11, tol_kind="rms"):
+                 flux="upwind", nu1=2, nu2=2, max_cycles=20, tol=1e-11, tol_kind="rms", arith="exact"):
         self.lib = M.load()
         self.n = tuple(int(v) for v in (n if not np.isscalar(n) else (n, n, n)))
         self.length = tuple(float(v) for v in (length if not np.isscalar(length) else (length,) * 3))
@@ -80,6 +80,7 @@ class FieldSolver:
         o.nu1, o.nu2, o.max_cycles = int(nu1), int(nu2), int(max_cycles)
         o.tol = float(tol)
         o.tol_kind = _TOL[tol_kind] if isinstance(tol_kind, str) else int(tol_kind)
+        o.arith = {"exact": M.ARITH_EXACT, "fast": M.ARITH_FAST}[arith] if isinstance(arith, str) else int(arith)
         self.opts = o
         self.max_cycles = int(max_cycles)
         h = C.c_void_p()
