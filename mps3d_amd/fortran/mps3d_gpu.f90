@@ -1,0 +1,204 @@
+! ISO_C_BINDING interface to the mps3d_amd HIP library (include/mps3d.h).
+!
+! The reference's equation modules call perform_vcycle
+! (mgridlib/mgridsteps.f90:384-385) in a loop of at most nvcycles cycles
+! (plasma_pde/poissoneq.f90:427-450, generic_pde/pdedefs.f90:378-402).
+! gpu_perform_vcycle keeps that argument list -- with the six bcvals faces
+! passed as arrays, since the boundarycondition type of
+! mgridlib/globalvars.f90:57-66 lives in the reference tree -- and
+! gpu_solve replaces the whole loop with one device-resident call.
+! Solution and coefficient arrays are the reference's ghosted real*8
+! arrays (0:lx+1, 0:ly+1, 0:lz+1); they are passed by address, unchanged.
+module mps3d_gpu
+
+    use iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: MPS_OK = 0, MPS_NOT_CONVERGED = 1, MPS_ERR_NAN = -1
+    integer(c_int), parameter :: MPS_ERR_ARG = -2, MPS_ERR_HIP = -3, MPS_ERR_STATE = -4, MPS_ERR_NOLIB = -5
+    integer(c_int), parameter :: MPS_HIER_REF = 0, MPS_HIER_ALIGNED = 1
+    integer(c_int), parameter :: MPS_PREC_FP64 = 0, MPS_PREC_FP32_DC = 1
+    integer(c_int), parameter :: MPS_FLUX_UPWIND = 0, MPS_FLUX_SG = 1
+    integer(c_int), parameter :: MPS_TOL_RMS = 0, MPS_TOL_REL_L2 = 1
+    integer(c_int), parameter :: MPS_ARITH_EXACT = 0, MPS_ARITH_FAST = 1
+
+    type, bind(C) :: mps_grid
+        integer(c_int) :: nx, ny, nz
+        real(c_double) :: lenx, leny, lenz
+        integer(c_int) :: px, py, pz
+    end type mps_grid
+
+    type, bind(C) :: mps_opts
+        integer(c_int) :: hierarchy, precision, flux, nu1, nu2, max_cycles
+        real(c_double) :: tol
+        integer(c_int) :: tol_kind, arith
+    end type mps_opts
+
+    interface
+        subroutine mps_default_opts(opts) bind(C, name='mps_default_opts')
+            import :: mps_opts
+            type(mps_opts), intent(out) :: opts
+        end subroutine mps_default_opts
+
+        integer(c_int) function mps_create(ctx, grid, opts) bind(C, name='mps_create')
+            import :: c_int, c_ptr, mps_grid, mps_opts
+            type(c_ptr), intent(out) :: ctx
+            type(mps_grid), intent(in) :: grid
+            type(mps_opts), intent(in) :: opts
+        end function mps_create
+
+        integer(c_int) function mps_destroy(ctx) bind(C, name='mps_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function mps_destroy
+
+        integer(c_int) function mps_set_bc_str(ctx, codes, faces) bind(C, name='mps_set_bc_str')
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: ctx
+            character(kind=c_char), dimension(*), intent(in) :: codes
+            type(c_ptr), dimension(6), intent(in) :: faces
+        end function mps_set_bc_str
+
+        integer(c_int) function mps_set_coeffs(ctx, dcoeff, velx, vely, velz, reac, source, timederiv, dt) &
+                bind(C, name='mps_set_coeffs')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, dcoeff, velx, vely, velz, reac, source
+            integer(c_int), value :: timederiv
+            real(c_double), value :: dt
+        end function mps_set_coeffs
+
+        integer(c_int) function mps_vcycle(ctx, soln, mgridsterm, resnorm) bind(C, name='mps_vcycle')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, soln, mgridsterm
+            real(c_double), intent(out) :: resnorm
+        end function mps_vcycle
+
+        integer(c_int) function mps_solve(ctx, soln, ncycles, resnorm, history) bind(C, name='mps_solve')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, soln, history
+            integer(c_int), intent(out) :: ncycles
+            real(c_double), intent(out) :: resnorm
+        end function mps_solve
+
+        integer(c_int) function mps_fill_ghosts(ctx, soln) bind(C, name='mps_fill_ghosts')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, soln
+        end function mps_fill_ghosts
+
+        type(c_ptr) function mps_last_error_c() bind(C, name='mps_last_error')
+            import :: c_ptr
+        end function mps_last_error_c
+    end interface
+
+contains
+
+    ! context for the grid and process layout of the reference's globals
+    ! (g_Nx.., g_xlen.., g_px.. of mgridlib/globalvars.f90:33-52)
+    integer function gpu_open(ctx, nx, ny, nz, xlen, ylen, zlen, px, py, pz, opts) result(status)
+        type(c_ptr), intent(out) :: ctx
+        integer, intent(in) :: nx, ny, nz, px, py, pz
+        real*8, intent(in) :: xlen, ylen, zlen
+        type(mps_opts), intent(in) :: opts
+        type(mps_grid) :: g
+        g%nx = nx
+        g%ny = ny
+        g%nz = nz
+        g%lenx = xlen
+        g%leny = ylen
+        g%lenz = zlen
+        g%px = px
+        g%py = py
+        g%pz = pz
+        status = mps_create(ctx, g, opts)
+    end function gpu_open
+
+    ! boundary codes and faces, coefficients (as perform_vcycle receives them)
+    integer function gpu_set_problem(ctx, lx, ly, lz, timederivflag, dt, vel, dcoeff, reac, source, &
+            bc_codes, left, right, bottom, top, back, front) result(status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: lx, ly, lz
+        logical, intent(in) :: timederivflag
+        real*8, intent(in) :: dt
+        real*8, intent(in), target :: vel(0:lx+1, 0:ly+1, 0:lz+1, 3)
+        real*8, intent(in), target :: dcoeff(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in), target :: reac(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in), target :: source(0:lx+1, 0:ly+1, 0:lz+1)
+        character(len=4), intent(in) :: bc_codes(6)
+        real*8, intent(in), target :: left(ly, lz), right(ly, lz), bottom(lx, lz), top(lx, lz)
+        real*8, intent(in), target :: back(lx, ly), front(lx, ly)
+        character(kind=c_char, len=24) :: codes
+        type(c_ptr) :: faces(6)
+        integer :: f
+        integer(c_int) :: td
+        do f = 1, 6
+            codes(4*f-3:4*f) = bc_codes(f)
+        enddo
+        faces(1) = c_loc(left)
+        faces(2) = c_loc(right)
+        faces(3) = c_loc(bottom)
+        faces(4) = c_loc(top)
+        faces(5) = c_loc(back)
+        faces(6) = c_loc(front)
+        status = mps_set_bc_str(ctx, codes, faces)
+        if (status /= MPS_OK) return
+        td = 0
+        if (timederivflag) td = 1
+        status = mps_set_coeffs(ctx, c_loc(dcoeff), c_loc(vel(0, 0, 0, 1)), c_loc(vel(0, 0, 0, 2)), &
+            c_loc(vel(0, 0, 0, 3)), c_loc(reac), c_loc(source), td, dt)
+    end function gpu_set_problem
+
+    ! perform_vcycle (mgridlib/mgridsteps.f90:384-579) on the finest level,
+    ! argument for argument; the coefficients are set up on every call, as
+    ! the reference restricts them on every call
+    subroutine gpu_perform_vcycle(ctx, solnvar, timederivflag, dt, lx, ly, lz, vel, dcoeff, reac, source, &
+            bc_codes, left, right, bottom, top, back, front, mgridsterm, resnorm, status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: lx, ly, lz
+        real*8, intent(inout), target :: solnvar(0:lx+1, 0:ly+1, 0:lz+1)
+        logical, intent(in) :: timederivflag
+        real*8, intent(in) :: dt
+        real*8, intent(in), target :: vel(0:lx+1, 0:ly+1, 0:lz+1, 3)
+        real*8, intent(in), target :: dcoeff(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in), target :: reac(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in), target :: source(0:lx+1, 0:ly+1, 0:lz+1)
+        character(len=4), intent(in) :: bc_codes(6)
+        real*8, intent(in), target :: left(ly, lz), right(ly, lz), bottom(lx, lz), top(lx, lz)
+        real*8, intent(in), target :: back(lx, ly), front(lx, ly)
+        real*8, intent(in), target :: mgridsterm(lx, ly, lz)
+        real*8, intent(out) :: resnorm
+        integer, intent(out) :: status
+        status = gpu_set_problem(ctx, lx, ly, lz, timederivflag, dt, vel, dcoeff, reac, source, &
+            bc_codes, left, right, bottom, top, back, front)
+        if (status /= MPS_OK) return
+        status = mps_vcycle(ctx, c_loc(solnvar), c_loc(mgridsterm), resnorm)
+    end subroutine gpu_perform_vcycle
+
+    ! the equation modules' loop of V-cycles in one call: set the problem
+    ! once with gpu_set_problem, then solve to the context's tolerance
+    subroutine gpu_solve(ctx, solnvar, lx, ly, lz, ncycles, resnorm, history, status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: lx, ly, lz
+        real*8, intent(inout), target :: solnvar(0:lx+1, 0:ly+1, 0:lz+1)
+        integer, intent(out) :: ncycles
+        real*8, intent(out) :: resnorm
+        real*8, intent(out), target :: history(*)
+        integer, intent(out) :: status
+        integer(c_int) :: n
+        status = mps_solve(ctx, c_loc(solnvar), n, resnorm, c_loc(history))
+        ncycles = n
+    end subroutine gpu_solve
+
+    ! text of the library's last error
+    function gpu_last_error() result(msg)
+        character(len=256) :: msg
+        character(kind=c_char), pointer :: p(:)
+        integer :: k
+        call c_f_pointer(mps_last_error_c(), p, [256])
+        msg = ' '
+        do k = 1, 256
+            if (p(k) == c_null_char) exit
+            msg(k:k) = p(k)
+        enddo
+    end function gpu_last_error
+
+end module mps3d_gpu
