@@ -176,6 +176,13 @@ int mps_gmres(mps_ctx *ctx, const double *b, double *x, int m, int max_restarts,
 int mps_solve_transpose(mps_ctx *ctx, const double *g, double *lambda, int *ncycles,
                         double *resnorm);
 
+/* Adjoint sensitivity to the cell velocities (no reference counterpart):
+ * gv_d(c) = -d(lam^T A(v) x)/dv_d(c) for d = x, y, z (interior outputs),
+ * with lam and x ghosted. For x = A^-1 b and lam = A^-T dJ/dx this is dJ/dv
+ * through the operator (upwind or SG fluxes, boundary rows included). */
+int mps_grad_velocity(mps_ctx *ctx, const double *lam, const double *x, double *gvx, double *gvy,
+                      double *gvz);
+
 /* Timing of the last mps_solve / mps_solve_transpose on the context's stream
  * (HIP events): total milliseconds and per-cycle milliseconds. */
 int mps_last_timing(mps_ctx *ctx, double *total_ms, int *ncycles);

@@ -28,7 +28,7 @@ EXPORTS = (
     "mps_default_opts", "mps_create", "mps_destroy", "mps_set_bc", "mps_set_bc_str", "mps_set_coeffs",
     "mps_vcycle", "mps_solve", "mps_apply", "mps_apply_transpose", "mps_bvec", "mps_smooth",
     "mps_fill_ghosts", "mps_gmres", "mps_solve_transpose", "mps_last_timing", "mps_stream",
-    "mps_error_string", "mps_last_error", "mps_time_kernel",
+    "mps_error_string", "mps_last_error", "mps_time_kernel", "mps_grad_velocity",
 )
 
 
@@ -81,6 +81,7 @@ def load():
     L.mps_last_timing.argtypes = [_vp, _dp, C.POINTER(C.c_int)]
     L.mps_stream.argtypes = [_vp]
     L.mps_stream.restype = _vp
+    L.mps_grad_velocity.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
     L.mps_time_kernel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_char_p, C.c_int]
     L.mps_error_string.argtypes = [C.c_int]
     L.mps_error_string.restype = C.c_char_p

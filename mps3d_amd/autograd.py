@@ -1,0 +1,128 @@
+"""Differentiable linear solves through the hand-written adjoint.
+
+``LinearSolve`` is a thin ctypes ``torch.autograd.Function``: its forward is
+one multigrid solve ``x = A(v)^-1 b`` (``mps_solve``); its backward is the
+adjoint solve ``lam = A(v)^-T xbar`` (``mps_solve_transpose``: the transposed
+operator with the transposed V-cycle, or the forward cycle itself for the
+self-adjoint diffusion operator) plus the velocity sensitivity
+``-lam^T (dA/dv) x`` (``mps_grad_velocity``). Gradients flow to the
+right-hand side and to the three cell-velocity fields; everything else about
+the operator (diffusion coefficient, reaction, time step, BC codes) is fixed
+per :class:`Operator`. Dirichlet or flux boundary data enter through the
+right-hand side, where torch differentiates them.
+
+Tensors are float64 on the GPU. Interior fields have shape (nz, ny, nx)
+(C order, i.e. the reference's Fortran (nx, ny, nz) layout); the library's
+ghosted arrays are (nz+2, ny+2, nx+2).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib as M
+from .solver import FieldSolver
+
+
+def pad(t):
+    """Interior (nz,ny,nx) -> zero-ghost (nz+2,ny+2,nx+2), contiguous."""
+    return torch.nn.functional.pad(t, (1, 1, 1, 1, 1, 1)).contiguous()
+
+
+def interior(g):
+    return g[1:-1, 1:-1, 1:-1]
+
+
+class Operator:
+    """A fixed operator form for LinearSolve: grid, hierarchy, flux scheme,
+    diffusion coefficient, reaction, time term and BC codes (homogeneous
+    faces). Velocities and right-hand sides are supplied per solve."""
+
+    def __init__(self, n, length, dcoeff, bc_codes, flux="upwind", reac=None, timederiv=False, dt=1.0,
+                 hierarchy="ref", p=(1, 1, 1), tol=1e-13, max_cycles=80, precision="fp64", cycles=None):
+        """cycles: a fixed number of V-cycles per solve instead of a
+        tolerance, which makes the discrete solution map a smooth function
+        of the inputs (needed when finite differences probe it)."""
+        self.n = tuple(int(v) for v in n)
+        self.fixed = cycles is not None
+        if self.fixed:
+            tol, max_cycles = 0.0, int(cycles)
+        self.solver = FieldSolver(self.n, length, p=p, hierarchy=hierarchy, precision=precision, flux=flux,
+                                  max_cycles=max_cycles, tol=tol, tol_kind="rel")
+        self.solver.set_bc(bc_codes, (0.0,) * 6)
+        self.D = pad(dcoeff.to(torch.float64))
+        self.reac = None if reac is None else pad(reac.to(torch.float64))
+        self.timederiv, self.dt = bool(timederiv), float(dt)
+        self._zero = torch.zeros_like(self.D)
+        self.cycles = []
+
+    def configure(self, vel=None, rhs=None):
+        """Sets the operator for velocities vel (3 interior tensors or None)
+        and the right-hand side rhs (interior, or None for zero)."""
+        if vel is None:
+            vg = (None, None, None)
+        else:
+            vg = tuple(pad(v.detach()) if v is not None else self._zero for v in vel)
+        src = self._zero if rhs is None else pad(rhs.detach())
+        self._keep = (vg, src)
+        self.solver.set_coeffs(self.D, *vg, self.reac, src, self.timederiv, self.dt)
+
+    def solve(self, x0=None):
+        x = torch.zeros_like(self.D) if x0 is None else pad(x0.detach())
+        k, r, _, st = self.solver.solve(x)
+        if st != M.OK and not (self.fixed and st == M.NOT_CONVERGED):
+            raise M.MpsError(st, "forward solve did not converge (%g after %d cycles)" % (r, k))
+        self.cycles.append(k)
+        return x
+
+    def solve_transpose(self, g):
+        lam = torch.zeros_like(self.D)
+        k, r, st = self.solver.solve_transpose(g.detach().contiguous(), lam)
+        if st != M.OK and not (self.fixed and st == M.NOT_CONVERGED):
+            raise M.MpsError(st, "adjoint solve did not converge (%g after %d cycles)" % (r, k))
+        return lam
+
+    def grad_velocity(self, lam, x):
+        out = [torch.empty(self.n[::-1], dtype=torch.float64, device=lam.device) for _ in range(3)]
+        lib = self.solver.lib
+        M.check(lib.mps_grad_velocity(self.solver._h, C.c_void_p(lam.data_ptr()), C.c_void_p(x.data_ptr()),
+                                      *[C.c_void_p(o.data_ptr()) for o in out]))
+        return out
+
+    def close(self):
+        self.solver.close()
+
+
+class LinearSolve(torch.autograd.Function):
+    """x = A(v)^-1 b with the hand-written adjoint in backward."""
+
+    @staticmethod
+    def forward(ctx, op: Operator, b, vx=None, vy=None, vz=None, x0=None):
+        vel = None if (vx is None and vy is None and vz is None) else (vx, vy, vz)
+        op.configure(vel, b)
+        xg = op.solve(x0)
+        ctx.op = op
+        ctx.has_vel = vel is not None
+        ctx.save_for_backward(xg, *(v if v is not None else torch.empty(0) for v in (vx, vy, vz)))
+        return interior(xg).clone()
+
+    @staticmethod
+    def backward(ctx, xbar):
+        op = ctx.op
+        xg, vx, vy, vz = ctx.saved_tensors
+        vel = None
+        if ctx.has_vel:
+            vel = tuple(v if v.numel() else None for v in (vx, vy, vz))
+        op.configure(vel, None)  # the operator of this solve
+        lam = op.solve_transpose(xbar.contiguous())
+        gb = interior(lam).clone()
+        gv = [None, None, None]
+        if ctx.has_vel and any(ctx.needs_input_grad[2:5]):
+            gv = op.grad_velocity(lam, xg)
+        return None, gb, gv[0], gv[1], gv[2], None
+
+
+def linear_solve(op, b, vel=None, x0=None):
+    vx, vy, vz = vel if vel is not None else (None, None, None)
+    return LinearSolve.apply(op, b, vx, vy, vz, x0)

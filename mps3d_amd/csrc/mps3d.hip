@@ -479,28 +479,103 @@ void launch_colour(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int red, int 
     });
 }
 
+// z-planes per block of the plane-marching kernels: enough blocks to fill
+// the chip several times over on large grids, short serial z chains (each
+// plane waits on one memory round trip) on small ones
+int z_chunk(const Geom &g, int tx, int ty, int target_blocks, int kmax)
+{
+    const long long cols = (long long)((g.nx + tx - 1) / tx) * ((g.ny + ty - 1) / ty);
+    const long long kz = cols * g.nz / target_blocks;
+    return (int)std::max(2LL, std::min((long long)kmax, kz));
+}
+
+constexpr int kResTX = 32, kResTY = 8, kResKZ = 16;
+
+dim3 residual_planes_grid(const Geom &g, int &kz)
+{
+    kz = z_chunk(g, kResTX, kResTY, 4096, kResKZ);
+    return dim3((unsigned)((g.nx + kResTX - 1) / kResTX), (unsigned)((g.ny + kResTY - 1) / kResTY),
+                (unsigned)((g.nz + kz - 1) / kz));
+}
+
+// aligned-hierarchy transfers (plane-marching kernels): restriction of fine
+// level l's r into level l+1's b, prolongation-correction of level l+1's u
+// into level l's u
+constexpr int kRestCX = 32, kRestCY = 8, kProlTX = 64, kProlTY = 4;
+
+template <typename T>
+void launch_restrict_fw(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l)
+{
+    const Geom &gc = c->geom[l + 1];
+    Lev<T> vf = view(c, L, l), vc = view(c, L, l + 1);
+    AlignedW W{L[l + 1].w[0], L[l + 1].w[1], L[l + 1].w[2]};
+    const int kz = z_chunk(gc, kRestCX, kRestCY, 2048, 16);
+    dim3 gr((unsigned)((gc.nx + kRestCX - 1) / kRestCX), (unsigned)((gc.ny + kRestCY - 1) / kRestCY),
+            (unsigned)((gc.nz + kz - 1) / kz));
+    hipLaunchKernelGGL((k_restrict_fw_planes<T, kRestCX, kRestCY>), gr, dim3(kRestCX * kRestCY), 0, c->stream, vf,
+                       vc, W, (const T *)L[l].r, L[l + 1].b, kz);
+}
+
+template <typename T>
+void launch_prolong_tri(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l)
+{
+    const Geom &g = c->geom[l];
+    Lev<T> vf = view(c, L, l), vc = view(c, L, l + 1);
+    const int kz = std::max(2, z_chunk(g, kProlTX, kProlTY, 4096, 16) & ~1);
+    dim3 gr((unsigned)((g.nx + kProlTX - 1) / kProlTX), (unsigned)((g.ny + kProlTY - 1) / kProlTY),
+            (unsigned)((g.nz + kz - 1) / kz));
+    hipLaunchKernelGGL((k_prolong_tri_planes<T, kProlTX, kProlTY>), gr, dim3(kProlTX * kProlTY), 0, c->stream, vf,
+                       vc, L[l].u, (const T *)L[l + 1].u, kz);
+}
+
+// most block partial sums any residual launch on level 0 writes
+int residual_max_partials(const Geom &g)
+{
+    int kz;
+    dim3 a = residual_planes_grid(g, kz), b = red_grid(g);
+    return (int)std::max(a.x * a.y * a.z, b.x * b.y);
+}
+
+// r = b - A u on level l (r nullable) and, with partials, one partial sum of
+// r^2 per block; returns the number of partials written. Rows without
+// convection in fast arithmetic take the plane-marching kernel.
 template <typename T, typename R_>
-void launch_residual(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *u, const T *b, R_ *r, double *partials,
-                     int op, bool reduce_grid = false)
+int launch_residual(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *u, const T *b, R_ *r, double *partials,
+                    int op, bool reduce_grid = false)
 {
     Lev<T> v = view(c, L, l);
     const Geom &g = c->geom[l];
+    if (v.fast && op != OP_STENCIL && !c->has_vel) {
+        int kz;
+        const dim3 gr = residual_planes_grid(g, kz);
+        auto go = [&](auto O, auto Q) {
+            hipLaunchKernelGGL((k_residual_planes<T, R_, decltype(O)::value, decltype(Q)::value, kResTX, kResTY>),
+                               gr, dim3(kResTX * kResTY), 0, c->stream, v, u, b, r, partials, kz);
+        };
+        if (op == OP_REF) {
+            if (c->has_reac) go(IntC<OP_REF>{}, BoolC<true>{});
+            else go(IntC<OP_REF>{}, BoolC<false>{});
+        } else {
+            if (c->has_reac) go(IntC<OP_ALIGNED>{}, BoolC<true>{});
+            else go(IntC<OP_ALIGNED>{}, BoolC<false>{});
+        }
+        return partials ? (int)(gr.x * gr.y * gr.z) : 0;
+    }
+    const dim3 gr = (reduce_grid || partials) ? red_grid(g) : cells_grid(g);
     with_op(op, c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
-        hipLaunchKernelGGL((k_residual<T, R_, decltype(O)::value, decltype(V)::value, decltype(R)::value>),
-                           (reduce_grid || partials) ? red_grid(g) : cells_grid(g), block(), 0, c->stream, v, u, b,
-                           r, partials);
+        hipLaunchKernelGGL((k_residual<T, R_, decltype(O)::value, decltype(V)::value, decltype(R)::value>), gr,
+                           block(), 0, c->stream, v, u, b, r, partials);
     });
+    return partials ? (int)(gr.x * gr.y) : 0;
 }
 
 // sum of squares of the residual of level 0 -> c->d_scalar (device)
 template <typename T>
 void residual_sumsq(mps_ctx *c, std::vector<LevelBuf<T>> &L, const T *u, const T *b, int op, T *r_out = nullptr)
 {
-    const Geom &g = c->geom[0];
-    dim3 gr = red_grid(g);
-    launch_residual<T, T>(c, L, 0, u, b, r_out, c->partials, op, true);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials,
-                       (int)(gr.x * gr.y), c->d_scalar);
+    const int np = launch_residual<T, T>(c, L, 0, u, b, r_out, c->partials, op, true);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials, np,
+                       c->d_scalar);
 }
 
 double read_scalar(mps_ctx *c)
@@ -551,15 +626,16 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
 {
     Lev<T> v = view(c, L, l);
     const Geom &g = c->geom[l];
+    const int kz = z_chunk(g, kSweepTX, kSweepTY, 2048, kSweepKZ);
     dim3 gr((unsigned)((g.nx + kSweepTX - 1) / kSweepTX), (unsigned)((g.ny + kSweepTY - 1) / kSweepTY),
-            (unsigned)((g.nz + kSweepKZ - 1) / kSweepKZ));
+            (unsigned)((g.nz + kz - 1) / kz));
     auto go = [&](auto O, auto R) {
         if (v.fast)
             hipLaunchKernelGGL((k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
-                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kSweepKZ);
+                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
         else
             hipLaunchKernelGGL((k_sweep<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
-                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kSweepKZ);
+                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
     };
     if (op == OP_REF) {
         if (c->has_reac) go(IntC<OP_REF>{}, BoolC<true>{});
@@ -624,13 +700,10 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
         hipLaunchKernelGGL(k_prolong1_correct<T>, cells_grid(gc), block(), 0, c->stream, vf, vc, L[l].u,
                            (const T *)L[l + 1].u, (const T *)L[l + 1].uold, G);
     } else {
-        AlignedW W{L[l + 1].w[0], L[l + 1].w[1], L[l + 1].w[2]};
-        hipLaunchKernelGGL(k_restrict_fw<T>, cells_grid(gc), block(), 0, c->stream, vf, vc, W, (const T *)L[l].r,
-                           L[l + 1].b);
+        launch_restrict_fw<T>(c, L, l);
         fill<T>(c, L[l + 1].u, gc.ng(), T(0));
         vcycle<T>(c, L, l + 1, bwork, homog);
-        hipLaunchKernelGGL(k_prolong_tri_correct<T>, cells_grid(g), block(), 0, c->stream, vf, vc, L[l].u,
-                           (const T *)L[l + 1].u);
+        launch_prolong_tri<T>(c, L, l);
     }
     smooth_level<T>(c, L, l, c->opts.nu2, op);
 }
@@ -828,13 +901,19 @@ void defect_step(mps_ctx *c, double *u, bool do_cycle, const double *b = nullptr
     const Geom &g = c->geom[0];
     Lev<double> v = view(c, c->L64, 0);
     if (!b) b = c->b0;
-    with_op(level_op(c, 0), c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
-        hipLaunchKernelGGL((k_defect32<decltype(O)::value, decltype(V)::value, decltype(R)::value>), red_grid(g),
-                           block(), 0, c->stream, v, (const double *)u, b, F[0].b, c->partials);
-    });
-    dim3 gr = red_grid(g);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials,
-                       (int)(gr.x * gr.y), c->d_scalar);
+    int np;
+    if (v.fast && level_op(c, 0) != OP_STENCIL && !c->has_vel) {
+        np = launch_residual<double, float>(c, c->L64, 0, u, b, F[0].b, c->partials, level_op(c, 0));
+    } else {
+        with_op(level_op(c, 0), c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
+            hipLaunchKernelGGL((k_defect32<decltype(O)::value, decltype(V)::value, decltype(R)::value>),
+                               red_grid(g), block(), 0, c->stream, v, (const double *)u, b, F[0].b, c->partials);
+        });
+        dim3 gr = red_grid(g);
+        np = (int)(gr.x * gr.y);
+    }
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials, np,
+                       c->d_scalar);
     if (!do_cycle) return;
     fill<float>(c, F[0].u, g.ng(), 0.0f);
     vcycle<float>(c, F, 0, c->bottom32, true);
@@ -1103,8 +1182,7 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
         c->b0 = dalloc<double>(g0.ng());
         c->user_tmp = dalloc<double>(g0.ng());
         c->user_tmp2 = dalloc<double>(g0.ng());
-        dim3 gr = red_grid(g0);
-        c->max_partials = (int)(gr.x * gr.y);
+        c->max_partials = residual_max_partials(g0);
         c->partials = dalloc<double>(c->max_partials);
         c->d_scalar = dalloc<double>(4);
         HIPCHK(hipHostMalloc((void **)&c->h_scalar, 4 * sizeof(double), hipHostMallocDefault));
@@ -1684,9 +1762,9 @@ int mps_gmres(mps_ctx *c, const double *b, double *x, int m, int max_restarts, d
         }
         const double bnorm = std::sqrt(bn2);
         auto true_residual = [&]() {  // w = b - A x, returns ||w||
-            launch_residual<double, double>(c, c->L64, 0, u, G.bg, G.w, c->partials, level_op(c, 0));
-            hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials,
-                               (int)(gr.x * gr.y), c->d_scalar);
+            const int np = launch_residual<double, double>(c, c->L64, 0, u, G.bg, G.w, c->partials, level_op(c, 0));
+            hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials, np,
+                               c->d_scalar);
             return std::sqrt(read_scalar(c));
         };
         int total = 0;
@@ -1920,7 +1998,7 @@ int time_kernel_t(mps_ctx *c, std::vector<LevelBuf<T>> &L, int kernel, int reps,
     } else if (kernel == 1) {
         launch = [&]() { launch_residual<T, T>(c, L, 0, L[0].u, L[0].b, L[0].r, nullptr, op); };
         *bytes = (cf + 3.0) * w * N;
-        name = "k_residual";
+        name = (view(c, L, 0).fast && op != OP_STENCIL && !c->has_vel) ? "k_residual_planes" : "k_residual";
     } else if (kernel == 2) {
         if (c->nlev() < 2) return fail(MPS_ERR_ARG, "no coarse level");
         const Geom &gc = c->geom[1];
@@ -1933,13 +2011,9 @@ int time_kernel_t(mps_ctx *c, std::vector<LevelBuf<T>> &L, int kernel, int reps,
             *bytes = (2.0 * N + 3.0 * (double)gc.n()) * w;
             name = "k_restrict_fas";
         } else {
-            AlignedW W{L[1].w[0], L[1].w[1], L[1].w[2]};
-            launch = [&, vf, vc, W]() {
-                hipLaunchKernelGGL(k_restrict_fw<T>, cells_grid(gc), block(), 0, c->stream, vf, vc, W,
-                                   (const T *)L[0].r, L[1].b);
-            };
+            launch = [&]() { launch_restrict_fw<T>(c, L, 0); };
             *bytes = (N + (double)gc.n()) * w;
-            name = "k_restrict_fw";
+            name = "k_restrict_fw_planes";
         }
     } else if (kernel == 3) {
         if (c->nlev() < 2) return fail(MPS_ERR_ARG, "no coarse level");
@@ -1954,12 +2028,9 @@ int time_kernel_t(mps_ctx *c, std::vector<LevelBuf<T>> &L, int kernel, int reps,
             *bytes = (2.0 * N + 2.0 * (double)gc.n()) * w;
             name = "k_prolong1_correct";
         } else {
-            launch = [&, vf, vc]() {
-                hipLaunchKernelGGL(k_prolong_tri_correct<T>, cells_grid(g), block(), 0, c->stream, vf, vc, L[0].u,
-                                   (const T *)L[1].u);
-            };
+            launch = [&]() { launch_prolong_tri<T>(c, L, 0); };
             *bytes = (2.0 * N + (double)gc.n()) * w;
-            name = "k_prolong_tri_correct";
+            name = "k_prolong_tri_planes";
         }
     } else {
         return fail(MPS_ERR_ARG, "unknown kernel %d", kernel);
@@ -2000,4 +2071,31 @@ extern "C" int mps_time_kernel(mps_ctx *c, int kernel, int fp32, int reps, doubl
     } catch (const HipError &e) {
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
     }
+}
+
+extern "C" int mps_grad_velocity(mps_ctx *c, const double *lam, const double *x, double *gvx, double *gvy,
+                                 double *gvz)
+{
+    int st = guard(c);
+    if (st) return st;
+    if (!lam || !x || !gvx || !gvy || !gvz) return fail(MPS_ERR_ARG, "null argument");
+    if (!c->has_vel) return fail(MPS_ERR_STATE, "the operator has no velocity field");
+    if (!is_device_ptr(lam) || !is_device_ptr(x) || !is_device_ptr(gvx) || !is_device_ptr(gvy) ||
+        !is_device_ptr(gvz))
+        return fail(MPS_ERR_ARG, "mps_grad_velocity takes device arrays");
+    try {
+        ExactScope exact(c);
+        const Geom &g = c->geom[0];
+        Lev<double> v = view(c, c->L64, 0);
+        if (c->opts.flux == MPS_FLUX_SG)
+            hipLaunchKernelGGL((k_velocity_grad<OP_REF, MPS_FLUX_SG_K>), cells_grid(g), block(), 0, c->stream, v, lam,
+                               x, gvx, gvy, gvz);
+        else
+            hipLaunchKernelGGL((k_velocity_grad<OP_REF, MPS_FLUX_UPWIND_K>), cells_grid(g), block(), 0, c->stream, v,
+                               lam, x, gvx, gvy, gvz);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
 }

@@ -380,3 +380,106 @@ __global__ void __launch_bounds__(kBlock) k_transpose_stencil(Lev<T> L, const T 
 }
 
 }  // namespace mps
+
+namespace mps {
+
+// dB/dx of the Bernoulli function, B'(x) = B(x) (1 - B(-x)) / x, with its
+// Taylor series near 0 (B' = -1/2 + x/6 - x^3/180).
+__device__ __forceinline__ double bernoulli_prime(double x)
+{
+    if (fabs(x) < 1e-3) {
+        const double x2 = x * x;
+        return -0.5 + x * (1.0 / 6.0 - x2 * (1.0 / 180.0));
+    }
+    return bernoulli(x) * (1.0 - bernoulli(-x)) / x;
+}
+
+// d(wL)/d(c) and d(wR)/d(c) of a face's flux weights with respect to the
+// face velocity c = (cL + cR)/2 (SG: dwL/dc = -B'(-Pe), dwR/dc = -B'(Pe);
+// upwind: the flux is c u_upwind); for vanishing D the SG weights are the
+// upwind ones with the cell velocities, returned through dL/dR per cell.
+template <int FLUX>
+__device__ __forceinline__ void flux_dc(double cL, double cR, double DL, double DR, double d, double uL, double uR,
+                                        double &g_face, double &g_L, double &g_R)
+{
+    const double ch = 0.5 * (cL + cR);
+    g_face = 0.0;
+    g_L = 0.0;
+    g_R = 0.0;
+    if (FLUX == MPS_FLUX_UPWIND_K) {
+        g_face = ch > 0.0 ? uL : uR;
+        return;
+    }
+    const double eps = 1e-10;
+    const double Dh = 2.0 * DL * DR / (DL + DR + eps);
+    if (fabs(Dh) < eps) {
+        const double sgn = ch <= 0.0 ? -1.0 : 1.0;
+        g_L = 0.5 * (1.0 + sgn) * uL;
+        g_R = 0.5 * (1.0 - sgn) * uR;
+        return;
+    }
+    const double Pe = ch * d / Dh;
+    g_face = -bernoulli_prime(-Pe) * uL - bernoulli_prime(Pe) * uR;
+}
+
+// Velocity sensitivity of the adjoint pairing:  g_v(c) = -d(lam^T A(v) x)/dv(c)
+// for the three cell velocity components, i.e. the gradient of a scalar J
+// through x = A(v)^-1 b once lam = A^-T dJ/dx. Each face couples its two
+// cells through the face velocity (v_L + v_R)/2 and the flux pairing
+// (lam_L/w_L - lam_R/w_R) F; the boundary rows add their convective terms.
+// Every cell evaluates both of its faces, so no atomics are needed.
+template <int OP, int FLUX>
+__global__ void __launch_bounds__(kBlock) k_velocity_grad(Lev<double> L, const double *__restrict__ lam,
+                                                         const double *__restrict__ x, double *gx, double *gy,
+                                                         double *gz)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    const double *vel[3] = {L.vx, L.vy, L.vz};
+    const long long s[3] = {1, L.sy, L.sz};
+    const int idx[3] = {i, j, k};
+    const int nn[3] = {L.nx, L.ny, L.nz};
+    const double h[3] = {L.dx, L.dy, L.dz};
+    const double ai[3] = {L.ax, L.ay, L.az};
+    const double ab[3] = {L.axb, L.ayb, L.azb};
+    double *out[3] = {gx, gy, gz};
+    const long long n = (long long)(i - 1) + (long long)L.nx * ((j - 1) + (long long)L.ny * (k - 1));
+    for (int dim = 0; dim < 3; ++dim) {
+        double g = 0.0;
+        const double vc = vel[dim] ? vel[dim][c] : 0.0;
+        auto abv_of = [&](int ii) {
+            const bool edge = ii == 1 || ii == nn[dim];
+            return (OP == OP_ALIGNED && edge) ? ab[dim] : ai[dim];
+        };
+        if (idx[dim] > 1) {  // face (m | c)
+            const long long m = c - s[dim];
+            double gf, gl, gr;
+            flux_dc<FLUX>(vel[dim] ? vel[dim][m] : 0.0, vc, L.D[m], L.D[c], h[dim], x[m], x[c], gf, gl, gr);
+            const double pair = lam[m] * abv_of(idx[dim] - 1) - lam[c] * abv_of(idx[dim]);
+            g += pair * (0.5 * gf + gr);
+        }
+        if (idx[dim] < nn[dim]) {  // face (c | p)
+            const long long p = c + s[dim];
+            double gf, gl, gr;
+            flux_dc<FLUX>(vc, vel[dim] ? vel[dim][p] : 0.0, L.D[c], L.D[p], h[dim], x[c], x[p], gf, gl, gr);
+            const double pair = lam[c] * abv_of(idx[dim]) - lam[p] * abv_of(idx[dim] + 1);
+            g += pair * (0.5 * gf + gl);
+        }
+        // boundary rows: DIRC upwind outflow and ZGRD convection
+        // (discretepde.f90:810-828 and the + face copy)
+        const double abv = abv_of(idx[dim]);
+        const double hh = OP == OP_REF ? 1.0 / h[dim] : abv;
+        if (idx[dim] == 1) {
+            const int code = L.bc[2 * dim];
+            if ((code == BC_DIRC && vc < 0.0) || code == BC_ZGRD) g += lam[c] * (-hh) * x[c];
+        }
+        if (idx[dim] == nn[dim]) {
+            const int code = L.bc[2 * dim + 1];
+            if ((code == BC_DIRC && vc > 0.0) || code == BC_ZGRD) g += lam[c] * hh * x[c];
+        }
+        out[dim][n] = -g;
+    }
+}
+
+}  // namespace mps

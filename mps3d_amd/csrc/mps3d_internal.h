@@ -23,6 +23,8 @@ enum BcCode { BC_NONE = 0, BC_DIRC = 1, BC_ZGRD = 2, BC_FLUX = 3 };
 //  OP_STENCIL  explicit 7-point rows: c u + xm u(i-1) + xp u(i+1) + ...
 //              (Scharfetter-Gummel operators and transposed operators).
 enum OpKind { OP_REF = 0, OP_ALIGNED = 1, OP_STENCIL = 2 };
+// flux schemes as the kernels see them (values of MPS_FLUX_*)
+enum { MPS_FLUX_UPWIND_K = 0, MPS_FLUX_SG_K = 1 };
 
 // Stencil slots for OP_STENCIL.
 enum { ST_C = 0, ST_XM, ST_XP, ST_YM, ST_YP, ST_ZM, ST_ZP, ST_N };

@@ -223,13 +223,22 @@ __device__ __forceinline__ double fast_rcp(double d)
 
 // Fused red-black sweep in fast arithmetic (rows without convection).
 //
-// Same schedule and LDS rings as k_sweep. Rows are branch-free: each lane
+// Same schedule as k_sweep: plane s's red cells (on the tile and its ring,
+// from old black values) and plane s-1's black cells (from new red values)
+// per step, two barriers per step. Rows are branch-free: each lane
 // precomputes, for its cells, the six face factors abv/(2d) with the faces
 // on the physical boundary masked to zero and the DIRC wall factor, so a row
 // is  w_f = (D_c + D_nb) h_f,  diag = sum w_f + D_c g + 1/dt - reac,
 // u = (b + sum w_f u_nb) / diag.  Values outside the interior are staged as
-// zeros, so masked faces multiply zeros. Global addressing uses 32-bit lane
-// offsets against per-plane base pointers.
+// zeros, so masked faces multiply zeros. LDS holds three u planes and five
+// D planes of the tile with a 2-cell halo and three planes of red results;
+// each lane's own b values (red and black cell) travel in a two-plane
+// register queue instead. Ring depths: step s stores u/D plane s+1 before
+// its first barrier, and its red pass writes red plane s after it. The u
+// slot it reuses (plane s-2) was last read by the red pass of step s-1 and
+// the red slot (plane s-3) by the black pass of step s-1, both behind a
+// barrier; the D slot it reuses was last read by the black pass of step
+// s-1 (planes s-3..s-1) with no barrier in between, hence five D planes.
 template <typename T, int OP, bool REAC, int TX, int TY>
 __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__restrict__ uin, T *__restrict__ uout,
                                                       const T *__restrict__ b, int kz)
@@ -245,11 +254,10 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     static_assert(W1 % 2 == 0, "tile width must be even");
     static_assert(NR <= NT, "one red cell per lane");
     constexpr int DR = 5;
-    __shared__ T su[4][P2];
+    __shared__ T su[3][P2];
     __shared__ T sD[DR][P2];
-    __shared__ T sb[4][P1];
     __shared__ T sq[REAC ? 4 : 1][REAC ? P1 : 1];
-    __shared__ T sr[4][P1];
+    __shared__ T sr[3][P1];
 
     const int tid = threadIdx.x;
     const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1;
@@ -274,7 +282,7 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     for (int r = 0; r < N1; ++r) {
         const int q = tid + r * NT;
         o1[r] = -1;
-        if (q < P1) {
+        if (REAC && q < P1) {
             const int gi = i0 - 1 + q % W1, gj = j0 - 1 + q / W1;
             if (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny) o1[r] = gi + sy * gj;
         }
@@ -314,21 +322,21 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     const bool red_lane = tid < NR;
     const int jr = tid / RW, mr = tid - jr * RW;
     const int jrg = j0 - 1 + jr;
-    int rq[2], re[2];
-    bool rin[2];
+    int rq[2], re[2], ro[2];
     T rfxm[2], rfxp[2], rfym, rfyp, rgxy[2];
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
         const int ii = 2 * mr + p, i = i0 - 1 + ii;
         rq[p] = ii + W1 * jr;
         re[p] = (ii + 1) + W2 * (jr + 1);
-        rin[p] = red_lane && i >= 1 && i <= nx && jrg >= 1 && jrg <= ny;
-        T a, c2, d2;
+        const bool rin = red_lane && i >= 1 && i <= nx && jrg >= 1 && jrg <= ny;
+        ro[p] = rin ? i + sy * jrg : -1;
+        T a, c2;
         xy_factors(i, jrg, rfxm[p], rfxp[p], a, c2, rgxy[p]);
         rfym = a;
         rfyp = c2;
-        (void)d2;
     }
+    const int rpar = (i0 + j0 + jr + 1) & 1;  // parity of the red cell at plane k: (rpar + k) & 1
     // black lane: its tile cell
     const int tx = tid % TX, ty = tid / TX;
     const int bi = i0 + tx, bj = j0 + ty;
@@ -339,13 +347,13 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     xy_factors(bi, bj, bfxm, bfxp, bfym, bfyp, bgxy);
     const int cb = bi + sy * bj;
 
-    T pu[N2], pD[N2], pb[N1], pq[N1];
+    // gload(kp): u and D tiles of plane kp (for LDS), the red-cell b of
+    // plane kp and the black-cell b of plane kp - 1 (both used at step kp)
+    T pu[N2], pD[N2], pq[N1], nbr, nbb;
     auto gload = [&](int kp) {
         const bool ok = kp >= 1 && kp <= nz;
         const T *ub = uin + sz * kp;
         const T *Db = L.D + sz * kp;
-        const T *bb = b + sz * kp;
-        const T *qb = REAC ? L.reac + sz * kp : nullptr;
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
             pu[r] = T(0);
@@ -355,32 +363,31 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
                 pD[r] = Db[o2[r]];
             }
         }
+        if (REAC) {
+            const T *qb = L.reac + sz * kp;
 #pragma unroll
-        for (int r = 0; r < N1; ++r) {
-            pb[r] = T(0);
-            pq[r] = T(0);
-            if (ok && o1[r] >= 0) {
-                pb[r] = bb[o1[r]];
-                if (REAC) pq[r] = qb[o1[r]];
-            }
+            for (int r = 0; r < N1; ++r) pq[r] = (ok && o1[r] >= 0) ? qb[o1[r]] : T(0);
         }
+        const int orr = ro[(rpar + kp) & 1];
+        nbr = (ok && orr >= 0) ? b[sz * kp + orr] : T(0);
+        const int kbk = kp - 1;
+        nbb = (kbk >= 1 && kbk <= nz && bin) ? b[sz * kbk + cb] : T(0);
     };
     auto lstore = [&](int kp) {
-        const int s4 = kp & 3, s5 = (kp + 5) % DR;
+        const int s3 = kp % 3, s5 = kp % DR;
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
             const int q = tid + r * NT;
             if (q < P2) {
-                su[s4][q] = pu[r];
+                su[s3][q] = pu[r];
                 sD[s5][q] = pD[r];
             }
         }
+        if (REAC) {
 #pragma unroll
-        for (int r = 0; r < N1; ++r) {
-            const int q = tid + r * NT;
-            if (q < P1) {
-                sb[s4][q] = pb[r];
-                if (REAC) sq[s4][q] = pq[r];
+            for (int r = 0; r < N1; ++r) {
+                const int q = tid + r * NT;
+                if (q < P1) sq[kp & 3][q] = pq[r];
             }
         }
     };
@@ -402,55 +409,340 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
         return s * fast_rcp(d);
     };
 
+    // planes are >= kb - 2 >= -1: offset the ring index by 3 * 5 so it stays
+    // non-negative
+    const int kofs = 15;
     gload(kb - 2);
-    lstore(kb - 2);
+    lstore(kb - 2 + kofs);
     gload(kb - 1);
-    lstore(kb - 1);
+    lstore(kb - 1 + kofs);
+    T br0 = nbr, bb0 = nbb;
     gload(kb);
+    T br1 = nbr, bb1 = nbb;
     for (int s = kb - 1; s <= ke + 1; ++s) {
-        lstore(s + 1);
+        lstore(s + 1 + kofs);
         __syncthreads();
         if (s <= ke) gload(s + 2);
         if (s >= 1 && s <= nz) {
-            const int p = (i0 + j0 + jr + s + 1) & 1;
-            const bool in = p ? rin[1] : rin[0];
-            if (in) {
+            const int p = (rpar + s) & 1;
+            const int orr = p ? ro[1] : ro[0];
+            if (orr >= 0) {
                 T fzm, fzp, gz;
                 z_factors(s, fzm, fzp, gz);
                 const int q = p ? rq[1] : rq[0];
                 const int e = p ? re[1] : re[0];
-                const T *um = su[(s - 1) & 3], *u0 = su[s & 3], *up = su[(s + 1) & 3];
-                const T *Dm = sD[(s - 1 + DR) % DR], *D0 = sD[s % DR], *Dp = sD[(s + 1) % DR];
-                const T rc = REAC ? sq[REAC ? (s & 3) : 0][q] : T(0);
-                sr[s & 3][q] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], u0[e - 1],
+                const int t = s + kofs;
+                const T *um = su[(t - 1) % 3], *u0 = su[t % 3], *up = su[(t + 1) % 3];
+                const T *Dm = sD[(t - 1) % DR], *D0 = sD[t % DR], *Dp = sD[(t + 1) % DR];
+                const T rc = REAC ? sq[REAC ? (t & 3) : 0][q] : T(0);
+                sr[t % 3][q] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], u0[e - 1],
                                         u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], p ? rfxm[1] : rfxm[0],
                                         p ? rfxp[1] : rfxp[0], rfym, rfyp, fzm, fzp, (p ? rgxy[1] : rgxy[0]) + gz, rc,
-                                        sb[s & 3][q]);
+                                        br0);
             }
         }
         __syncthreads();
         const int k = s - 1;
         if (k >= kb && k <= ke && bin) {
+            const int t = k + kofs;
             T val;
             if ((bi + bj + k) & 1) {
-                val = sr[k & 3][q1c];
+                val = sr[t % 3][q1c];
             } else {
                 T fzm, fzp, gz;
                 z_factors(k, fzm, fzp, gz);
-                const T *rm = sr[(k - 1) & 3], *r0 = sr[k & 3], *rp = sr[(k + 1) & 3];
-                const T *Dm = sD[(k - 1 + DR) % DR], *D0 = sD[k % DR], *Dp = sD[(k + 1) % DR];
-                const T rc = REAC ? sq[REAC ? (k & 3) : 0][q1c] : T(0);
+                const T *rm = sr[(t - 1) % 3], *r0 = sr[t % 3], *rp = sr[(t + 1) % 3];
+                const T *Dm = sD[(t - 1) % DR], *D0 = sD[t % DR], *Dp = sD[(t + 1) % DR];
+                const T rc = REAC ? sq[REAC ? (t & 3) : 0][q1c] : T(0);
                 // red neighbours outside the interior hold stale values:
                 // their face factors are zero, so select them away
                 const T rzm = k > 1 ? rm[q1c] : T(0), rzp = k < nz ? rp[q1c] : T(0);
                 const T rxm = bi > 1 ? r0[q1c - 1] : T(0), rxp = bi < nx ? r0[q1c + 1] : T(0);
                 const T rym = bj > 1 ? r0[q1c - W1] : T(0), ryp = bj < ny ? r0[q1c + W1] : T(0);
                 val = fast_row(D0[e2c], D0[e2c - 1], D0[e2c + 1], D0[e2c - W2], D0[e2c + W2], Dm[e2c], Dp[e2c], rxm,
-                               rxp, rym, ryp, rzm, rzp, bfxm, bfxp, bfym, bfyp, fzm, fzp, bgxy + gz, rc,
-                               sb[k & 3][q1c]);
+                               rxp, rym, ryp, rzm, rzp, bfxm, bfxp, bfym, bfyp, fzm, fzp, bgxy + gz, rc, bb0);
             }
             uout[cb + sz * k] = val;
         }
+        br0 = br1;
+        bb0 = bb1;
+        br1 = nbr;
+        bb1 = nbb;
+    }
+}
+
+}  // namespace mps
+
+namespace mps {
+
+// Residual r = b - A u of rows without convection in fast arithmetic (and
+// the block's sum of r^2), marching the z-planes of a TX x TY tile: each
+// lane keeps its own column's z neighbours in registers and reads its x/y
+// neighbours from a (TX+2) x (TY+2) plane in LDS (the halo ring, corners
+// excluded, is loaded by the first 2(TX+TY) lanes). The next plane's loads
+// are in flight while the current plane is evaluated; two LDS planes
+// alternate, so each plane costs one barrier. Values outside the interior
+// are staged as zeros and their face factors are zero. R is the stored
+// type of r (float for the FP32 defect of the FP64 iterate); r is nullable.
+template <typename T, typename R, int OP, bool REAC, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) k_residual_planes(Lev<T> L, const T *__restrict__ u,
+                                                           const T *__restrict__ b, R *__restrict__ r,
+                                                           double *partials, int kz)
+{
+#pragma clang fp contract(fast)
+    constexpr int W = TX + 2, H = TY + 2, NT = TX * TY;
+    constexpr int NH = 2 * (TX + TY);
+    static_assert(NH <= NT, "halo ring must fit one cell per lane");
+    static_assert(NT <= kBlock, "block_sum holds kBlock / 64 waves");
+    __shared__ T su[2][W * H];
+    __shared__ T sD[2][W * H];
+
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1;
+    const int kb = blockIdx.z * kz + 1, ke = min(kb + kz - 1, L.nz);
+    const int nx = L.nx, ny = L.ny, nz = L.nz, sy = L.sy;
+    const long long sz = L.sz;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool in = i <= nx && j <= ny;
+    const int oc = in ? i + sy * j : -1;
+    const int q = (tx + 1) + W * (ty + 1);
+    int oh = -1, qh = 0;
+    if (tid < NH) {
+        int hi, hj;
+        if (tid < TX) {
+            hi = tid, hj = -1;
+        } else if (tid < 2 * TX) {
+            hi = tid - TX, hj = TY;
+        } else if (tid < 2 * TX + TY) {
+            hi = -1, hj = tid - 2 * TX;
+        } else {
+            hi = TX, hj = tid - 2 * TX - TY;
+        }
+        const int gi = i0 + hi, gj = j0 + hj;
+        qh = (hi + 1) + W * (hj + 1);
+        if (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny) oh = gi + sy * gj;
+    }
+
+    // own face factors (zero on the physical boundary) and x/y DIRC factor
+    T fxm, fxp, fym, fyp, gxy;
+    {
+        const bool ex = OP == OP_ALIGNED && (i == 1 || i == nx);
+        const bool ey = OP == OP_ALIGNED && (j == 1 || j == ny);
+        const T hx = ex ? L.hxb : L.hx, hy = ey ? L.hyb : L.hy;
+        fxm = i > 1 ? hx : T(0);
+        fxp = i < nx ? hx : T(0);
+        fym = j > 1 ? hy : T(0);
+        fyp = j < ny ? hy : T(0);
+        const T gx = ex ? L.gxb : L.gx, gy = ey ? L.gyb : L.gy;
+        T g = T(0);
+        if (i == 1 && L.bc[LEFT] == BC_DIRC) g += gx;
+        if (i == nx && L.bc[RIGHT] == BC_DIRC) g += gx;
+        if (j == 1 && L.bc[BOTTOM] == BC_DIRC) g += gy;
+        if (j == ny && L.bc[TOP] == BC_DIRC) g += gy;
+        gxy = g;
+    }
+
+    T pu, pD, pb, pq, phu, phD;
+    auto gload = [&](int k) {
+        const bool ok = k >= 1 && k <= nz;
+        const long long base = sz * k;
+        pu = pD = pb = pq = T(0);
+        if (ok && oc >= 0) {
+            pu = u[base + oc];
+            pD = L.D[base + oc];
+            pb = b[base + oc];
+            if (REAC) pq = L.reac[base + oc];
+        }
+        phu = phD = T(0);
+        if (ok && oh >= 0) {
+            phu = u[base + oh];
+            phD = L.D[base + oh];
+        }
+    };
+
+    gload(kb - 1);
+    T um = pu, Dm = pD;
+    gload(kb);
+    T uc = pu, Dc = pD, bc = pb, qc = pq, hu = phu, hD = phD;
+    gload(kb + 1);
+    double sq = 0.0;
+    for (int k = kb; k <= ke; ++k) {
+        const int s = k & 1;
+        su[s][q] = uc;
+        sD[s][q] = Dc;
+        if (tid < NH) {
+            su[s][qh] = hu;
+            sD[s][qh] = hD;
+        }
+        __syncthreads();
+        const T up = pu, Dp = pD, bn = pb, qn = pq, hun = phu, hDn = phD;
+        if (k < ke) gload(k + 2);
+        if (in) {
+            const bool ez = OP == OP_ALIGNED && (k == 1 || k == nz);
+            const T hz = ez ? L.hzb : L.hz;
+            const T fzm = k > 1 ? hz : T(0), fzp = k < nz ? hz : T(0);
+            T g = gxy;
+            if (k == 1 && L.bc[BACK] == BC_DIRC) g += ez ? L.gzb : L.gz;
+            if (k == nz && L.bc[FRONT] == BC_DIRC) g += ez ? L.gzb : L.gz;
+            const T *U = su[s], *Dl = sD[s];
+            const T wxm = (Dl[q - 1] + Dc) * fxm, wxp = (Dc + Dl[q + 1]) * fxp;
+            const T wym = (Dl[q - W] + Dc) * fym, wyp = (Dc + Dl[q + W]) * fyp;
+            const T wzm = (Dm + Dc) * fzm, wzp = (Dc + Dp) * fzp;
+            T d = ((wxm + wxp) + (wym + wyp)) + (wzm + wzp);
+            d = d + Dc * g + L.dtinv;
+            if (REAC) d = d - qc;
+            T nb = wxm * U[q - 1];
+            nb = nb + wxp * U[q + 1];
+            nb = nb + wym * U[q - W];
+            nb = nb + wyp * U[q + W];
+            nb = nb + wzm * um;
+            nb = nb + wzp * up;
+            const T rv = bc - (d * uc - nb);
+            if (r) r[sz * k + oc] = (R)rv;
+            sq += (double)rv * (double)rv;
+        }
+        um = uc;
+        Dm = Dc;
+        uc = up;
+        Dc = Dp;
+        bc = bn;
+        qc = qn;
+        hu = hun;
+        hD = hDn;
+    }
+    if (partials) {
+        const double t = block_sum(sq);
+        if (tid == 0) partials[blockIdx.x + gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = t;
+    }
+}
+
+}  // namespace mps
+
+namespace mps {
+
+// Volume-weighted full weighting onto the aligned coarse grid, marching
+// coarse z-planes: a block owns CX x CY coarse columns and stages, per
+// coarse plane K, the fine planes 2K-1 and 2K of its (2CX+1) x (2CY+1) fine
+// footprint in LDS; each lane forms the xy-weighted 3x3 sum of its coarse
+// cell on each fine plane (FP64 accumulation) and combines three planes
+// with the z weights, reusing plane 2K's sum as the next cell's plane 2K-2.
+// Fine values outside the interior are staged as zeros (their weights are
+// zero). The next step's fine planes are loaded while the current one is
+// summed.
+template <typename T, int CX, int CY>
+__global__ void __launch_bounds__(CX *CY) k_restrict_fw_planes(Lev<T> F, Lev<T> C, AlignedW W,
+                                                              const T *__restrict__ f, T *__restrict__ cf, int kz)
+{
+    constexpr int FW = 2 * CX + 1, FH = 2 * CY + 1, FP = FW * FH, NT = CX * CY;
+    constexpr int NS = (FP + NT - 1) / NT;
+    __shared__ T sf[2][FP];
+    const int tid = threadIdx.x, tx = tid % CX, ty = tid / CX;
+    const int I0 = blockIdx.x * CX + 1, J0 = blockIdx.y * CY + 1;
+    const int Kb = blockIdx.z * kz + 1, Ke = min(Kb + kz - 1, C.nz);
+    const int I = I0 + tx, J = J0 + ty;
+    const bool in = I <= C.nx && J <= C.ny;
+    int of[NS];
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        const int q = tid + r * NT;
+        of[r] = -1;
+        if (q < FP) {
+            const int gi = 2 * I0 - 2 + q % FW, gj = 2 * J0 - 2 + q / FW;
+            if (gi >= 1 && gi <= F.nx && gj >= 1 && gj <= F.ny) of[r] = gi + F.sy * gj;
+        }
+    }
+    double wx0 = 0, wx1 = 0, wx2 = 0, wy0 = 0, wy1 = 0, wy2 = 0;
+    if (in) {
+        wx0 = W.wx[3 * (I - 1)], wx1 = W.wx[3 * (I - 1) + 1], wx2 = W.wx[3 * (I - 1) + 2];
+        wy0 = W.wy[3 * (J - 1)], wy1 = W.wy[3 * (J - 1) + 1], wy2 = W.wy[3 * (J - 1) + 2];
+    }
+    const int qc = (2 * tx + 1) + FW * (2 * ty + 1);
+    T pa[NS], pb[NS];
+    auto gload = [&](int fk, T(&p)[NS]) {
+        const bool ok = fk >= 1 && fk <= F.nz;
+        const T *base = f + F.sz * fk;
+#pragma unroll
+        for (int r = 0; r < NS; ++r) p[r] = (ok && of[r] >= 0) ? base[of[r]] : T(0);
+    };
+    auto lstore = [&](int slot, const T(&p)[NS]) {
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
+            const int q = tid + r * NT;
+            if (q < FP) sf[slot][q] = p[r];
+        }
+    };
+    auto xysum = [&](int slot) {
+        const T *s = sf[slot];
+        const double r0 = (wx0 * (double)s[qc - FW - 1] + wx1 * (double)s[qc - FW]) + wx2 * (double)s[qc - FW + 1];
+        const double r1 = (wx0 * (double)s[qc - 1] + wx1 * (double)s[qc]) + wx2 * (double)s[qc + 1];
+        const double r2 = (wx0 * (double)s[qc + FW - 1] + wx1 * (double)s[qc + FW]) + wx2 * (double)s[qc + FW + 1];
+        return (wy0 * r0 + wy1 * r1) + wy2 * r2;
+    };
+
+    gload(2 * Kb - 2, pa);
+    lstore(0, pa);
+    __syncthreads();
+    double sprev = in ? xysum(0) : 0.0;
+    gload(2 * Kb - 1, pa);
+    gload(2 * Kb, pb);
+    for (int K = Kb; K <= Ke; ++K) {
+        __syncthreads();
+        lstore(0, pa);
+        lstore(1, pb);
+        __syncthreads();
+        if (K < Ke) {
+            gload(2 * K + 1, pa);
+            gload(2 * K + 2, pb);
+        }
+        if (in) {
+            const double s1 = xysum(0), s2 = xysum(1);
+            const double *wz = W.wz + 3 * (K - 1);
+            cf[I + (long long)C.sy * J + C.sz * K] = (T)((wz[0] * sprev + wz[1] * s1) + wz[2] * s2);
+            sprev = s2;
+        }
+    }
+}
+
+// Trilinear prolongation + correction on the aligned hierarchy, marching
+// fine z-planes in odd/even pairs: each lane interpolates the coarse grid
+// in x and y once per coarse plane (fine cell f takes coarse (f+1)/2 when f
+// is odd, the mean of f/2 and f/2+1 when even) and reuses it for the fine
+// planes on either side. Same arithmetic as k_prolong_tri_correct. kz must
+// be even, so every chunk starts on an odd fine plane.
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) k_prolong_tri_planes(Lev<T> F, Lev<T> C, T *__restrict__ uf,
+                                                              const T *__restrict__ uc, int kz)
+{
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * TX + tid % TX + 1, j = blockIdx.y * TY + tid / TX + 1;
+    if (i > F.nx || j > F.ny) return;
+    const int kb = blockIdx.z * kz + 1, ke = min(kb + kz - 1, F.nz);
+    const int ia = (i + 1) >> 1, ja = (j + 1) >> 1;
+    const int ib = (i & 1) ? ia : ia + 1, jb = (j & 1) ? ja : ja + 1;
+    const bool xo = i & 1, yo = j & 1;
+    const long long sy = C.sy;
+    const long long oaa = ia + sy * ja, oba = ib + sy * ja, oab = ia + sy * jb, obb = ib + sy * jb;
+    auto xy = [&](int K) {
+        const T *p = uc + C.sz * K;
+        if (xo) return yo ? p[oaa] : T(0.5) * (p[oaa] + p[oab]);
+        const T r0 = T(0.5) * (p[oaa] + p[oba]);
+        if (yo) return r0;
+        const T r1 = T(0.5) * (p[oab] + p[obb]);
+        return T(0.5) * (r0 + r1);
+    };
+    int K = (kb + 1) >> 1;
+    T e0 = xy(K);
+    T e1 = K + 1 <= C.nz ? xy(K + 1) : T(0);
+    T *col = uf + i + (long long)F.sy * j;
+    for (int k = kb; k <= ke; k += 2) {
+        const bool pair = k + 1 <= ke;
+        const T a = col[F.sz * k];
+        const T b2 = pair ? col[F.sz * (k + 1)] : T(0);
+        col[F.sz * k] = a + e0;
+        if (pair) col[F.sz * (k + 1)] = b2 + T(0.5) * (e0 + e1);
+        e0 = e1;
+        ++K;
+        e1 = K + 1 <= C.nz ? xy(K + 1) : T(0);
     }
 }
 

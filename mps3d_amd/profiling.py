@@ -1,0 +1,123 @@
+"""Summaries of rocprofv3 runs of ``bench.py`` (text for ``profiles/``).
+
+Three runs feed them, each of the same command
+(``python3 bench.py --steps 2 --warmup 1 --no-cpu``):
+
+* ``rocprofv3 --kernel-trace --stats --output-format csv`` gives the
+  per-kernel time table (``*_kernel_stats.csv``) and the per-dispatch trace
+  (``*_kernel_trace.csv``);
+* ``rocprofv3 --pmc FETCH_SIZE --output-format csv`` and
+  ``rocprofv3 --pmc WRITE_SIZE --output-format csv`` give the HBM-side bytes
+  of every dispatch (``*_counter_collection.csv``, KiB per dispatch).
+
+The counters are scaled by what they read for a plain device copy of known
+size, which ``bench.py`` runs once (``hipMemcpyAsync`` of ``COPY_BYTES``
+bytes, the ``__amd_rocclr_copyBuffer`` kernel): factor = known bytes /
+counter bytes, separately for reads and writes.
+
+Usage:
+    python -m mps3d_amd.profiling stats <kernel_stats.csv> [<kernel_trace.csv>]
+    python -m mps3d_amd.profiling traffic <fetch.csv> <write.csv> --kernel NAME [--grid THREADS]
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import json
+import sys
+from collections import defaultdict
+
+COPY_BYTES = 512 * 1024 * 1024  # bytes read (and written) by the calibration copy
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def _short(name, n=90):
+    name = name.replace("mps::", "")
+    return name if len(name) <= n else name[: n - 3] + "..."
+
+
+def stats_table(stats_csv, trace_csv=None, top=25):
+    """Markdown table of kernel time from ``--stats`` (and, with the trace,
+    the average per distinct grid of the top kernels)."""
+    rows = list(csv.DictReader(open(stats_csv)))
+    out = ["| kernel | calls | total ms | avg us | % |", "|---|---|---|---|---|"]
+    for r in rows[:top]:
+        out.append("| `%s` | %s | %.3f | %.2f | %.1f |" % (
+            _short(r["Name"]), r["Calls"], float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3,
+            float(r["Percentage"])))
+    if trace_csv:
+        by = defaultdict(list)
+        for r in csv.DictReader(open(trace_csv)):
+            key = (r["Kernel_Name"], _grid_total(r))
+            by[key].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        ranked = sorted(by.items(), key=lambda kv: -sum(kv[1]))[:top]
+        out += ["", "| kernel | grid (threads) | calls | avg us | total ms |", "|---|---|---|---|---|"]
+        for (name, g), d in ranked:
+            out.append("| `%s` | %d | %d | %.2f | %.3f |" % (
+                _short(name, 60), g, len(d), sum(d) / len(d) / 1e3, sum(d) / 1e6))
+    return "\n".join(out)
+
+
+def _grid_total(r):
+    """Total grid size in threads of a CSV row (either column layout)."""
+    if "Grid_Size_X" in r:
+        return int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+    return int(r["Grid_Size"])
+
+
+def _dispatches(counter_csv, counter):
+    """{dispatch id: (kernel name, grid threads, counter value in bytes)}"""
+    vals = {}
+    for r in csv.DictReader(open(counter_csv)):
+        if r["Counter_Name"] != counter:
+            continue
+        d = int(r["Dispatch_Id"])
+        name, g, v = vals.get(d, (r["Kernel_Name"], _grid_total(r), 0.0))
+        vals[d] = (name, g, v + float(r["Counter_Value"]) * 1024.0)
+    return vals
+
+
+def traffic(fetch_csv, write_csv, kernel, grid=None):
+    """HBM bytes per dispatch of ``kernel`` (optionally only dispatches of one
+    grid size), scaled by the calibration copy."""
+    out = {}
+    for label, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
+        disp = _dispatches(path, counter)
+        copies = [v for (n, g, v) in disp.values() if COPY_KERNEL in n and v > 0.1 * COPY_BYTES]
+        if not copies:
+            raise SystemExit("no calibration copy of %d bytes in %s" % (COPY_BYTES, path))
+        copy = max(copies)
+        factor = COPY_BYTES / copy
+        ks = [v for (n, g, v) in disp.values() if kernel in n and (grid is None or g == grid)]
+        if not ks:
+            raise SystemExit("no dispatch of %s (grid %s) in %s" % (kernel, grid, path))
+        avg = sum(ks) / len(ks)
+        out[label] = {"counter": counter, "dispatches": len(ks), "avg_counter_bytes": avg,
+                      "copy_counter_bytes": copy, "factor": factor, "bytes": avg * factor}
+    out["kernel"] = kernel
+    out["grid_threads"] = grid
+    out["traffic_bytes"] = out["fetch"]["bytes"] + out["write"]["bytes"]
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m mps3d_amd.profiling")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("stats_csv")
+    s.add_argument("trace_csv", nargs="?")
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_csv")
+    t.add_argument("write_csv")
+    t.add_argument("--kernel", required=True)
+    t.add_argument("--grid", type=int, default=None, help="total grid size in threads (one launch shape)")
+    a = ap.parse_args(argv)
+    if a.cmd == "stats":
+        print(stats_table(a.stats_csv, a.trace_csv))
+    else:
+        print(json.dumps(traffic(a.fetch_csv, a.write_csv, a.kernel, a.grid), indent=1))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
