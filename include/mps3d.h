@@ -75,8 +75,8 @@ enum {
     MPS_ARITH_FAST = 1    /* the same operator with precomputed face factors
                              and one reciprocal per row, for rows without
                              convection; rows with convection stay exact.
-                             find_AX/find_bvec/smoothing entry points are
-                             always exact                                   */
+                             The find_AX/find_bvec entry points are always
+                             exact; mps_smooth follows the context's mode  */
 };
 enum {
     MPS_TOL_RMS = 0,      /* RMS of the absolute residual (globalvars.f90:136-159) */
@@ -155,7 +155,8 @@ int mps_apply_transpose(mps_ctx *ctx, const double *x, double *atx);
 int mps_bvec(mps_ctx *ctx, const double *mgridsterm, int applybc, double *b);
 
 /* perform_gseidel_smoothing (discretepde.f90:132-236) on the finest level:
- * niter red-black sweeps, then res = b - A soln (interior, nullable). */
+ * niter red-black sweeps, then res = b - A soln (interior, nullable), in the
+ * context's arithmetic mode (bit-exact with the reference in EXACT). */
 int mps_smooth(mps_ctx *ctx, double *soln, const double *mgridsterm, int niter, double *res);
 
 /* update_boundary_ghostvalues (discretepde.f90:1121-1267): DIRC ghost =

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -163,6 +164,13 @@ struct mps_ctx {
     double *dinvT = nullptr;             // and of its transpose
     double *gbuf = nullptr;              // ghosted right-hand side of adjoint solves
     bool exact_now = false;              // reference-mirroring call in progress
+    // one outer solve step captured as a HIP graph and replayed every cycle
+    // (see outer_step); rebuilt when its pointers, kind or the problem change
+    bool graphs = true;                  // MPS3D_NO_GRAPHS=1 turns replay off
+    hipGraphExec_t step_graph = nullptr;
+    const void *graph_u = nullptr, *graph_b = nullptr;
+    int graph_kind = -1;
+    unsigned gen = 0, graph_gen = ~0u;   // gen: bumped by set_bc / set_coeffs
     bool mg_ok = true;                   // a multigrid hierarchy exists
     std::string mg_error;                // why not, when it does not
 
@@ -489,12 +497,27 @@ int z_chunk(const Geom &g, int tx, int ty, int target_blocks, int kmax)
     return (int)std::max(2LL, std::min((long long)kmax, kz));
 }
 
-constexpr int kResTX = 32, kResTY = 8, kResKZ = 16;
+#ifndef MPS_RES_TX
+#define MPS_RES_TX 32
+#endif
+#ifndef MPS_RES_TY
+#define MPS_RES_TY 8
+#endif
+#ifndef MPS_RES_RB
+#define MPS_RES_RB 1
+#endif
+#ifndef MPS_RES_KZ
+#define MPS_RES_KZ 16
+#endif
+#ifndef MPS_RES_TARGET
+#define MPS_RES_TARGET 4096
+#endif
+constexpr int kResTX = MPS_RES_TX, kResTY = MPS_RES_TY, kResRB = MPS_RES_RB, kResKZ = MPS_RES_KZ;
 
 dim3 residual_planes_grid(const Geom &g, int &kz)
 {
-    kz = z_chunk(g, kResTX, kResTY, 4096, kResKZ);
-    return dim3((unsigned)((g.nx + kResTX - 1) / kResTX), (unsigned)((g.ny + kResTY - 1) / kResTY),
+    kz = z_chunk(g, kResTX, kResTY * kResRB, MPS_RES_TARGET, kResKZ);
+    return dim3((unsigned)((g.nx + kResTX - 1) / kResTX), (unsigned)((g.ny + kResTY * kResRB - 1) / (kResTY * kResRB)),
                 (unsigned)((g.nz + kz - 1) / kz));
 }
 
@@ -549,7 +572,8 @@ int launch_residual(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *u, 
         int kz;
         const dim3 gr = residual_planes_grid(g, kz);
         auto go = [&](auto O, auto Q) {
-            hipLaunchKernelGGL((k_residual_planes<T, R_, decltype(O)::value, decltype(Q)::value, kResTX, kResTY>),
+            hipLaunchKernelGGL(
+                (k_residual_planes<T, R_, decltype(O)::value, decltype(Q)::value, kResTX, kResTY, kResRB>),
                                gr, dim3(kResTX * kResTY), 0, c->stream, v, u, b, r, partials, kz);
         };
         if (op == OP_REF) {
@@ -619,6 +643,9 @@ void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *wo
 #ifndef MPS_SWEEP_KZ
 #define MPS_SWEEP_KZ 16
 #endif
+#ifndef MPS_SWEEP_PD
+#define MPS_SWEEP_PD 2
+#endif
 constexpr int kSweepTX = MPS_SWEEP_TX, kSweepTY = MPS_SWEEP_TY, kSweepKZ = MPS_SWEEP_KZ;
 
 template <typename T>
@@ -631,7 +658,8 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
             (unsigned)((g.nz + kz - 1) / kz));
     auto go = [&](auto O, auto R) {
         if (v.fast)
-            hipLaunchKernelGGL((k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
+            hipLaunchKernelGGL((k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, MPS_SWEEP_PD>),
+                               gr,
                                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
         else
             hipLaunchKernelGGL((k_sweep<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
@@ -939,6 +967,62 @@ void b_sumsq(mps_ctx *c, const double *b)
     residual_sumsq<double>(c, L, L[0].r, b, level_op(c, 0));
 }
 
+// One outer step of solve_core: the FP32 V-cycle on the defect, the FP64
+// correction and the next FP64 defect (dc), or one FP64 V-cycle and its
+// residual norm. The step's ~10 launches per level are captured once into a
+// HIP graph and replayed, which takes the host launch cost off the critical
+// path between the per-cycle convergence checks. Every node is a kernel
+// (fills included), so replays do not depend on memset nodes.
+void outer_step_body(mps_ctx *c, double *u, double *b, bool homog, bool dc)
+{
+    if (dc) {
+        fill<float>(c, c->L32[0].u, c->geom[0].ng(), 0.0f);
+        vcycle<float>(c, c->L32, 0, c->bottom32, true);
+        Lev<double> v64 = view(c, c->L64, 0);
+        hipLaunchKernelGGL((k_add_interior<double, float>), cells_grid(c->geom[0]), block(), 0, c->stream, v64, u,
+                           (const float *)c->L32[0].u);
+        defect_step(c, u, false, b);
+    } else {
+        fp64_cycle(c, u, b, homog);
+    }
+}
+
+void outer_step(mps_ctx *c, double *u, double *b, bool homog, bool dc)
+{
+    if (!c->graphs) {
+        outer_step_body(c, u, b, homog, dc);
+        return;
+    }
+    const int kind = (dc ? 2 : 0) + (homog ? 1 : 0) + (c->exact_now ? 4 : 0);
+    if (!c->step_graph || c->graph_u != u || c->graph_b != b || c->graph_kind != kind || c->graph_gen != c->gen) {
+        if (c->step_graph) {
+            HIPCHK(hipGraphExecDestroy(c->step_graph));
+            c->step_graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            outer_step_body(c, u, b, homog, dc);
+        } catch (...) {
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(c->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->step_graph = nullptr;
+            throw HipError{e, "hipGraphInstantiate"};
+        }
+        c->graph_u = u;
+        c->graph_b = b;
+        c->graph_kind = kind;
+        c->graph_gen = c->gen;
+    }
+    HIPCHK(hipGraphLaunch(c->step_graph, c->stream));
+}
+
 // The solve loop shared by the forward and the self-adjoint solves: V-cycles
 // (FP64) or FP32 defect-correction steps on u for right-hand side b until
 // the opts.tol_kind measure <= opts.tol. Returns the cycles run.
@@ -960,12 +1044,7 @@ int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, do
             for (it = 0; it < c->opts.max_cycles;) {
                 // the FP32 cycle on the defect, the FP64 correction, then the
                 // FP64 residual of the corrected iterate (the next defect)
-                fill<float>(c, c->L32[0].u, c->geom[0].ng(), 0.0f);
-                vcycle<float>(c, c->L32, 0, c->bottom32, true);
-                Lev<double> v64 = view(c, c->L64, 0);
-                hipLaunchKernelGGL((k_add_interior<double, float>), cells_grid(c->geom[0]), block(), 0, c->stream,
-                                   v64, u, (const float *)c->L32[0].u);
-                defect_step(c, u, false, b);
+                outer_step(c, u, b, homog, true);
                 meas = measure(c, read_scalar(c), bn2);
                 if (history) history[it] = meas;
                 ++it;
@@ -975,7 +1054,7 @@ int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, do
         }
     } else {
         for (it = 0; it < c->opts.max_cycles;) {
-            fp64_cycle(c, u, b, homog);
+            outer_step(c, u, b, homog, false);
             meas = c->geom[0].bottom ? 0.0 : measure(c, read_scalar(c), bn2);
             if (history) history[it] = meas;
             ++it;
@@ -1118,6 +1197,7 @@ int mps_destroy(mps_ctx *c)
     if (!c) return MPS_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
     if (!c->L64.empty()) c->L64[0].b = nullptr;  // b0 is freed below
     if (!c->LT.empty()) c->LT[0].u = nullptr;
     c->L64.size() ? free_levels(c->L64) : void();
@@ -1173,6 +1253,10 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
     try {
         HIPCHK(hipGetDevice(&c->device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        {
+            const char *ng = std::getenv("MPS3D_NO_GRAPHS");
+            c->graphs = !(ng && ng[0] == '1');
+        }
         HIPCHK(hipEventCreate(&c->ev0));
         HIPCHK(hipEventCreate(&c->ev1));
         alloc_levels<double>(c, c->L64, true);
@@ -1282,6 +1366,7 @@ int mps_set_bc(mps_ctx *c, const int codes[6], const double *const faces[6])
             }
         }
         c->bc_set = true;
+        ++c->gen;
         c->transpose_ready = false;
     } catch (const HipError &e) {
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
@@ -1355,6 +1440,7 @@ int mps_set_coeffs(mps_ctx *c, const double *dcoeff, const double *velx, const d
         build_bottom_inverse(c);
         HIPCHK(hipStreamSynchronize(c->stream));
         c->coeffs_set = true;
+        ++c->gen;
         c->transpose_ready = false;
     } catch (const HipError &e) {
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
@@ -1432,7 +1518,6 @@ int mps_smooth(mps_ctx *c, double *soln, const double *sterm, int niter, double 
 {
     int st = guard(c);
     if (st) return st;
-    ExactScope exact(c);
     if (!soln || niter < 0) return fail(MPS_ERR_ARG, "bad arguments");
     try {
         const Geom &g = c->geom[0];

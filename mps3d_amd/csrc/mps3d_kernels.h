@@ -21,6 +21,31 @@ constexpr int kBlock = 256;
 constexpr int kBottomMaxDense = 512;  // largest coarsest grid solved by a dense inverse
 
 // ------------------------------------------------------------------------
+// XCD-aware tile order
+// ------------------------------------------------------------------------
+// MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs,
+// each with its own L2: consecutive block ids (neighbouring tiles) land on
+// different L2s, and every tile's halo is fetched from HBM once per XCD.
+// Tile kernels renumber their blocks so that each XCD works through one
+// contiguous range of tiles (x fastest, then y, then z); neighbouring tiles
+// then share an L2. The map is a bijection for any grid size.
+constexpr unsigned kXcds = 8;
+
+__device__ __forceinline__ void xcd_tile(int &bx, int &by, int &bz)
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned n = gx * gy * gridDim.z;
+    const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned x = lin % kXcds, m = lin / kXcds;
+    const unsigned per = n / kXcds, rem = n % kXcds;
+    const unsigned t = x * per + (x < rem ? x : rem) + m;
+    bx = (int)(t % gx);
+    const unsigned r = t / gx;
+    by = (int)(r % gy);
+    bz = (int)(r / gy);
+}
+
+// ------------------------------------------------------------------------
 // cell addressing
 // ------------------------------------------------------------------------
 template <typename T>

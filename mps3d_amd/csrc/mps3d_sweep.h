@@ -46,8 +46,10 @@ __global__ void __launch_bounds__(TX *TY) k_sweep(Lev<T> L, const T *__restrict_
     __shared__ T sr[4][P1];
 
     const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1;
-    const int kb = blockIdx.z * kz + 1;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int i0 = bx * TX + 1, j0 = by * TY + 1;
+    const int kb = bz * kz + 1;
     const int ke = min(kb + kz - 1, L.nz);
     const long long sy = L.sy, sz = L.sz;
     const T *__restrict__ Dg = L.D;
@@ -221,25 +223,34 @@ __device__ __forceinline__ double fast_rcp(double d)
     return r;
 }
 
+// compile-time register-set index
+template <int I>
+using SlotC = std::integral_constant<int, I>;
+
 // Fused red-black sweep in fast arithmetic (rows without convection).
 //
 // Same schedule as k_sweep: plane s's red cells (on the tile and its ring,
 // from old black values) and plane s-1's black cells (from new red values)
-// per step, two barriers per step. Rows are branch-free: each lane
-// precomputes, for its cells, the six face factors abv/(2d) with the faces
-// on the physical boundary masked to zero and the DIRC wall factor, so a row
-// is  w_f = (D_c + D_nb) h_f,  diag = sum w_f + D_c g + 1/dt - reac,
+// per step, two barriers per step. Lanes are assigned by colour so that no
+// wave runs both paths: the first (TX+2)(TY+2)/2 lanes each own one red cell
+// of the tile-plus-ring row pair (its x position alternates with the plane
+// parity) and write it out when it lies in the tile; the first TX*TY/2
+// lanes own one black cell of the tile in the same way. Rows are
+// branch-free: each lane precomputes the face factors abv/(2d) of its cells
+// (zero on the physical boundary) and their DIRC wall factors, so a row is
+// w_f = (D_c + D_nb) h_f,  diag = sum w_f + D_c g + 1/dt - reac,
 // u = (b + sum w_f u_nb) / diag.  Values outside the interior are staged as
 // zeros, so masked faces multiply zeros. LDS holds three u planes and five
 // D planes of the tile with a 2-cell halo and three planes of red results;
-// each lane's own b values (red and black cell) travel in a two-plane
-// register queue instead. Ring depths: step s stores u/D plane s+1 before
-// its first barrier, and its red pass writes red plane s after it. The u
-// slot it reuses (plane s-2) was last read by the red pass of step s-1 and
-// the red slot (plane s-3) by the black pass of step s-1, both behind a
-// barrier; the D slot it reuses was last read by the black pass of step
-// s-1 (planes s-3..s-1) with no barrier in between, hence five D planes.
-template <typename T, int OP, bool REAC, int TX, int TY>
+// each lane's own b values travel in registers. Ring depths: step s stores
+// u/D plane s+1 before its first barrier, and its red pass writes red plane
+// s after it. The u slot it reuses (plane s-2) was last read by the red pass
+// of step s-1 and the red slot (plane s-3) by the black pass of step s-1,
+// both behind a barrier; the D slot it reuses was last read by the black
+// pass of step s-1 (planes s-3..s-1) with no barrier in between, hence five
+// D planes. Global loads run PD steps ahead in two alternating register
+// sets.
+template <typename T, int OP, bool REAC, int TX, int TY, int PD>
 __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__restrict__ uin, T *__restrict__ uout,
                                                       const T *__restrict__ b, int kz)
 {
@@ -249,9 +260,11 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     constexpr int NT = TX * TY;
     constexpr int N2 = (P2 + NT - 1) / NT;
     constexpr int N1 = (P1 + NT - 1) / NT;
-    constexpr int RW = W1 / 2;
-    constexpr int NR = RW * H1;
-    static_assert(W1 % 2 == 0, "tile width must be even");
+    constexpr int RW = W1 / 2;   // red cells per ring row
+    constexpr int NR = RW * H1;  // red lanes
+    constexpr int BW = TX / 2;   // black cells per tile row
+    constexpr int NB = BW * TY;  // black lanes
+    static_assert(TX % 2 == 0, "tile width must be even");
     static_assert(NR <= NT, "one red cell per lane");
     constexpr int DR = 5;
     __shared__ T su[3][P2];
@@ -260,8 +273,10 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     __shared__ T sr[3][P1];
 
     const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1;
-    const int kb = blockIdx.z * kz + 1;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int i0 = bx * TX + 1, j0 = by * TY + 1;
+    const int kb = bz * kz + 1;
     const int ke = min(kb + kz - 1, L.nz);
     const int sy = L.sy;
     const long long sz = L.sz;
@@ -318,76 +333,103 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
         gz = g;
     };
 
-    // red lane: the red cell of E1 row jr for both plane parities
+    // Two-element per-lane quantities are scalar pairs (suffix 0/1 = the
+    // cell of plane parity 0/1), not arrays: a select between array
+    // elements may become a dynamic index, which moves the array out of
+    // registers.
+    //
+    // red lane: ring row jr, pair mr; its cell sits at ring column 2mr + p,
+    // p = (rpar + k) & 1 on plane k
     const bool red_lane = tid < NR;
     const int jr = tid / RW, mr = tid - jr * RW;
     const int jrg = j0 - 1 + jr;
-    int rq[2], re[2], ro[2];
-    T rfxm[2], rfxp[2], rfym, rfyp, rgxy[2];
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        const int ii = 2 * mr + p, i = i0 - 1 + ii;
-        rq[p] = ii + W1 * jr;
-        re[p] = (ii + 1) + W2 * (jr + 1);
-        const bool rin = red_lane && i >= 1 && i <= nx && jrg >= 1 && jrg <= ny;
-        ro[p] = rin ? i + sy * jrg : -1;
-        T a, c2;
-        xy_factors(i, jrg, rfxm[p], rfxp[p], a, c2, rgxy[p]);
-        rfym = a;
-        rfyp = c2;
-    }
-    const int rpar = (i0 + j0 + jr + 1) & 1;  // parity of the red cell at plane k: (rpar + k) & 1
-    // black lane: its tile cell
-    const int tx = tid % TX, ty = tid / TX;
-    const int bi = i0 + tx, bj = j0 + ty;
-    const bool bin = bi <= nx && bj <= ny;
-    const int q1c = (tx + 1) + W1 * (ty + 1);
-    const int e2c = (tx + 2) + W2 * (ty + 2);
-    T bfxm, bfxp, bfym, bfyp, bgxy;
-    xy_factors(bi, bj, bfxm, bfxp, bfym, bfyp, bgxy);
-    const int cb = bi + sy * bj;
+    const int rpar = (i0 + j0 + jr + 1) & 1;
+    const int ii0 = 2 * mr, ii1 = 2 * mr + 1;
+    const int i_r0 = i0 - 1 + ii0, i_r1 = i0 - 1 + ii1;
+    const int rq0 = ii0 + W1 * jr, rq1 = ii1 + W1 * jr;
+    const int re0 = (ii0 + 1) + W2 * (jr + 1), re1 = (ii1 + 1) + W2 * (jr + 1);
+    const bool jin = red_lane && jrg >= 1 && jrg <= ny;
+    const int ro0 = (jin && i_r0 >= 1 && i_r0 <= nx) ? i_r0 + sy * jrg : -1;
+    const int ro1 = (jin && i_r1 >= 1 && i_r1 <= nx) ? i_r1 + sy * jrg : -1;
+    // the red cell lies in this block's tile (it is written out by this lane)
+    const bool rowin = jr >= 1 && jr <= TY;
+    const bool rt0 = ro0 >= 0 && rowin && ii0 >= 1 && ii0 <= TX;
+    const bool rt1 = ro1 >= 0 && rowin && ii1 >= 1 && ii1 <= TX;
+    T rfxm0, rfxp0, rgxy0, rfxm1, rfxp1, rgxy1, rfym, rfyp;
+    xy_factors(i_r0, jrg, rfxm0, rfxp0, rfym, rfyp, rgxy0);
+    xy_factors(i_r1, jrg, rfxm1, rfxp1, rfym, rfyp, rgxy1);
+    // black lane: tile row yb, pair mb; its cell sits at tile column
+    // 2mb + pb, pb = (bpar + k) & 1 on plane k
+    const bool blk_lane = tid < NB;
+    const int yb = tid / BW, mb = tid - yb * BW;
+    const int bj = j0 + yb;
+    const int bpar = (i0 + bj) & 1;
+    const int tb0 = 2 * mb, tb1 = 2 * mb + 1;
+    const int bi0 = i0 + tb0, bi1 = i0 + tb1;
+    const bool bjin = blk_lane && bj <= ny;
+    const int cb0 = (bjin && bi0 <= nx) ? bi0 + sy * bj : -1;
+    const int cb1 = (bjin && bi1 <= nx) ? bi1 + sy * bj : -1;
+    const int bq0 = (tb0 + 1) + W1 * (yb + 1), bq1 = (tb1 + 1) + W1 * (yb + 1);
+    const int be0 = (tb0 + 2) + W2 * (yb + 2), be1 = (tb1 + 2) + W2 * (yb + 2);
+    T bfxm0, bfxp0, bgxy0, bfxm1, bfxp1, bgxy1, bfym, bfyp;
+    xy_factors(bi0, bj, bfxm0, bfxp0, bfym, bfyp, bgxy0);
+    xy_factors(bi1, bj, bfxm1, bfxp1, bfym, bfyp, bgxy1);
 
-    // gload(kp): u and D tiles of plane kp (for LDS), the red-cell b of
-    // plane kp and the black-cell b of plane kp - 1 (both used at step kp)
-    T pu[N2], pD[N2], pq[N1], nbr, nbb;
-    auto gload = [&](int kp) {
+    // gload(kp, X): into register set X, the u and D tiles of plane kp (for
+    // LDS), the red-cell b of plane kp - 1 and the black-cell b of plane
+    // kp - 2: everything step kp - 1 consumes. Step s consumes gload(s + 1)
+    // and issues gload(s + 1 + PD), so loads have PD steps to arrive; the
+    // two sets alternate with the step parity (the loop is unrolled by two,
+    // so no set is copied while its loads are in flight).
+    T pu[2][N2], pD[2][N2], pq[2][N1], nbr0, nbr1, nbb0, nbb1;
+    auto gload = [&](int kp, auto X) {
+        constexpr int x = decltype(X)::value;
         const bool ok = kp >= 1 && kp <= nz;
         const T *ub = uin + sz * kp;
         const T *Db = L.D + sz * kp;
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
-            pu[r] = T(0);
-            pD[r] = T(0);
+            pu[x][r] = T(0);
+            pD[x][r] = T(0);
             if (ok && o2[r] >= 0) {
-                pu[r] = ub[o2[r]];
-                pD[r] = Db[o2[r]];
+                pu[x][r] = ub[o2[r]];
+                pD[x][r] = Db[o2[r]];
             }
         }
         if (REAC) {
             const T *qb = L.reac + sz * kp;
 #pragma unroll
-            for (int r = 0; r < N1; ++r) pq[r] = (ok && o1[r] >= 0) ? qb[o1[r]] : T(0);
+            for (int r = 0; r < N1; ++r) pq[x][r] = (ok && o1[r] >= 0) ? qb[o1[r]] : T(0);
         }
-        const int orr = ro[(rpar + kp) & 1];
-        nbr = (ok && orr >= 0) ? b[sz * kp + orr] : T(0);
-        const int kbk = kp - 1;
-        nbb = (kbk >= 1 && kbk <= nz && bin) ? b[sz * kbk + cb] : T(0);
+        const int kr = kp - 1, kk = kp - 2;
+        const int orr = ((rpar + kr) & 1) ? ro1 : ro0;
+        const int obb = ((bpar + kk) & 1) ? cb1 : cb0;
+        const T vr = (kr >= 1 && kr <= nz && orr >= 0) ? b[sz * kr + orr] : T(0);
+        const T vb = (kk >= 1 && kk <= nz && obb >= 0) ? b[sz * kk + obb] : T(0);
+        if (x) {
+            nbr1 = vr;
+            nbb1 = vb;
+        } else {
+            nbr0 = vr;
+            nbb0 = vb;
+        }
     };
-    auto lstore = [&](int kp) {
+    auto lstore = [&](int kp, auto X) {
+        constexpr int x = decltype(X)::value;
         const int s3 = kp % 3, s5 = kp % DR;
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
             const int q = tid + r * NT;
             if (q < P2) {
-                su[s3][q] = pu[r];
-                sD[s5][q] = pD[r];
+                su[s3][q] = pu[x][r];
+                sD[s5][q] = pD[x][r];
             }
         }
         if (REAC) {
 #pragma unroll
             for (int r = 0; r < N1; ++r) {
                 const int q = tid + r * NT;
-                if (q < P1) sq[kp & 3][q] = pq[r];
+                if (q < P1) sq[kp & 3][q] = pq[x][r];
             }
         }
     };
@@ -412,62 +454,71 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     // planes are >= kb - 2 >= -1: offset the ring index by 3 * 5 so it stays
     // non-negative
     const int kofs = 15;
-    gload(kb - 2);
-    lstore(kb - 2 + kofs);
-    gload(kb - 1);
-    lstore(kb - 1 + kofs);
-    T br0 = nbr, bb0 = nbb;
-    gload(kb);
-    T br1 = nbr, bb1 = nbb;
-    for (int s = kb - 1; s <= ke + 1; ++s) {
-        lstore(s + 1 + kofs);
+    gload(kb - 2, SlotC<0>{});
+    lstore(kb - 2 + kofs, SlotC<0>{});
+    gload(kb - 1, SlotC<0>{});
+    lstore(kb - 1 + kofs, SlotC<0>{});
+    // set of gload(kp): (kp - kb) & 1
+    gload(kb, SlotC<0>{});
+    if (PD == 2) gload(kb + 1, SlotC<1>{});
+    auto step = [&](int s, auto X) {
+        constexpr int x = decltype(X)::value;
+        lstore(s + 1 + kofs, X);
+        const T br = x ? nbr1 : nbr0, bk = x ? nbb1 : nbb0;
         __syncthreads();
-        if (s <= ke) gload(s + 2);
+        if (PD == 2) {
+            if (s <= ke - 1) gload(s + 3, X);
+        } else {
+            if (s <= ke) gload(s + 2, SlotC<1 - x>{});
+        }
         if (s >= 1 && s <= nz) {
             const int p = (rpar + s) & 1;
-            const int orr = p ? ro[1] : ro[0];
+            const int orr = p ? ro1 : ro0;
             if (orr >= 0) {
                 T fzm, fzp, gz;
                 z_factors(s, fzm, fzp, gz);
-                const int q = p ? rq[1] : rq[0];
-                const int e = p ? re[1] : re[0];
+                const int q = p ? rq1 : rq0;
+                const int e = p ? re1 : re0;
                 const int t = s + kofs;
                 const T *um = su[(t - 1) % 3], *u0 = su[t % 3], *up = su[(t + 1) % 3];
                 const T *Dm = sD[(t - 1) % DR], *D0 = sD[t % DR], *Dp = sD[(t + 1) % DR];
                 const T rc = REAC ? sq[REAC ? (t & 3) : 0][q] : T(0);
-                sr[t % 3][q] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], u0[e - 1],
-                                        u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], p ? rfxm[1] : rfxm[0],
-                                        p ? rfxp[1] : rfxp[0], rfym, rfyp, fzm, fzp, (p ? rgxy[1] : rgxy[0]) + gz, rc,
-                                        br0);
+                const T val = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], u0[e - 1],
+                                       u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], p ? rfxm1 : rfxm0,
+                                       p ? rfxp1 : rfxp0, rfym, rfyp, fzm, fzp, (p ? rgxy1 : rgxy0) + gz, rc, br);
+                sr[t % 3][q] = val;
+                if ((p ? rt1 : rt0) && s >= kb && s <= ke) uout[orr + sz * s] = val;
             }
         }
         __syncthreads();
         const int k = s - 1;
-        if (k >= kb && k <= ke && bin) {
-            const int t = k + kofs;
-            T val;
-            if ((bi + bj + k) & 1) {
-                val = sr[t % 3][q1c];
-            } else {
+        if (k >= kb && k <= ke) {
+            const int pb = (bpar + k) & 1;
+            const int ob = pb ? cb1 : cb0;
+            if (ob >= 0) {
+                const int t = k + kofs;
+                const int q = pb ? bq1 : bq0;
+                const int e = pb ? be1 : be0;
+                const int bi = pb ? bi1 : bi0;
                 T fzm, fzp, gz;
                 z_factors(k, fzm, fzp, gz);
                 const T *rm = sr[(t - 1) % 3], *r0 = sr[t % 3], *rp = sr[(t + 1) % 3];
                 const T *Dm = sD[(t - 1) % DR], *D0 = sD[t % DR], *Dp = sD[(t + 1) % DR];
-                const T rc = REAC ? sq[REAC ? (t & 3) : 0][q1c] : T(0);
+                const T rc = REAC ? sq[REAC ? (t & 3) : 0][q] : T(0);
                 // red neighbours outside the interior hold stale values:
                 // their face factors are zero, so select them away
-                const T rzm = k > 1 ? rm[q1c] : T(0), rzp = k < nz ? rp[q1c] : T(0);
-                const T rxm = bi > 1 ? r0[q1c - 1] : T(0), rxp = bi < nx ? r0[q1c + 1] : T(0);
-                const T rym = bj > 1 ? r0[q1c - W1] : T(0), ryp = bj < ny ? r0[q1c + W1] : T(0);
-                val = fast_row(D0[e2c], D0[e2c - 1], D0[e2c + 1], D0[e2c - W2], D0[e2c + W2], Dm[e2c], Dp[e2c], rxm,
-                               rxp, rym, ryp, rzm, rzp, bfxm, bfxp, bfym, bfyp, fzm, fzp, bgxy + gz, rc, bb0);
+                const T rzm = k > 1 ? rm[q] : T(0), rzp = k < nz ? rp[q] : T(0);
+                const T rxm = bi > 1 ? r0[q - 1] : T(0), rxp = bi < nx ? r0[q + 1] : T(0);
+                const T rym = bj > 1 ? r0[q - W1] : T(0), ryp = bj < ny ? r0[q + W1] : T(0);
+                uout[ob + sz * k] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], rxm,
+                                             rxp, rym, ryp, rzm, rzp, pb ? bfxm1 : bfxm0, pb ? bfxp1 : bfxp0, bfym,
+                                             bfyp, fzm, fzp, (pb ? bgxy1 : bgxy0) + gz, rc, bk);
             }
-            uout[cb + sz * k] = val;
         }
-        br0 = br1;
-        bb0 = bb1;
-        br1 = nbr;
-        bb1 = nbb;
+    };
+    for (int s = kb - 1; s <= ke + 1; s += 2) {
+        step(s, SlotC<0>{});
+        if (s + 1 <= ke + 1) step(s + 1, SlotC<1>{});
     }
 }
 
@@ -476,139 +527,178 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
 namespace mps {
 
 // Residual r = b - A u of rows without convection in fast arithmetic (and
-// the block's sum of r^2), marching the z-planes of a TX x TY tile: each
-// lane keeps its own column's z neighbours in registers and reads its x/y
-// neighbours from a (TX+2) x (TY+2) plane in LDS (the halo ring, corners
-// excluded, is loaded by the first 2(TX+TY) lanes). The next plane's loads
-// are in flight while the current plane is evaluated; two LDS planes
-// alternate, so each plane costs one barrier. Values outside the interior
-// are staged as zeros and their face factors are zero. R is the stored
-// type of r (float for the FP32 defect of the FP64 iterate); r is nullable.
-template <typename T, typename R, int OP, bool REAC, int TX, int TY>
+// the block's sum of r^2), marching the z-planes of a TX x (TY*RB) tile:
+// each lane owns RB cells of its x column (rows ty, ty+TY, ...), keeps their
+// z neighbours in registers and reads x/y neighbours from a
+// (TX+2) x (TY*RB+2) plane in LDS (the halo ring, corners excluded, is
+// loaded by the first lanes). The next plane's loads are in flight while
+// the current plane is evaluated; two LDS planes alternate, so each plane
+// costs one barrier, and RB cells per lane put RB times the loads in flight
+// per barrier. Values outside the interior are staged as zeros and their
+// face factors are zero. R is the stored type of r (float for the FP32
+// defect of the FP64 iterate); r is nullable.
+template <typename T, typename R, int OP, bool REAC, int TX, int TY, int RB>
 __global__ void __launch_bounds__(TX *TY) k_residual_planes(Lev<T> L, const T *__restrict__ u,
                                                            const T *__restrict__ b, R *__restrict__ r,
                                                            double *partials, int kz)
 {
 #pragma clang fp contract(fast)
-    constexpr int W = TX + 2, H = TY + 2, NT = TX * TY;
-    constexpr int NH = 2 * (TX + TY);
-    static_assert(NH <= NT, "halo ring must fit one cell per lane");
+    constexpr int HY = TY * RB;
+    constexpr int W = TX + 2, H = HY + 2, NT = TX * TY;
+    constexpr int NH = 2 * (TX + HY);
+    constexpr int NHL = (NH + NT - 1) / NT;  // halo cells per lane
     static_assert(NT <= kBlock, "block_sum holds kBlock / 64 waves");
     __shared__ T su[2][W * H];
     __shared__ T sD[2][W * H];
 
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
-    const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1;
-    const int kb = blockIdx.z * kz + 1, ke = min(kb + kz - 1, L.nz);
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int i0 = bx * TX + 1, j0 = by * HY + 1;
+    const int kb = bz * kz + 1, ke = min(kb + kz - 1, L.nz);
     const int nx = L.nx, ny = L.ny, nz = L.nz, sy = L.sy;
     const long long sz = L.sz;
-    const int i = i0 + tx, j = j0 + ty;
-    const bool in = i <= nx && j <= ny;
-    const int oc = in ? i + sy * j : -1;
-    const int q = (tx + 1) + W * (ty + 1);
-    int oh = -1, qh = 0;
-    if (tid < NH) {
-        int hi, hj;
-        if (tid < TX) {
-            hi = tid, hj = -1;
-        } else if (tid < 2 * TX) {
-            hi = tid - TX, hj = TY;
-        } else if (tid < 2 * TX + TY) {
-            hi = -1, hj = tid - 2 * TX;
-        } else {
-            hi = TX, hj = tid - 2 * TX - TY;
-        }
-        const int gi = i0 + hi, gj = j0 + hj;
-        qh = (hi + 1) + W * (hj + 1);
-        if (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny) oh = gi + sy * gj;
-    }
+    const int i = i0 + tx;
 
-    // own face factors (zero on the physical boundary) and x/y DIRC factor
-    T fxm, fxp, fym, fyp, gxy;
-    {
+    int oc[RB], q[RB];
+    bool in[RB];
+    T fxm[RB], fxp[RB], fym[RB], fyp[RB], gxy[RB];
+#pragma unroll
+    for (int m = 0; m < RB; ++m) {
+        const int yy = ty + m * TY, j = j0 + yy;
+        in[m] = i <= nx && j <= ny;
+        oc[m] = in[m] ? i + sy * j : -1;
+        q[m] = (tx + 1) + W * (yy + 1);
         const bool ex = OP == OP_ALIGNED && (i == 1 || i == nx);
         const bool ey = OP == OP_ALIGNED && (j == 1 || j == ny);
         const T hx = ex ? L.hxb : L.hx, hy = ey ? L.hyb : L.hy;
-        fxm = i > 1 ? hx : T(0);
-        fxp = i < nx ? hx : T(0);
-        fym = j > 1 ? hy : T(0);
-        fyp = j < ny ? hy : T(0);
+        fxm[m] = i > 1 ? hx : T(0);
+        fxp[m] = i < nx ? hx : T(0);
+        fym[m] = j > 1 ? hy : T(0);
+        fyp[m] = j < ny ? hy : T(0);
         const T gx = ex ? L.gxb : L.gx, gy = ey ? L.gyb : L.gy;
         T g = T(0);
         if (i == 1 && L.bc[LEFT] == BC_DIRC) g += gx;
         if (i == nx && L.bc[RIGHT] == BC_DIRC) g += gx;
         if (j == 1 && L.bc[BOTTOM] == BC_DIRC) g += gy;
         if (j == ny && L.bc[TOP] == BC_DIRC) g += gy;
-        gxy = g;
+        gxy[m] = g;
+    }
+    int oh[NHL], qh[NHL];
+#pragma unroll
+    for (int h = 0; h < NHL; ++h) {
+        const int t = tid + h * NT;
+        oh[h] = -1;
+        qh[h] = -1;
+        if (t < NH) {
+            int hi, hj;
+            if (t < TX) {
+                hi = t, hj = -1;
+            } else if (t < 2 * TX) {
+                hi = t - TX, hj = HY;
+            } else if (t < 2 * TX + HY) {
+                hi = -1, hj = t - 2 * TX;
+            } else {
+                hi = TX, hj = t - 2 * TX - HY;
+            }
+            const int gi = i0 + hi, gj = j0 + hj;
+            qh[h] = (hi + 1) + W * (hj + 1);
+            if (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny) oh[h] = gi + sy * gj;
+        }
     }
 
-    T pu, pD, pb, pq, phu, phD;
+    T pu[RB], pD[RB], pb[RB], pq[RB], phu[NHL], phD[NHL];
     auto gload = [&](int k) {
         const bool ok = k >= 1 && k <= nz;
         const long long base = sz * k;
-        pu = pD = pb = pq = T(0);
-        if (ok && oc >= 0) {
-            pu = u[base + oc];
-            pD = L.D[base + oc];
-            pb = b[base + oc];
-            if (REAC) pq = L.reac[base + oc];
+#pragma unroll
+        for (int m = 0; m < RB; ++m) {
+            pu[m] = pD[m] = pb[m] = pq[m] = T(0);
+            if (ok && oc[m] >= 0) {
+                pu[m] = u[base + oc[m]];
+                pD[m] = L.D[base + oc[m]];
+                pb[m] = b[base + oc[m]];
+                if (REAC) pq[m] = L.reac[base + oc[m]];
+            }
         }
-        phu = phD = T(0);
-        if (ok && oh >= 0) {
-            phu = u[base + oh];
-            phD = L.D[base + oh];
+#pragma unroll
+        for (int h = 0; h < NHL; ++h) {
+            phu[h] = phD[h] = T(0);
+            if (ok && oh[h] >= 0) {
+                phu[h] = u[base + oh[h]];
+                phD[h] = L.D[base + oh[h]];
+            }
         }
     };
 
+    T um[RB], Dm[RB], uc[RB], Dc[RB], bc[RB], qc[RB], hu[NHL], hD[NHL];
     gload(kb - 1);
-    T um = pu, Dm = pD;
+#pragma unroll
+    for (int m = 0; m < RB; ++m) um[m] = pu[m], Dm[m] = pD[m];
     gload(kb);
-    T uc = pu, Dc = pD, bc = pb, qc = pq, hu = phu, hD = phD;
+#pragma unroll
+    for (int m = 0; m < RB; ++m) uc[m] = pu[m], Dc[m] = pD[m], bc[m] = pb[m], qc[m] = pq[m];
+#pragma unroll
+    for (int h = 0; h < NHL; ++h) hu[h] = phu[h], hD[h] = phD[h];
     gload(kb + 1);
     double sq = 0.0;
     for (int k = kb; k <= ke; ++k) {
         const int s = k & 1;
-        su[s][q] = uc;
-        sD[s][q] = Dc;
-        if (tid < NH) {
-            su[s][qh] = hu;
-            sD[s][qh] = hD;
+#pragma unroll
+        for (int m = 0; m < RB; ++m) {
+            su[s][q[m]] = uc[m];
+            sD[s][q[m]] = Dc[m];
+        }
+#pragma unroll
+        for (int h = 0; h < NHL; ++h) {
+            if (qh[h] >= 0) {
+                su[s][qh[h]] = hu[h];
+                sD[s][qh[h]] = hD[h];
+            }
         }
         __syncthreads();
-        const T up = pu, Dp = pD, bn = pb, qn = pq, hun = phu, hDn = phD;
+        T up[RB], Dp[RB], bn[RB], qn[RB], hun[NHL], hDn[NHL];
+#pragma unroll
+        for (int m = 0; m < RB; ++m) up[m] = pu[m], Dp[m] = pD[m], bn[m] = pb[m], qn[m] = pq[m];
+#pragma unroll
+        for (int h = 0; h < NHL; ++h) hun[h] = phu[h], hDn[h] = phD[h];
         if (k < ke) gload(k + 2);
-        if (in) {
-            const bool ez = OP == OP_ALIGNED && (k == 1 || k == nz);
-            const T hz = ez ? L.hzb : L.hz;
-            const T fzm = k > 1 ? hz : T(0), fzp = k < nz ? hz : T(0);
-            T g = gxy;
-            if (k == 1 && L.bc[BACK] == BC_DIRC) g += ez ? L.gzb : L.gz;
-            if (k == nz && L.bc[FRONT] == BC_DIRC) g += ez ? L.gzb : L.gz;
-            const T *U = su[s], *Dl = sD[s];
-            const T wxm = (Dl[q - 1] + Dc) * fxm, wxp = (Dc + Dl[q + 1]) * fxp;
-            const T wym = (Dl[q - W] + Dc) * fym, wyp = (Dc + Dl[q + W]) * fyp;
-            const T wzm = (Dm + Dc) * fzm, wzp = (Dc + Dp) * fzp;
-            T d = ((wxm + wxp) + (wym + wyp)) + (wzm + wzp);
-            d = d + Dc * g + L.dtinv;
-            if (REAC) d = d - qc;
-            T nb = wxm * U[q - 1];
-            nb = nb + wxp * U[q + 1];
-            nb = nb + wym * U[q - W];
-            nb = nb + wyp * U[q + W];
-            nb = nb + wzm * um;
-            nb = nb + wzp * up;
-            const T rv = bc - (d * uc - nb);
-            if (r) r[sz * k + oc] = (R)rv;
-            sq += (double)rv * (double)rv;
+        const bool ez = OP == OP_ALIGNED && (k == 1 || k == nz);
+        const T hz = ez ? L.hzb : L.hz;
+        const T fzm = k > 1 ? hz : T(0), fzp = k < nz ? hz : T(0);
+        T gz = T(0);
+        if (k == 1 && L.bc[BACK] == BC_DIRC) gz += ez ? L.gzb : L.gz;
+        if (k == nz && L.bc[FRONT] == BC_DIRC) gz += ez ? L.gzb : L.gz;
+        const T *U = su[s], *Dl = sD[s];
+#pragma unroll
+        for (int m = 0; m < RB; ++m) {
+            if (in[m]) {
+                const int qq = q[m];
+                const T D0 = Dc[m];
+                const T wxm = (Dl[qq - 1] + D0) * fxm[m], wxp = (D0 + Dl[qq + 1]) * fxp[m];
+                const T wym = (Dl[qq - W] + D0) * fym[m], wyp = (D0 + Dl[qq + W]) * fyp[m];
+                const T wzm = (Dm[m] + D0) * fzm, wzp = (D0 + Dp[m]) * fzp;
+                T d = ((wxm + wxp) + (wym + wyp)) + (wzm + wzp);
+                d = d + D0 * (gxy[m] + gz) + L.dtinv;
+                if (REAC) d = d - qc[m];
+                T nb = wxm * U[qq - 1];
+                nb = nb + wxp * U[qq + 1];
+                nb = nb + wym * U[qq - W];
+                nb = nb + wyp * U[qq + W];
+                nb = nb + wzm * um[m];
+                nb = nb + wzp * up[m];
+                const T rv = bc[m] - (d * uc[m] - nb);
+                if (r) r[sz * k + oc[m]] = (R)rv;
+                sq += (double)rv * (double)rv;
+            }
         }
-        um = uc;
-        Dm = Dc;
-        uc = up;
-        Dc = Dp;
-        bc = bn;
-        qc = qn;
-        hu = hun;
-        hD = hDn;
+#pragma unroll
+        for (int m = 0; m < RB; ++m) {
+            um[m] = uc[m], Dm[m] = Dc[m];
+            uc[m] = up[m], Dc[m] = Dp[m], bc[m] = bn[m], qc[m] = qn[m];
+        }
+#pragma unroll
+        for (int h = 0; h < NHL; ++h) hu[h] = hun[h], hD[h] = hDn[h];
     }
     if (partials) {
         const double t = block_sum(sq);
@@ -637,8 +727,10 @@ __global__ void __launch_bounds__(CX *CY) k_restrict_fw_planes(Lev<T> F, Lev<T> 
     constexpr int NS = (FP + NT - 1) / NT;
     __shared__ T sf[2][FP];
     const int tid = threadIdx.x, tx = tid % CX, ty = tid / CX;
-    const int I0 = blockIdx.x * CX + 1, J0 = blockIdx.y * CY + 1;
-    const int Kb = blockIdx.z * kz + 1, Ke = min(Kb + kz - 1, C.nz);
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int I0 = bx * CX + 1, J0 = by * CY + 1;
+    const int Kb = bz * kz + 1, Ke = min(Kb + kz - 1, C.nz);
     const int I = I0 + tx, J = J0 + ty;
     const bool in = I <= C.nx && J <= C.ny;
     int of[NS];
@@ -714,9 +806,11 @@ __global__ void __launch_bounds__(TX *TY) k_prolong_tri_planes(Lev<T> F, Lev<T> 
                                                               const T *__restrict__ uc, int kz)
 {
     const int tid = threadIdx.x;
-    const int i = blockIdx.x * TX + tid % TX + 1, j = blockIdx.y * TY + tid / TX + 1;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int i = bx * TX + tid % TX + 1, j = by * TY + tid / TX + 1;
     if (i > F.nx || j > F.ny) return;
-    const int kb = blockIdx.z * kz + 1, ke = min(kb + kz - 1, F.nz);
+    const int kb = bz * kz + 1, ke = min(kb + kz - 1, F.nz);
     const int ia = (i + 1) >> 1, ja = (j + 1) >> 1;
     const int ib = (i & 1) ? ia : ia + 1, jb = (j & 1) ? ja : ja + 1;
     const bool xo = i & 1, yo = j & 1;
