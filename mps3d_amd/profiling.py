@@ -18,6 +18,7 @@ counter bytes, separately for reads and writes.
 Usage:
     python -m mps3d_amd.profiling stats <kernel_stats.csv> [<kernel_trace.csv>]
     python -m mps3d_amd.profiling traffic <fetch.csv> <write.csv> --kernel NAME [--grid THREADS]
+    python -m mps3d_amd.profiling table <fetch.csv> <write.csv> NAME [NAME ...]
 """
 from __future__ import annotations
 
@@ -100,6 +101,32 @@ def traffic(fetch_csv, write_csv, kernel, grid=None):
     return out
 
 
+def traffic_table(fetch_csv, write_csv, kernels):
+    """Markdown table of scaled HBM bytes per dispatch for each kernel name
+    filter, at the launch shape that reads the most (the finest level; grid
+    sizes alone do not say which level a launch is, since the z-chunking
+    differs per level)."""
+    disp = _dispatches(fetch_csv, "FETCH_SIZE")
+    out = ["| kernel | grid (threads) | dispatches | read MB | write MB | total MB |", "|---|---|---|---|---|---|"]
+    for k in kernels:
+        per = defaultdict(list)
+        for n, g, v in disp.values():
+            if k in n:
+                per[g].append(v)
+        if not per:
+            continue
+        g = max(per, key=lambda x: sum(per[x]) / len(per[x]))
+        t = traffic(fetch_csv, write_csv, k, g)
+        out.append("| `%s` | %d | %d | %.1f | %.1f | %.1f |" % (
+            k, g, t["fetch"]["dispatches"], t["fetch"]["bytes"] / 1e6, t["write"]["bytes"] / 1e6,
+            t["traffic_bytes"] / 1e6))
+    t = traffic(fetch_csv, write_csv, COPY_KERNEL)
+    out.append("")
+    out.append("Calibration copy (%d bytes read and written): FETCH_SIZE factor %.4f, WRITE_SIZE factor %.4f." % (
+        COPY_BYTES, t["fetch"]["factor"], t["write"]["factor"]))
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m mps3d_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -111,8 +138,14 @@ def main(argv=None):
     t.add_argument("write_csv")
     t.add_argument("--kernel", required=True)
     t.add_argument("--grid", type=int, default=None, help="total grid size in threads (one launch shape)")
+    tt = sub.add_parser("table")
+    tt.add_argument("fetch_csv")
+    tt.add_argument("write_csv")
+    tt.add_argument("kernels", nargs="+", help="kernel name filters")
     a = ap.parse_args(argv)
-    if a.cmd == "stats":
+    if a.cmd == "table":
+        print(traffic_table(a.fetch_csv, a.write_csv, a.kernels))
+    elif a.cmd == "stats":
         print(stats_table(a.stats_csv, a.trace_csv))
     else:
         print(json.dumps(traffic(a.fetch_csv, a.write_csv, a.kernel, a.grid), indent=1))
