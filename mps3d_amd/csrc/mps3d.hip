@@ -644,7 +644,7 @@ void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *wo
 #define MPS_SWEEP_KZ 16
 #endif
 #ifndef MPS_SWEEP_PD
-#define MPS_SWEEP_PD 2
+#define MPS_SWEEP_PD 1
 #endif
 constexpr int kSweepTX = MPS_SWEEP_TX, kSweepTY = MPS_SWEEP_TY, kSweepKZ = MPS_SWEEP_KZ;
 
@@ -677,7 +677,8 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
 template <typename T>
 bool fused_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op)
 {
-    return op != OP_STENCIL && !c->has_vel && L[l].u2;
+    // (the fused kernels address planes with 32-bit offsets)
+    return op != OP_STENCIL && !c->has_vel && L[l].u2 && c->geom[l].ng() < (1LL << 31);
 }
 
 // nu red-black sweeps on level l: pairs of fused sweeps u -> u2 -> u where

@@ -282,7 +282,10 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     const long long sz = L.sz;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
 
-    // staging offsets (interior positions only; -1 elsewhere)
+    // staging offsets (interior positions only; -1 elsewhere). Loads are
+    // issued for every lane from a valid address (the offset clamped to 0,
+    // the plane clamped to the ghost planes) and the value selected away,
+    // so the staging has no divergent branches.
     int o2[N2], o1[N1];
 #pragma unroll
     for (int r = 0; r < N2; ++r) {
@@ -380,56 +383,65 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     // kp - 2: everything step kp - 1 consumes. Step s consumes gload(s + 1)
     // and issues gload(s + 1 + PD), so loads have PD steps to arrive; the
     // two sets alternate with the step parity (the loop is unrolled by two,
-    // so no set is copied while its loads are in flight).
+    // so no set is copied while its loads are in flight). Plane offsets are
+    // 32-bit (the host keeps ghosted fields below 2^31 cells).
+    const int szi = (int)sz;
+    auto plane = [&](int kp) { return min(max(kp, 0), nz + 1) * szi; };
+    bool on2[N2], on1[N1];
+#pragma unroll
+    for (int r = 0; r < N2; ++r) on2[r] = o2[r] >= 0;
+#pragma unroll
+    for (int r = 0; r < N1; ++r) on1[r] = o1[r] >= 0;
+    // raw loads; the validity flags travel beside them and are applied
+    // where the values are consumed (a select right after a load would wait
+    // for it and void the prefetch)
     T pu[2][N2], pD[2][N2], pq[2][N1], nbr0, nbr1, nbb0, nbb1;
+    bool okp0 = false, okp1 = false, rok0 = false, rok1 = false, bok0 = false, bok1 = false;
     auto gload = [&](int kp, auto X) {
         constexpr int x = decltype(X)::value;
-        const bool ok = kp >= 1 && kp <= nz;
-        const T *ub = uin + sz * kp;
-        const T *Db = L.D + sz * kp;
+        const int pk = plane(kp);
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
-            pu[x][r] = T(0);
-            pD[x][r] = T(0);
-            if (ok && o2[r] >= 0) {
-                pu[x][r] = ub[o2[r]];
-                pD[x][r] = Db[o2[r]];
-            }
+            const int o = pk + (on2[r] ? o2[r] : 0);
+            pu[x][r] = uin[o];
+            pD[x][r] = L.D[o];
         }
         if (REAC) {
-            const T *qb = L.reac + sz * kp;
 #pragma unroll
-            for (int r = 0; r < N1; ++r) pq[x][r] = (ok && o1[r] >= 0) ? qb[o1[r]] : T(0);
+            for (int r = 0; r < N1; ++r) pq[x][r] = L.reac[pk + (on1[r] ? o1[r] : 0)];
         }
         const int kr = kp - 1, kk = kp - 2;
         const int orr = ((rpar + kr) & 1) ? ro1 : ro0;
         const int obb = ((bpar + kk) & 1) ? cb1 : cb0;
-        const T vr = (kr >= 1 && kr <= nz && orr >= 0) ? b[sz * kr + orr] : T(0);
-        const T vb = (kk >= 1 && kk <= nz && obb >= 0) ? b[sz * kk + obb] : T(0);
+        const T vr = b[plane(kr) + (orr >= 0 ? orr : 0)];
+        const T vb = b[plane(kk) + (obb >= 0 ? obb : 0)];
+        const bool ok = kp >= 1 && kp <= nz;
+        const bool rok = kr >= 1 && kr <= nz && orr >= 0;
+        const bool bok = kk >= 1 && kk <= nz && obb >= 0;
         if (x) {
-            nbr1 = vr;
-            nbb1 = vb;
+            nbr1 = vr, nbb1 = vb, okp1 = ok, rok1 = rok, bok1 = bok;
         } else {
-            nbr0 = vr;
-            nbb0 = vb;
+            nbr0 = vr, nbb0 = vb, okp0 = ok, rok0 = rok, bok0 = bok;
         }
     };
-    auto lstore = [&](int kp, auto X) {
+    // stores plane kp (u into ring slot s3, D into s5, reac into s4)
+    auto lstore = [&](int s3, int s5, int s4, auto X) {
         constexpr int x = decltype(X)::value;
-        const int s3 = kp % 3, s5 = kp % DR;
+        const bool ok = x ? okp1 : okp0;
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
             const int q = tid + r * NT;
-            if (q < P2) {
-                su[s3][q] = pu[x][r];
-                sD[s5][q] = pD[x][r];
+            if (r * NT + NT <= P2 || q < P2) {
+                const bool on = ok && on2[r];
+                su[s3][q] = on ? pu[x][r] : T(0);
+                sD[s5][q] = on ? pD[x][r] : T(0);
             }
         }
         if (REAC) {
 #pragma unroll
             for (int r = 0; r < N1; ++r) {
                 const int q = tid + r * NT;
-                if (q < P1) sq[kp & 3][q] = pq[x][r];
+                if (q < P1) sq[s4][q] = (ok && on1[r]) ? pq[x][r] : T(0);
             }
         }
     };
@@ -451,43 +463,66 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
         return s * fast_rcp(d);
     };
 
-    // planes are >= kb - 2 >= -1: offset the ring index by 3 * 5 so it stays
-    // non-negative
-    const int kofs = 15;
+    // Ring slots advance by one per step: u3/d5/q4 are the slots of plane s
+    // in the u and red rings (3), the D ring (5) and the reac ring (4).
+    auto next3 = [](int a) { return a == 2 ? 0 : a + 1; };
+    auto prev3 = [](int a) { return a == 0 ? 2 : a - 1; };
+    auto next5 = [](int a) { return a == 4 ? 0 : a + 1; };
+    auto prev5 = [](int a) { return a == 0 ? 4 : a - 1; };
+    // z face factors of interior planes; planes 1 and nz get their own
+    T fzm_i, fzp_i, gz_i;
+    z_factors(2, fzm_i, fzp_i, gz_i);
+    if (nz < 3) z_factors(nz, fzm_i, fzp_i, gz_i);
+    auto zf = [&](int k, T &fzm, T &fzp, T &gz) {
+        if (k == 1 || k == nz) {
+            z_factors(k, fzm, fzp, gz);
+        } else {
+            fzm = fzm_i, fzp = fzp_i, gz = gz_i;
+        }
+    };
+    // slots of plane kb - 2 (the first one staged)
+    int u3 = 0, d5 = 0, q4 = 0;
     gload(kb - 2, SlotC<0>{});
-    lstore(kb - 2 + kofs, SlotC<0>{});
+    lstore(u3, d5, q4, SlotC<0>{});
     gload(kb - 1, SlotC<0>{});
-    lstore(kb - 1 + kofs, SlotC<0>{});
+    lstore(next3(u3), next5(d5), (q4 + 1) & 3, SlotC<0>{});
+    // now plane s = kb - 1 has slots next3(0), next5(0), 1
+    u3 = 1, d5 = 1, q4 = 1;
     // set of gload(kp): (kp - kb) & 1
     gload(kb, SlotC<0>{});
     if (PD == 2) gload(kb + 1, SlotC<1>{});
     auto step = [&](int s, auto X) {
         constexpr int x = decltype(X)::value;
-        lstore(s + 1 + kofs, X);
-        const T br = x ? nbr1 : nbr0, bk = x ? nbb1 : nbb0;
+        const int u3p = next3(u3), u3m = prev3(u3);
+        const int d5p = next5(d5), d5m = prev5(d5), d5mm = prev5(d5m);
+        lstore(u3p, d5p, (q4 + 1) & 3, X);
+        const T br = (x ? rok1 : rok0) ? (x ? nbr1 : nbr0) : T(0);
+        const T bk = (x ? bok1 : bok0) ? (x ? nbb1 : nbb0) : T(0);
         __syncthreads();
+        // the prefetch is issued on every step (past the last plane this
+        // block needs it re-reads plane ke + 2, an L2 hit), so the loop body
+        // is straight-line and the compiler keeps the register sets in place
         if (PD == 2) {
-            if (s <= ke - 1) gload(s + 3, X);
+            gload(min(s + 3, ke + 2), X);
         } else {
-            if (s <= ke) gload(s + 2, SlotC<1 - x>{});
+            gload(min(s + 2, ke + 2), SlotC<1 - x>{});
         }
         if (s >= 1 && s <= nz) {
             const int p = (rpar + s) & 1;
             const int orr = p ? ro1 : ro0;
             if (orr >= 0) {
                 T fzm, fzp, gz;
-                z_factors(s, fzm, fzp, gz);
+                zf(s, fzm, fzp, gz);
                 const int q = p ? rq1 : rq0;
                 const int e = p ? re1 : re0;
-                const int t = s + kofs;
-                const T *um = su[(t - 1) % 3], *u0 = su[t % 3], *up = su[(t + 1) % 3];
-                const T *Dm = sD[(t - 1) % DR], *D0 = sD[t % DR], *Dp = sD[(t + 1) % DR];
-                const T rc = REAC ? sq[REAC ? (t & 3) : 0][q] : T(0);
+                const T *um = su[u3m], *u0 = su[u3], *up = su[u3p];
+                const T *Dm = sD[d5m], *D0 = sD[d5], *Dp = sD[d5p];
+                const T rc = REAC ? sq[REAC ? q4 : 0][q] : T(0);
                 const T val = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], u0[e - 1],
                                        u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], p ? rfxm1 : rfxm0,
                                        p ? rfxp1 : rfxp0, rfym, rfyp, fzm, fzp, (p ? rgxy1 : rgxy0) + gz, rc, br);
-                sr[t % 3][q] = val;
-                if ((p ? rt1 : rt0) && s >= kb && s <= ke) uout[orr + sz * s] = val;
+                sr[u3][q] = val;
+                if ((p ? rt1 : rt0) && s >= kb && s <= ke) uout[orr + szi * s] = val;
             }
         }
         __syncthreads();
@@ -496,30 +531,35 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
             const int pb = (bpar + k) & 1;
             const int ob = pb ? cb1 : cb0;
             if (ob >= 0) {
-                const int t = k + kofs;
                 const int q = pb ? bq1 : bq0;
                 const int e = pb ? be1 : be0;
                 const int bi = pb ? bi1 : bi0;
                 T fzm, fzp, gz;
-                z_factors(k, fzm, fzp, gz);
-                const T *rm = sr[(t - 1) % 3], *r0 = sr[t % 3], *rp = sr[(t + 1) % 3];
-                const T *Dm = sD[(t - 1) % DR], *D0 = sD[t % DR], *Dp = sD[(t + 1) % DR];
-                const T rc = REAC ? sq[REAC ? (t & 3) : 0][q] : T(0);
+                zf(k, fzm, fzp, gz);
+                // red planes k-1, k, k+1 = s-2, s-1, s: slots u3p, u3m, u3
+                const T *rm = sr[u3p], *r0 = sr[u3m], *rp = sr[u3];
+                const T *Dm = sD[d5mm], *D0 = sD[d5m], *Dp = sD[d5];
+                const T rc = REAC ? sq[REAC ? ((q4 + 3) & 3) : 0][q] : T(0);
                 // red neighbours outside the interior hold stale values:
                 // their face factors are zero, so select them away
                 const T rzm = k > 1 ? rm[q] : T(0), rzp = k < nz ? rp[q] : T(0);
                 const T rxm = bi > 1 ? r0[q - 1] : T(0), rxp = bi < nx ? r0[q + 1] : T(0);
                 const T rym = bj > 1 ? r0[q - W1] : T(0), ryp = bj < ny ? r0[q + W1] : T(0);
-                uout[ob + sz * k] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], rxm,
-                                             rxp, rym, ryp, rzm, rzp, pb ? bfxm1 : bfxm0, pb ? bfxp1 : bfxp0, bfym,
-                                             bfyp, fzm, fzp, (pb ? bgxy1 : bgxy0) + gz, rc, bk);
+                uout[ob + szi * k] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], rxm,
+                                              rxp, rym, ryp, rzm, rzp, pb ? bfxm1 : bfxm0, pb ? bfxp1 : bfxp0, bfym,
+                                              bfyp, fzm, fzp, (pb ? bgxy1 : bgxy0) + gz, rc, bk);
             }
         }
+        u3 = u3p;
+        d5 = d5p;
+        q4 = (q4 + 1) & 3;
     };
-    for (int s = kb - 1; s <= ke + 1; s += 2) {
+    int s = kb - 1;
+    for (; s + 1 <= ke + 1; s += 2) {
         step(s, SlotC<0>{});
-        if (s + 1 <= ke + 1) step(s + 1, SlotC<1>{});
+        step(s + 1, SlotC<1>{});
     }
+    if (s <= ke + 1) step(s, SlotC<0>{});
 }
 
 }  // namespace mps
