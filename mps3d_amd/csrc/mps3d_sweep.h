@@ -378,6 +378,28 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     xy_factors(bi0, bj, bfxm0, bfxp0, bfym, bfyp, bgxy0);
     xy_factors(bi1, bj, bfxm1, bfxp1, bfym, bfyp, bgxy1);
 
+    // The loop runs step s in register slot (s - kb + 1) & 1, so each slot
+    // always sees the same plane parity, and each lane the same red and
+    // black cell: resolve the column parity per slot once, here, instead of
+    // selecting on every step. Slot y's red cell is column parity
+    // (rpar + kb - 1 + y) & 1, its black cell (bpar + kb - 2 + y) & 1.
+    struct Cell {
+        int q, e, o;    // LDS index (ring plane), LDS index (halo-2 plane), field offset (-1: none)
+        bool tile;      // red cell inside this block's tile (written out)
+        int i;          // x index (black cells)
+        T fxm, fxp, g;  // x face factors and x/y DIRC factor
+    };
+    auto red_cell = [&](int pc) {
+        return pc ? Cell{rq1, re1, ro1, rt1, i_r1, rfxm1, rfxp1, rgxy1} : Cell{rq0, re0, ro0, rt0, i_r0, rfxm0, rfxp0, rgxy0};
+    };
+    auto blk_cell = [&](int pc) {
+        return pc ? Cell{bq1, be1, cb1, false, bi1, bfxm1, bfxp1, bgxy1}
+                  : Cell{bq0, be0, cb0, false, bi0, bfxm0, bfxp0, bgxy0};
+    };
+    const int pr = (rpar + kb - 1) & 1, pbk = (bpar + kb - 2) & 1;
+    const Cell R0 = red_cell(pr), R1 = red_cell(pr ^ 1);
+    const Cell B0 = blk_cell(pbk), B1 = blk_cell(pbk ^ 1);
+
     // gload(kp, X): into register set X, the u and D tiles of plane kp (for
     // LDS), the red-cell b of plane kp - 1 and the black-cell b of plane
     // kp - 2: everything step kp - 1 consumes. Step s consumes gload(s + 1)
@@ -410,9 +432,11 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
 #pragma unroll
             for (int r = 0; r < N1; ++r) pq[x][r] = L.reac[pk + (on1[r] ? o1[r] : 0)];
         }
+        // (gload(kp) fills set (kp - kb) & 1, the slot of step kp - 1 that
+        // consumes it; outside that pattern the b values are not consumed)
         const int kr = kp - 1, kk = kp - 2;
-        const int orr = ((rpar + kr) & 1) ? ro1 : ro0;
-        const int obb = ((bpar + kk) & 1) ? cb1 : cb0;
+        const int orr = x ? R1.o : R0.o;
+        const int obb = x ? B1.o : B0.o;
         const T vr = b[plane(kr) + (orr >= 0 ? orr : 0)];
         const T vb = b[plane(kk) + (obb >= 0 ? obb : 0)];
         const bool ok = kp >= 1 && kp <= nz;
@@ -507,48 +531,40 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
         } else {
             gload(min(s + 2, ke + 2), SlotC<1 - x>{});
         }
-        if (s >= 1 && s <= nz) {
-            const int p = (rpar + s) & 1;
-            const int orr = p ? ro1 : ro0;
-            if (orr >= 0) {
+        // red cells of plane s; red cells outside the interior (or planes
+        // outside 1..nz) hold 0 in the red ring, so the black pass reads its
+        // neighbours without masking them (their face factors are zero)
+        if (red_lane) {
+            const Cell &C = x ? R1 : R0;
+            T val = T(0);
+            if (s >= 1 && s <= nz && C.o >= 0) {
                 T fzm, fzp, gz;
                 zf(s, fzm, fzp, gz);
-                const int q = p ? rq1 : rq0;
-                const int e = p ? re1 : re0;
+                const int e = C.e;
                 const T *um = su[u3m], *u0 = su[u3], *up = su[u3p];
                 const T *Dm = sD[d5m], *D0 = sD[d5], *Dp = sD[d5p];
-                const T rc = REAC ? sq[REAC ? q4 : 0][q] : T(0);
-                const T val = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], u0[e - 1],
-                                       u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], p ? rfxm1 : rfxm0,
-                                       p ? rfxp1 : rfxp0, rfym, rfyp, fzm, fzp, (p ? rgxy1 : rgxy0) + gz, rc, br);
-                sr[u3][q] = val;
-                if ((p ? rt1 : rt0) && s >= kb && s <= ke) uout[orr + szi * s] = val;
+                const T rc = REAC ? sq[REAC ? q4 : 0][C.q] : T(0);
+                val = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], u0[e - 1], u0[e + 1],
+                               u0[e - W2], u0[e + W2], um[e], up[e], C.fxm, C.fxp, rfym, rfyp, fzm, fzp, C.g + gz, rc,
+                               br);
+                if (C.tile && s >= kb && s <= ke) uout[C.o + szi * s] = val;
             }
+            sr[u3][C.q] = val;
         }
         __syncthreads();
         const int k = s - 1;
-        if (k >= kb && k <= ke) {
-            const int pb = (bpar + k) & 1;
-            const int ob = pb ? cb1 : cb0;
-            if (ob >= 0) {
-                const int q = pb ? bq1 : bq0;
-                const int e = pb ? be1 : be0;
-                const int bi = pb ? bi1 : bi0;
-                T fzm, fzp, gz;
-                zf(k, fzm, fzp, gz);
-                // red planes k-1, k, k+1 = s-2, s-1, s: slots u3p, u3m, u3
-                const T *rm = sr[u3p], *r0 = sr[u3m], *rp = sr[u3];
-                const T *Dm = sD[d5mm], *D0 = sD[d5m], *Dp = sD[d5];
-                const T rc = REAC ? sq[REAC ? ((q4 + 3) & 3) : 0][q] : T(0);
-                // red neighbours outside the interior hold stale values:
-                // their face factors are zero, so select them away
-                const T rzm = k > 1 ? rm[q] : T(0), rzp = k < nz ? rp[q] : T(0);
-                const T rxm = bi > 1 ? r0[q - 1] : T(0), rxp = bi < nx ? r0[q + 1] : T(0);
-                const T rym = bj > 1 ? r0[q - W1] : T(0), ryp = bj < ny ? r0[q + W1] : T(0);
-                uout[ob + szi * k] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], rxm,
-                                              rxp, rym, ryp, rzm, rzp, pb ? bfxm1 : bfxm0, pb ? bfxp1 : bfxp0, bfym,
-                                              bfyp, fzm, fzp, (pb ? bgxy1 : bgxy0) + gz, rc, bk);
-            }
+        const Cell &Bc = x ? B1 : B0;
+        if (k >= kb && k <= ke && Bc.o >= 0) {
+            const int q = Bc.q, e = Bc.e;
+            T fzm, fzp, gz;
+            zf(k, fzm, fzp, gz);
+            // red planes k-1, k, k+1 = s-2, s-1, s: slots u3p, u3m, u3
+            const T *rm = sr[u3p], *r0 = sr[u3m], *rp = sr[u3];
+            const T *Dm = sD[d5mm], *D0 = sD[d5m], *Dp = sD[d5];
+            const T rc = REAC ? sq[REAC ? ((q4 + 3) & 3) : 0][q] : T(0);
+            uout[Bc.o + szi * k] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e],
+                                            r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q], rp[q], Bc.fxm, Bc.fxp,
+                                            bfym, bfyp, fzm, fzp, Bc.g + gz, rc, bk);
         }
         u3 = u3p;
         d5 = d5p;
