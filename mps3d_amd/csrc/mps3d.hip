@@ -763,6 +763,56 @@ void derive_coarse(mps_ctx *c)
     }
 }
 
+#ifndef MPS_CONDUCTANCE
+#define MPS_CONDUCTANCE 1
+#endif
+// Aligned hierarchy, rows without convection: the coarse levels take
+// 7-point rows built from conductance-coarsened face coefficients
+// (k_face_means, k_coarsen_faces, k_faces_stencil) instead of rows from
+// restricted cell coefficients; other setups keep their level operators.
+void conductance_coarse_levels(mps_ctx *c)
+{
+    auto &L = c->L64;
+    const bool on = MPS_CONDUCTANCE && c->mg_ok && c->opts.hierarchy == MPS_HIER_ALIGNED &&
+                    c->opts.flux != MPS_FLUX_SG && !c->has_vel && c->nlev() > 1;
+    for (int l = 1; l < c->nlev(); ++l) {
+        L[l].op = on ? OP_STENCIL : -1;
+        if (l < (int)c->L32.size()) c->L32[l].op = L[l].op;
+    }
+    if (!on) return;
+    std::vector<double *> f(3 * (size_t)c->nlev(), nullptr);
+    try {
+        for (int l = 0; l < c->nlev(); ++l) {
+            for (int d = 0; d < 3; ++d) {
+                f[3 * l + d] = dalloc<double>(c->geom[l].ng());
+                fill<double>(c, f[3 * l + d], c->geom[l].ng(), 0.0);
+            }
+        }
+        hipLaunchKernelGGL(k_face_means, cells_grid(c->geom[0]), block(), 0, c->stream, view(c, L, 0), f[0], f[1],
+                           f[2]);
+        for (int l = 1; l < c->nlev(); ++l) {
+            const Geom &g = c->geom[l];
+            AlignedW W{L[l].w[0], L[l].w[1], L[l].w[2]};
+            hipLaunchKernelGGL(k_coarsen_faces, cells_grid(g), block(), 0, c->stream, view(c, L, l - 1), view(c, L, l),
+                               W, (const double *)f[3 * l - 3], (const double *)f[3 * l - 2],
+                               (const double *)f[3 * l - 1], f[3 * l], f[3 * l + 1], f[3 * l + 2]);
+            LevelBuf<double> &B = L[l];
+            for (int q = 0; q < ST_N; ++q) {
+                if (!B.st[q]) B.st[q] = dalloc<double>(g.ng());
+                fill<double>(c, B.st[q], g.ng(), 0.0);
+            }
+            hipLaunchKernelGGL(k_faces_stencil, cells_grid(g), block(), 0, c->stream, view(c, L, l),
+                               (const double *)f[3 * l], (const double *)f[3 * l + 1], (const double *)f[3 * l + 2],
+                               B.st[0], B.st[1], B.st[2], B.st[3], B.st[4], B.st[5], B.st[6]);
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        for (auto p : f) dfree(p);
+        throw;
+    }
+    for (auto p : f) dfree(p);
+}
+
 void build_stencils(mps_ctx *c, std::vector<LevelBuf<double>> &L, bool transpose)
 {
     for (int l = 0; l < c->nlev(); ++l) {
@@ -1081,7 +1131,7 @@ void build_bottom_inverse(mps_ctx *c)
     const long long ng = g.ng();
     std::vector<std::vector<double>> st(ST_N, std::vector<double>(ng, 0.0));
     auto &B = c->L64[lb];
-    if (level_op(c, lb) == OP_STENCIL) {
+    if (level_op(c, c->L64, lb) == OP_STENCIL) {
         for (int q = 0; q < ST_N; ++q)
             HIPCHK(hipMemcpyAsync(st[q].data(), B.st[q], ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     } else {
@@ -1431,6 +1481,7 @@ int mps_set_coeffs(mps_ctx *c, const double *dcoeff, const double *velx, const d
         if (source) stage_in(c, source, c->source, ng);
         else fill<double>(c, c->source, ng, 0.0);
         derive_coarse(c);
+        conductance_coarse_levels(c);
         for (int l = 0; l < c->nlev(); ++l) {
             if (level_op(c, l) == OP_STENCIL) {
                 build_stencils(c, L, false);

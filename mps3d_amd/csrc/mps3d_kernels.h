@@ -810,6 +810,127 @@ __global__ void __launch_bounds__(kBlock) k_build_stencil(Lev<T> L, T *st0, T *s
 namespace mps {
 
 // ------------------------------------------------------------------------
+// conductance coarsening for the aligned hierarchy (rows without
+// convection)
+// ------------------------------------------------------------------------
+// face means of the finest level: f_d[c] = (D[c] + D[c + s_d]) / 2 on the +
+// face of cell c in direction d (the reference face rule), 0 on the
+// boundary
+__global__ void __launch_bounds__(kBlock) k_face_means(Lev<double> L, double *fx, double *fy, double *fz)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    const double *D = L.D;
+    fx[c] = i < L.nx ? 0.5 * (D[c] + D[c + 1]) : 0.0;
+    fy[c] = j < L.ny ? 0.5 * (D[c] + D[c + L.sy]) : 0.0;
+    fz[c] = k < L.nz ? 0.5 * (D[c] + D[c + L.sz]) : 0.0;
+}
+
+__device__ __forceinline__ double series(double a, double b) { return (a > 0.0 && b > 0.0) ? 2.0 * a * b / (a + b) : 0.0; }
+
+// Coarse face coefficients: the + face of coarse cell I in x joins coarse
+// centres that sit on fine centres 2I-1 and 2I+1, so it spans the fine
+// faces (2I-1|2I) and (2I|2I+1) in series (the harmonic mean of their face
+// coefficients: equal conductance over twice the distance); across the face
+// the fine rows 2J-2..2J and planes 2K-2..2K conduct in parallel, weighted
+// like the restriction. For a coefficient jump on a fine face this keeps
+// the coarse conductance exact, where restricting cell coefficients smears
+// the jump over the coarse cell.
+__global__ void __launch_bounds__(kBlock) k_coarsen_faces(Lev<double> F, Lev<double> C, AlignedW W, const double *ffx,
+                                                         const double *ffy, const double *ffz, double *cfx,
+                                                         double *cfy, double *cfz)
+{
+    int I, J, K;
+    long long c;
+    if (!cell_of(C, I, J, K, c)) return;
+    const long long sy = F.sy, sz = F.sz;
+    auto at = [&](int a, int b, int d) { return (long long)a + sy * b + sz * d; };
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int p = -1; p <= 1; ++p) {
+        for (int q = -1; q <= 1; ++q) {
+            if (I < C.nx) {  // x face: transverse (j, k)
+                const double w = W.wy[3 * (J - 1) + p + 1] * W.wz[3 * (K - 1) + q + 1];
+                if (w != 0.0) {
+                    const int fj = 2 * J - 1 + p, fk = 2 * K - 1 + q;
+                    ax += w * series(ffx[at(2 * I - 1, fj, fk)], ffx[at(2 * I, fj, fk)]);
+                }
+            }
+            if (J < C.ny) {  // y face: transverse (i, k)
+                const double w = W.wx[3 * (I - 1) + p + 1] * W.wz[3 * (K - 1) + q + 1];
+                if (w != 0.0) {
+                    const int fi = 2 * I - 1 + p, fk = 2 * K - 1 + q;
+                    ay += w * series(ffy[at(fi, 2 * J - 1, fk)], ffy[at(fi, 2 * J, fk)]);
+                }
+            }
+            if (K < C.nz) {  // z face: transverse (i, j)
+                const double w = W.wx[3 * (I - 1) + p + 1] * W.wy[3 * (J - 1) + q + 1];
+                if (w != 0.0) {
+                    const int fi = 2 * I - 1 + p, fj = 2 * J - 1 + q;
+                    az += w * series(ffz[at(fi, fj, 2 * K - 1)], ffz[at(fi, fj, 2 * K)]);
+                }
+            }
+        }
+    }
+    cfx[c] = ax;
+    cfy[c] = ay;
+    cfz[c] = az;
+}
+
+// 7-point rows of an aligned coarse level from its face coefficients: face
+// weight (f / d) * abv, with abv the reciprocal width of the row's cell;
+// DIRC walls D_c * idist * abv (the cell coefficient at the wall); then
+// -reac and +1/dt, as the cell-coefficient rows of the aligned grids
+__global__ void __launch_bounds__(kBlock) k_faces_stencil(Lev<double> L, const double *fx, const double *fy,
+                                                         const double *fz, double *st0, double *st1, double *st2,
+                                                         double *st3, double *st4, double *st5, double *st6)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    double d = 0.0, w;
+    const double Dc = L.D[c];
+    {
+        const double abv = (i == 1 || i == L.nx) ? L.axb : L.ax;
+        w = i > 1 ? fx[c - 1] / L.dx * abv : 0.0;
+        st1[c] = -w;
+        d += w;
+        w = i < L.nx ? fx[c] / L.dx * abv : 0.0;
+        st2[c] = -w;
+        d += w;
+        if (i == 1 && L.bc[LEFT] == BC_DIRC) d += Dc * L.ix0 * abv;
+        if (i == L.nx && L.bc[RIGHT] == BC_DIRC) d += Dc * L.ix0 * abv;
+    }
+    {
+        const long long s = L.sy;
+        const double abv = (j == 1 || j == L.ny) ? L.ayb : L.ay;
+        w = j > 1 ? fy[c - s] / L.dy * abv : 0.0;
+        st3[c] = -w;
+        d += w;
+        w = j < L.ny ? fy[c] / L.dy * abv : 0.0;
+        st4[c] = -w;
+        d += w;
+        if (j == 1 && L.bc[BOTTOM] == BC_DIRC) d += Dc * L.iy0 * abv;
+        if (j == L.ny && L.bc[TOP] == BC_DIRC) d += Dc * L.iy0 * abv;
+    }
+    {
+        const long long s = L.sz;
+        const double abv = (k == 1 || k == L.nz) ? L.azb : L.az;
+        w = k > 1 ? fz[c - s] / L.dz * abv : 0.0;
+        st5[c] = -w;
+        d += w;
+        w = k < L.nz ? fz[c] / L.dz * abv : 0.0;
+        st6[c] = -w;
+        d += w;
+        if (k == 1 && L.bc[BACK] == BC_DIRC) d += Dc * L.iz0 * abv;
+        if (k == L.nz && L.bc[FRONT] == BC_DIRC) d += Dc * L.iz0 * abv;
+    }
+    if (L.reac) d -= L.reac[c];
+    if (L.timederiv) d += 1.0 / L.dt;
+    st0[c] = d;
+}
+
+// ------------------------------------------------------------------------
 // Krylov vector kernels (FP64): fused multi-dot and multi-axpy over the
 // interior of ghosted-layout vectors, with block partial sums reduced in a
 // fixed order by k_reduce_cols.

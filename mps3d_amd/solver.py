@@ -84,6 +84,15 @@ class FieldSolver:
         self.opts = o
         self.max_cycles = int(max_cycles)
         h = C.c_void_p()
+        # torch (which owns the device tensors passed in) initialises the HIP
+        # runtime before the library does; in the other order torch has been
+        # seen to report no device
+        try:
+            import torch
+        except ImportError:
+            torch = None
+        if torch is not None and torch.cuda.device_count() > 0:
+            torch.cuda.init()
         M.check(self.lib.mps_create(C.byref(h), C.byref(g), C.byref(o)))
         self._h = h
         self._keep = []
