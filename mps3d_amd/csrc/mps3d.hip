@@ -7,6 +7,7 @@
 #include "mps3d_bottom.h"
 #include "mps3d_kernels.h"
 #include "mps3d_sweep.h"
+#include "mps3d_lds_cycle.h"
 
 #include <hip/hip_runtime.h>
 
@@ -699,6 +700,53 @@ void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op
     }
 }
 
+// ---------------------------------------------- LDS-resident small levels
+// k_lds_cycle takes over the V-cycle from the first level whose levels down
+// to the coarsest fit kLdsBudget bytes of LDS: aligned hierarchy with
+// explicit-stencil (conductance) coarse rows and a dense coarsest solve.
+constexpr long long kLdsBudget = 64 * 1024;
+
+template <typename T>
+int lds_cycle_from(const mps_ctx *c, const std::vector<LevelBuf<T>> &L)
+{
+    if (c->opts.hierarchy != MPS_HIER_ALIGNED || !L.back().inv) return -1;
+    long long words = 0;
+    int from = -1;
+    for (int l = c->nlev() - 1; l >= 1; --l) {
+        if (level_op(c, L, l) != OP_STENCIL) break;
+        const Geom &g = c->geom[l];
+        words += lds_level_words(g.nx, g.ny, g.nz);
+        if (words * (long long)sizeof(T) > kLdsBudget || c->nlev() - l > kLdsMaxLev) break;
+        from = l;
+    }
+    return (from >= 0 && from < c->nlev() - 1) ? from : -1;
+}
+
+template <typename T>
+void launch_lds_cycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l)
+{
+    LdsLevels<T> S{};
+    S.nlev = c->nlev() - l;
+    long long words = 0;
+    for (int q = 0; q < S.nlev; ++q) {
+        const Geom &g = c->geom[l + q];
+        S.nx[q] = g.nx, S.ny[q] = g.ny, S.nz[q] = g.nz;
+        for (int s = 0; s < ST_N; ++s) S.st[q][s] = L[l + q].st[s];
+        S.W[q] = AlignedW{L[l + q].w[0], L[l + q].w[1], L[l + q].w[2]};
+        words += lds_level_words(g.nx, g.ny, g.nz);
+    }
+    const Geom &g0 = c->geom[l];
+    S.b0 = L[l].b;
+    S.u0 = L[l].u;
+    S.sy0 = g0.nx + 2;
+    S.sz0 = (long long)(g0.nx + 2) * (g0.ny + 2);
+    S.inv = L.back().inv;
+    S.nu1 = c->opts.nu1;
+    S.nu2 = c->opts.nu2;
+    S.fast = view(c, L, l).fast;
+    hipLaunchKernelGGL(k_lds_cycle<T>, dim3(1), dim3(kLdsBlock), (size_t)(words * sizeof(T)), c->stream, S);
+}
+
 // --------------------------------------------------------------- V-cycle
 // One V-cycle from level l down (perform_vcycle, mgridsteps.f90:384-579).
 // Level l's b must hold its right-hand side.
@@ -709,6 +757,10 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
     const int op = level_op(c, L, l);
     if (g.bottom) {
         launch_bottom<T>(c, L, l, op, bwork);
+        return;
+    }
+    if (l > 0 && l == lds_cycle_from<T>(c, L)) {
+        launch_lds_cycle<T>(c, L, l);
         return;
     }
     smooth_level<T>(c, L, l, c->opts.nu1, op);
