@@ -666,7 +666,13 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
             hipLaunchKernelGGL((k_sweep<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
                                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
     };
-    if (op == OP_REF) {
+    if (op == OP_STENCIL) {
+        // explicit rows carry their own reaction and time terms; the fast
+        // kernel's arithmetic for them is the per-level kernels' in either
+        // arithmetic mode
+        hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, MPS_SWEEP_PD>), gr,
+                           dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
+    } else if (op == OP_REF) {
         if (c->has_reac) go(IntC<OP_REF>{}, BoolC<true>{});
         else go(IntC<OP_REF>{}, BoolC<false>{});
     } else {
@@ -678,8 +684,9 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
 template <typename T>
 bool fused_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op)
 {
-    // (the fused kernels address planes with 32-bit offsets)
-    return op != OP_STENCIL && !c->has_vel && L[l].u2 && c->geom[l].ng() < (1LL << 31);
+    // (the fused kernels address planes with 32-bit offsets; explicit rows
+    // are fused only without convection, i.e. the conductance coarse rows)
+    return !c->has_vel && L[l].u2 && c->geom[l].ng() < (1LL << 31);
 }
 
 // nu red-black sweeps on level l: pairs of fused sweeps u -> u2 -> u where

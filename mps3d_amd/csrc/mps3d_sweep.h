@@ -267,8 +267,9 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     static_assert(TX % 2 == 0, "tile width must be even");
     static_assert(NR <= NT, "one red cell per lane");
     constexpr int DR = 5;
+    constexpr bool STN = OP == OP_STENCIL;  // explicit 7-point rows (no D tiles)
     __shared__ T su[3][P2];
-    __shared__ T sD[DR][P2];
+    __shared__ T sD[STN ? 1 : DR][STN ? 1 : P2];
     __shared__ T sq[REAC ? 4 : 1][REAC ? P1 : 1];
     __shared__ T sr[3][P1];
 
@@ -418,6 +419,8 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     // where the values are consumed (a select right after a load would wait
     // for it and void the prefetch)
     T pu[2][N2], pD[2][N2], pq[2][N1], nbr0, nbr1, nbb0, nbb1;
+    // stencil rows: the 7 coefficients of the slot's red and black cell
+    T rs0[STN ? ST_N : 1], rs1[STN ? ST_N : 1], bs0[STN ? ST_N : 1], bs1[STN ? ST_N : 1];
     bool okp0 = false, okp1 = false, rok0 = false, rok1 = false, bok0 = false, bok1 = false;
     auto gload = [&](int kp, auto X) {
         constexpr int x = decltype(X)::value;
@@ -426,9 +429,9 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
         for (int r = 0; r < N2; ++r) {
             const int o = pk + (on2[r] ? o2[r] : 0);
             pu[x][r] = uin[o];
-            pD[x][r] = L.D[o];
+            if (!STN) pD[x][r] = L.D[o];
         }
-        if (REAC) {
+        if (REAC && !STN) {
 #pragma unroll
             for (int r = 0; r < N1; ++r) pq[x][r] = L.reac[pk + (on1[r] ? o1[r] : 0)];
         }
@@ -437,8 +440,22 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
         const int kr = kp - 1, kk = kp - 2;
         const int orr = x ? R1.o : R0.o;
         const int obb = x ? B1.o : B0.o;
-        const T vr = b[plane(kr) + (orr >= 0 ? orr : 0)];
-        const T vb = b[plane(kk) + (obb >= 0 ? obb : 0)];
+        const int ar = plane(kr) + (orr >= 0 ? orr : 0), ab = plane(kk) + (obb >= 0 ? obb : 0);
+        const T vr = b[ar];
+        const T vb = b[ab];
+        if constexpr (STN) {
+#pragma unroll
+            for (int q = 0; q < ST_N; ++q) {
+                const T sr_ = L.st[q][ar], sb_ = L.st[q][ab];
+                if (x) {
+                    rs1[q] = sr_;
+                    bs1[q] = sb_;
+                } else {
+                    rs0[q] = sr_;
+                    bs0[q] = sb_;
+                }
+            }
+        }
         const bool ok = kp >= 1 && kp <= nz;
         const bool rok = kr >= 1 && kr <= nz && orr >= 0;
         const bool bok = kk >= 1 && kk <= nz && obb >= 0;
@@ -458,10 +475,10 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
             if (r * NT + NT <= P2 || q < P2) {
                 const bool on = ok && on2[r];
                 su[s3][q] = on ? pu[x][r] : T(0);
-                sD[s5][q] = on ? pD[x][r] : T(0);
+                if (!STN) sD[s5][q] = on ? pD[x][r] : T(0);
             }
         }
-        if (REAC) {
+        if (REAC && !STN) {
 #pragma unroll
             for (int r = 0; r < N1; ++r) {
                 const int q = tid + r * NT;
@@ -487,6 +504,18 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
         return s * fast_rcp(d);
     };
 
+    // one stencil row (the arithmetic of row() and gs_update for explicit
+    // rows): neighbours in the order x-, x+, y-, y+, z-, z+
+    auto stencil_row = [&](const T *st, T uxm, T uxp, T uym, T uyp, T uzm, T uzp, T bv) {
+        T o = T(0);
+        o = o + st[ST_XM] * uxm;
+        o = o + st[ST_XP] * uxp;
+        o = o + st[ST_YM] * uym;
+        o = o + st[ST_YP] * uyp;
+        o = o + st[ST_ZM] * uzm;
+        o = o + st[ST_ZP] * uzp;
+        return gs_update(L, bv, o, st[ST_C]);
+    };
     // Ring slots advance by one per step: u3/d5/q4 are the slots of plane s
     // in the u and red rings (3), the D ring (5) and the reac ring (4).
     auto next3 = [](int a) { return a == 2 ? 0 : a + 1; };
@@ -542,11 +571,15 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
                 zf(s, fzm, fzp, gz);
                 const int e = C.e;
                 const T *um = su[u3m], *u0 = su[u3], *up = su[u3p];
-                const T *Dm = sD[d5m], *D0 = sD[d5], *Dp = sD[d5p];
-                const T rc = REAC ? sq[REAC ? q4 : 0][C.q] : T(0);
-                val = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], u0[e - 1], u0[e + 1],
-                               u0[e - W2], u0[e + W2], um[e], up[e], C.fxm, C.fxp, rfym, rfyp, fzm, fzp, C.g + gz, rc,
-                               br);
+                if constexpr (STN) {
+                    val = stencil_row(x ? rs1 : rs0, u0[e - 1], u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], br);
+                } else {
+                    const T *Dm = sD[d5m], *D0 = sD[d5], *Dp = sD[d5p];
+                    const T rc = REAC ? sq[REAC ? q4 : 0][C.q] : T(0);
+                    val = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], u0[e - 1],
+                                   u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], C.fxm, C.fxp, rfym, rfyp, fzm, fzp,
+                                   C.g + gz, rc, br);
+                }
                 if (C.tile && s >= kb && s <= ke) uout[C.o + szi * s] = val;
             }
             sr[u3][C.q] = val;
@@ -560,11 +593,16 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
             zf(k, fzm, fzp, gz);
             // red planes k-1, k, k+1 = s-2, s-1, s: slots u3p, u3m, u3
             const T *rm = sr[u3p], *r0 = sr[u3m], *rp = sr[u3];
-            const T *Dm = sD[d5mm], *D0 = sD[d5m], *Dp = sD[d5];
-            const T rc = REAC ? sq[REAC ? ((q4 + 3) & 3) : 0][q] : T(0);
-            uout[Bc.o + szi * k] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e],
-                                            r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q], rp[q], Bc.fxm, Bc.fxp,
-                                            bfym, bfyp, fzm, fzp, Bc.g + gz, rc, bk);
+            if constexpr (STN) {
+                uout[Bc.o + szi * k] = stencil_row(x ? bs1 : bs0, r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q],
+                                                   rp[q], bk);
+            } else {
+                const T *Dm = sD[d5mm], *D0 = sD[d5m], *Dp = sD[d5];
+                const T rc = REAC ? sq[REAC ? ((q4 + 3) & 3) : 0][q] : T(0);
+                uout[Bc.o + szi * k] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e],
+                                                r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q], rp[q], Bc.fxm,
+                                                Bc.fxp, bfym, bfyp, fzm, fzp, Bc.g + gz, rc, bk);
+            }
         }
         u3 = u3p;
         d5 = d5p;
