@@ -402,6 +402,9 @@ int build_geometry(mps_ctx *c)
 }
 
 template <typename T>
+void fill(mps_ctx *c, T *p, long long n, T v);
+
+template <typename T>
 void alloc_levels(mps_ctx *c, std::vector<LevelBuf<T>> &L, bool with_coeffs)
 {
     L.assign(c->geom.size(), LevelBuf<T>{});
@@ -411,6 +414,9 @@ void alloc_levels(mps_ctx *c, std::vector<LevelBuf<T>> &L, bool with_coeffs)
         LevelBuf<T> &B = L[l];
         B.u = dalloc<T>(ng);
         B.u2 = dalloc<T>(ng);
+        // ghost layers stay zero from here on (kernels write interiors only)
+        fill<T>(c, B.u, ng, T(0));
+        fill<T>(c, B.u2, ng, T(0));
         B.b = dalloc<T>(ng);
         B.r = dalloc<T>(ng);
         if (l > 0) B.uold = dalloc<T>(ng);
@@ -525,7 +531,13 @@ dim3 residual_planes_grid(const Geom &g, int &kz)
 // aligned-hierarchy transfers (plane-marching kernels): restriction of fine
 // level l's r into level l+1's b, prolongation-correction of level l+1's u
 // into level l's u
-constexpr int kRestCX = 32, kRestCY = 8, kProlTX = 64, kProlTY = 4;
+#ifndef MPS_PROL_TX
+#define MPS_PROL_TX 64
+#endif
+#ifndef MPS_PROL_TY
+#define MPS_PROL_TY 4
+#endif
+constexpr int kRestCX = 32, kRestCY = 8, kProlTX = MPS_PROL_TX, kProlTY = MPS_PROL_TY;
 
 template <typename T>
 void launch_restrict_fw(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l)
@@ -649,19 +661,27 @@ void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *wo
 #endif
 constexpr int kSweepTX = MPS_SWEEP_TX, kSweepTY = MPS_SWEEP_TY, kSweepKZ = MPS_SWEEP_KZ;
 
+// zero_in: sweep from a zero guess, uin unread (fused fast kernels only)
 template <typename T>
-void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, T *uout, int op)
+void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, T *uout, int op, bool zero_in = false)
 {
     Lev<T> v = view(c, L, l);
     const Geom &g = c->geom[l];
-    const int kz = z_chunk(g, kSweepTX, kSweepTY, 2048, kSweepKZ);
+#ifndef MPS_SWEEP_TARGET
+#define MPS_SWEEP_TARGET 2048
+#endif
+    const int kz = z_chunk(g, kSweepTX, kSweepTY, MPS_SWEEP_TARGET, kSweepKZ);
     dim3 gr((unsigned)((g.nx + kSweepTX - 1) / kSweepTX), (unsigned)((g.ny + kSweepTY - 1) / kSweepTY),
             (unsigned)((g.nz + kz - 1) / kz));
     auto go = [&](auto O, auto R) {
-        if (v.fast)
-            hipLaunchKernelGGL((k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, MPS_SWEEP_PD>),
-                               gr,
-                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
+        if (v.fast && zero_in)
+            hipLaunchKernelGGL(
+                (k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, MPS_SWEEP_PD, true>), gr,
+                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
+        else if (v.fast)
+            hipLaunchKernelGGL(
+                (k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, MPS_SWEEP_PD, false>), gr,
+                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
         else
             hipLaunchKernelGGL((k_sweep<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
                                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
@@ -670,8 +690,12 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
         // explicit rows carry their own reaction and time terms; the fast
         // kernel's arithmetic for them is the per-level kernels' in either
         // arithmetic mode
-        hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, MPS_SWEEP_PD>), gr,
-                           dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
+        if (zero_in)
+            hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, MPS_SWEEP_PD, true>), gr,
+                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
+        else
+            hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, MPS_SWEEP_PD, false>), gr,
+                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
     } else if (op == OP_REF) {
         if (c->has_reac) go(IntC<OP_REF>{}, BoolC<true>{});
         else go(IntC<OP_REF>{}, BoolC<false>{});
@@ -692,12 +716,12 @@ bool fused_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op
 // nu red-black sweeps on level l: pairs of fused sweeps u -> u2 -> u where
 // the operator has no convection, in-place colour passes otherwise
 template <typename T>
-void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op)
+void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op, bool zero_first = false)
 {
     int s = 0;
     if (fused_ok(c, L, l, op)) {
         for (; s + 2 <= nu; s += 2) {
-            launch_sweep<T>(c, L, l, L[l].u, L[l].u2, op);
+            launch_sweep<T>(c, L, l, L[l].u, L[l].u2, op, zero_first && s == 0);
             launch_sweep<T>(c, L, l, L[l].u2, L[l].u, op);
         }
     }
@@ -705,6 +729,16 @@ void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op
         launch_colour<T>(c, L, l, 1, op);
         launch_colour<T>(c, L, l, 0, op);
     }
+}
+
+// A level whose correction starts at zero can skip zero-filling u when its
+// first pre-smoothing sweep is a fused fast sweep from a zero guess (the
+// ping-pong buffers' ghost layers are zeroed once at allocation and only
+// interiors are written afterwards).
+template <typename T>
+bool zero_first_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op)
+{
+    return c->opts.nu1 >= 2 && fused_ok(c, L, l, op) && (op == OP_STENCIL || view(c, L, l).fast);
 }
 
 // ---------------------------------------------- LDS-resident small levels
@@ -758,11 +792,15 @@ void launch_lds_cycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l)
 // One V-cycle from level l down (perform_vcycle, mgridsteps.f90:384-579).
 // Level l's b must hold its right-hand side.
 template <typename T>
-void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog)
+// zero_u: level l's correction starts at zero and L[l].u is not yet
+// zeroed (the correction scheme of the aligned hierarchy): the level zeroes
+// it, or starts from a zero-guess sweep, or solves it outright.
+void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog, bool zero_u = false)
 {
     const Geom &g = c->geom[l];
     const int op = level_op(c, L, l);
     if (g.bottom) {
+        if (zero_u && !L[l].inv) fill<T>(c, L[l].u, g.ng(), T(0));  // the iterative bottom solve reads u
         launch_bottom<T>(c, L, l, op, bwork);
         return;
     }
@@ -770,7 +808,12 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
         launch_lds_cycle<T>(c, L, l);
         return;
     }
-    smooth_level<T>(c, L, l, c->opts.nu1, op);
+    bool zg = false;
+    if (zero_u) {
+        zg = zero_first_ok<T>(c, L, l, op);
+        if (!zg) fill<T>(c, L[l].u, g.ng(), T(0));
+    }
+    smooth_level<T>(c, L, l, c->opts.nu1, op, zg);
     launch_residual<T, T>(c, L, l, L[l].u, L[l].b, L[l].r, nullptr, op);
     const Geom &gc = c->geom[l + 1];
     Lev<T> vf = view(c, L, l), vc = view(c, L, l + 1);
@@ -789,8 +832,7 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
                            (const T *)L[l + 1].u, (const T *)L[l + 1].uold, G);
     } else {
         launch_restrict_fw<T>(c, L, l);
-        fill<T>(c, L[l + 1].u, gc.ng(), T(0));
-        vcycle<T>(c, L, l + 1, bwork, homog);
+        vcycle<T>(c, L, l + 1, bwork, homog, true);
         launch_prolong_tri<T>(c, L, l);
     }
     smooth_level<T>(c, L, l, c->opts.nu2, op);
@@ -1053,8 +1095,7 @@ void defect_step(mps_ctx *c, double *u, bool do_cycle, const double *b = nullptr
     hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials, np,
                        c->d_scalar);
     if (!do_cycle) return;
-    fill<float>(c, F[0].u, g.ng(), 0.0f);
-    vcycle<float>(c, F, 0, c->bottom32, true);
+    vcycle<float>(c, F, 0, c->bottom32, true, true);
     Lev<double> v64 = view(c, c->L64, 0);
     hipLaunchKernelGGL((k_add_interior<double, float>), cells_grid(g), block(), 0, c->stream, v64, u,
                        (const float *)F[0].u);
@@ -1086,8 +1127,7 @@ void b_sumsq(mps_ctx *c, const double *b)
 void outer_step_body(mps_ctx *c, double *u, double *b, bool homog, bool dc)
 {
     if (dc) {
-        fill<float>(c, c->L32[0].u, c->geom[0].ng(), 0.0f);
-        vcycle<float>(c, c->L32, 0, c->bottom32, true);
+        vcycle<float>(c, c->L32, 0, c->bottom32, true, true);
         Lev<double> v64 = view(c, c->L64, 0);
         hipLaunchKernelGGL((k_add_interior<double, float>), cells_grid(c->geom[0]), block(), 0, c->stream, v64, u,
                            (const float *)c->L32[0].u);

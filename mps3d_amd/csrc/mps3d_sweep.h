@@ -250,7 +250,7 @@ using SlotC = std::integral_constant<int, I>;
 // pass of step s-1 (planes s-3..s-1) with no barrier in between, hence five
 // D planes. Global loads run PD steps ahead in two alternating register
 // sets.
-template <typename T, int OP, bool REAC, int TX, int TY, int PD>
+template <typename T, int OP, bool REAC, int TX, int TY, int PD, bool ZG>
 __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__restrict__ uin, T *__restrict__ uout,
                                                       const T *__restrict__ b, int kz)
 {
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
             const int o = pk + (on2[r] ? o2[r] : 0);
-            pu[x][r] = uin[o];
+            pu[x][r] = ZG ? T(0) : uin[o];  // ZG: sweep from a zero guess (uin unread)
             if (!STN) pD[x][r] = L.D[o];
         }
         if (REAC && !STN) {
