@@ -20,6 +20,14 @@
  *
  * Threading: a context owns one HIP stream and its device buffers; it is not
  * re-entrant. Use one host thread per context.
+ *
+ * Ordering of device arrays: every call first orders the context's stream
+ * after the work already queued on the null (default) stream, and returns
+ * with its results complete (the context's stream synchronized), so device
+ * arrays written on the null stream before a call, or read anywhere after
+ * it, need nothing more. Arrays produced on another non-blocking stream must
+ * be ordered before the call by the caller (e.g. hipStreamWaitEvent on
+ * mps_stream(ctx)).
  */
 #ifndef MPS3D_H
 #define MPS3D_H
@@ -164,11 +172,20 @@ int mps_smooth(mps_ctx *ctx, double *soln, const double *mgridsterm, int niter, 
 int mps_fill_ghosts(mps_ctx *ctx, double *soln);
 
 /* performgmres (gmres.f90:197-434) on the finest level: restarted GMRES(m)
- * for A x = b (b interior, x ghosted in/out), left-preconditioned by one
- * V-cycle (precond = 1) or by one red-black sweep from zero as
- * gseidel_precond (precond = 0). Converged when the preconditioned
- * residual <= tol relative to ||M^-1 b||; the in-loop exit of
- * mps1d/source/gmres.f90:343 is used. iters and relres are nullable. */
+ * for A x = b (b interior, x ghosted in/out).
+ *   precond = 0: the reference's method, left-preconditioned by one
+ *     red-black sweep from zero (gseidel_precond), classical Gram-Schmidt,
+ *     converged when the preconditioned residual <= tol relative to
+ *     ||M^-1 b||; one workgroup, at most 512 cells (coarse grids).
+ *   precond = 1: flexible GMRES(m), right-preconditioned by one V-cycle
+ *     (the FP32 V-cycle under MPS_PREC_FP32_DC, Arnoldi and Givens in FP64),
+ *     Gram-Schmidt with one re-orthogonalisation pass; converged when
+ *     ||b - A x|| <= tol ||b|| (the Givens estimate inside a cycle, as the
+ *     in-loop exit of mps1d/source/gmres.f90:343, confirmed by the true
+ *     residual at its end). The m + 1 Krylov and m preconditioned vectors
+ *     are allocated on the first call with a given m and kept by the
+ *     context until mps_destroy.
+ * iters and relres are nullable. */
 int mps_gmres(mps_ctx *ctx, const double *b, double *x, int m, int max_restarts, double tol,
               int precond, int *iters, double *relres);
 

@@ -40,12 +40,23 @@ class Operator:
     faces). Velocities and right-hand sides are supplied per solve."""
 
     def __init__(self, n, length, dcoeff, bc_codes, flux="upwind", reac=None, timederiv=False, dt=1.0,
-                 hierarchy="ref", p=(1, 1, 1), tol=1e-13, max_cycles=80, precision="fp64", cycles=None):
+                 hierarchy="ref", p=(1, 1, 1), tol=1e-13, max_cycles=80, precision="fp64", cycles=None,
+                 method="vcycle", gmres_m=30, gmres_restarts=20):
         """cycles: a fixed number of V-cycles per solve instead of a
         tolerance, which makes the discrete solution map a smooth function
-        of the inputs (needed when finite differences probe it)."""
+        of the inputs (needed when finite differences probe it).
+        method: "vcycle" iterates V-cycles (mps_solve); "gmres" runs
+        FGMRES(gmres_m) preconditioned by one V-cycle (mps_gmres, precond 1)
+        to the same relative tolerance. The adjoint solve iterates V-cycles
+        of the transposed operator either way."""
         self.n = tuple(int(v) for v in n)
+        if method not in ("vcycle", "gmres"):
+            raise ValueError("method must be 'vcycle' or 'gmres'")
+        self.method, self.gmres_m, self.gmres_restarts = method, int(gmres_m), int(gmres_restarts)
         self.fixed = cycles is not None
+        if self.fixed and method == "gmres":
+            raise ValueError("a fixed cycle count needs method='vcycle'")
+        self.tol = float(tol)
         if self.fixed:
             tol, max_cycles = 0.0, int(cycles)
         self.solver = FieldSolver(self.n, length, p=p, hierarchy=hierarchy, precision=precision, flux=flux,
@@ -66,13 +77,20 @@ class Operator:
             vg = tuple(pad(v.detach()) if v is not None else self._zero for v in vel)
         src = self._zero if rhs is None else pad(rhs.detach())
         self._keep = (vg, src)
+        self._rhs = None if rhs is None else rhs.detach().contiguous()
         self.solver.set_coeffs(self.D, *vg, self.reac, src, self.timederiv, self.dt)
 
     def solve(self, x0=None):
         x = torch.zeros_like(self.D) if x0 is None else pad(x0.detach())
-        k, r, _, st = self.solver.solve(x)
+        if self.method == "gmres":
+            b = self._rhs if self._rhs is not None else torch.zeros(self.n[::-1], dtype=torch.float64,
+                                                                    device=x.device)
+            k, r, st = self.solver.gmres(b, x, m=self.gmres_m, restarts=self.gmres_restarts, tol=self.tol)
+        else:
+            k, r, _, st = self.solver.solve(x)
         if st != M.OK and not (self.fixed and st == M.NOT_CONVERGED):
-            raise M.MpsError(st, "forward solve did not converge (%g after %d cycles)" % (r, k))
+            raise M.MpsError(st, "forward solve did not converge (%g after %d %s)" % (
+                r, k, "iterations" if self.method == "gmres" else "cycles"))
         self.cycles.append(k)
         return x
 
@@ -86,8 +104,12 @@ class Operator:
     def grad_velocity(self, lam, x):
         out = [torch.empty(self.n[::-1], dtype=torch.float64, device=lam.device) for _ in range(3)]
         lib = self.solver.lib
-        M.check(lib.mps_grad_velocity(self.solver._h, C.c_void_p(lam.data_ptr()), C.c_void_p(x.data_ptr()),
-                                      *[C.c_void_p(o.data_ptr()) for o in out]))
+        self.solver.order_torch_in()
+        try:
+            M.check(lib.mps_grad_velocity(self.solver._h, C.c_void_p(lam.data_ptr()), C.c_void_p(x.data_ptr()),
+                                          *[C.c_void_p(o.data_ptr()) for o in out]))
+        finally:
+            self.solver.order_torch_out()
         return out
 
     def close(self):

@@ -129,6 +129,16 @@ struct LevelBuf {
     int op = -1;  // operator form override (>= 0), e.g. transposed stencils
 };
 
+// FGMRES(m) work vectors of the finest level, kept by the context between
+// calls (allocated on the first call with a given m or more)
+struct GmresBufs {
+    int m = 0;
+    std::vector<double *> V, Z;
+    double *w = nullptr, *bg = nullptr, *dh = nullptr, *dpart = nullptr;
+    double *hh = nullptr;  // pinned host mirror of dh
+    int nblocks = 0;
+};
+
 }  // namespace
 
 struct mps_ctx {
@@ -151,6 +161,7 @@ struct mps_ctx {
     double *partials = nullptr;
     int max_partials = 0;
     double *d_scalar = nullptr;
+    double *d_dcscale = nullptr;         // FP32 defect scale and its inverse
     double *h_scalar = nullptr;          // pinned
     int *d_status = nullptr;
     double *bottom64 = nullptr;
@@ -159,6 +170,7 @@ struct mps_ctx {
     double *user_tmp = nullptr;          // staging for host arrays (ghosted)
     double *user_tmp2 = nullptr;         // staging for interior arrays
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_in = nullptr;          // orders calls after the null stream
     double last_ms = 0.0;
     int last_cycles = 0;
     double *dinv = nullptr;              // dense inverse of the coarsest operator
@@ -174,6 +186,7 @@ struct mps_ctx {
     unsigned gen = 0, graph_gen = ~0u;   // gen: bumped by set_bc / set_coeffs
     bool mg_ok = true;                   // a multigrid hierarchy exists
     std::string mg_error;                // why not, when it does not
+    GmresBufs gm;                        // FGMRES work vectors (mps_gmres)
 
     int nlev() const { return (int)geom.size(); }
 };
@@ -577,7 +590,7 @@ int residual_max_partials(const Geom &g)
 // convection in fast arithmetic take the plane-marching kernel.
 template <typename T, typename R_>
 int launch_residual(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *u, const T *b, R_ *r, double *partials,
-                    int op, bool reduce_grid = false)
+                    int op, bool reduce_grid = false, const double *rscale = nullptr)
 {
     Lev<T> v = view(c, L, l);
     const Geom &g = c->geom[l];
@@ -587,7 +600,7 @@ int launch_residual(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *u, 
         auto go = [&](auto O, auto Q) {
             hipLaunchKernelGGL(
                 (k_residual_planes<T, R_, decltype(O)::value, decltype(Q)::value, kResTX, kResTY, kResRB>),
-                               gr, dim3(kResTX * kResTY), 0, c->stream, v, u, b, r, partials, kz);
+                               gr, dim3(kResTX * kResTY), 0, c->stream, v, u, b, r, partials, kz, rscale);
         };
         if (op == OP_REF) {
             if (c->has_reac) go(IntC<OP_REF>{}, BoolC<true>{});
@@ -601,7 +614,7 @@ int launch_residual(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *u, 
     const dim3 gr = (reduce_grid || partials) ? red_grid(g) : cells_grid(g);
     with_op(op, c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
         hipLaunchKernelGGL((k_residual<T, R_, decltype(O)::value, decltype(V)::value, decltype(R)::value>), gr,
-                           block(), 0, c->stream, v, u, b, r, partials);
+                           block(), 0, c->stream, v, u, b, r, partials, rscale);
     });
     return partials ? (int)(gr.x * gr.y) : 0;
 }
@@ -977,6 +990,17 @@ struct ExactScope {
 };
 
 // --------------------------------------------------------------- helpers
+// Orders the context's stream after the work already queued on the null
+// stream (the default stream of frameworks such as torch), so that device
+// arrays produced there are complete before this call reads them.
+int after_null_stream(mps_ctx *c)
+{
+    hipError_t e = hipEventRecord(c->ev_in, nullptr);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->ev_in, 0);
+    if (e != hipSuccess) return fail(MPS_ERR_HIP, "ordering after the null stream: %s", hipGetErrorString(e));
+    return MPS_OK;
+}
+
 int guard(mps_ctx *c, bool need_coeffs = true)
 {
     if (!c) return fail(MPS_ERR_ARG, "null context");
@@ -984,7 +1008,7 @@ int guard(mps_ctx *c, bool need_coeffs = true)
         return fail(MPS_ERR_STATE, "set boundary conditions and coefficients first");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return fail(MPS_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
-    return MPS_OK;
+    return after_null_stream(c);
 }
 
 int guard_mg(mps_ctx *c)
@@ -1050,21 +1074,24 @@ void fp64_cycle(mps_ctx *c, double *u) { fp64_cycle(c, u, c->b0, false); }
 
 }  // namespace
 
-// FP64 residual of the current iterate, rounded to FP32 for the inner cycle
+// FP64 residual of the current iterate, scaled by rscale[0] (a power of two)
+// and rounded to FP32 for the inner cycle
 namespace mps {
 template <int OP, bool VEL, bool REAC>
 __global__ void __launch_bounds__(kBlock) k_defect32(Lev<double> L, const double *__restrict__ u,
-                                                    const double *__restrict__ b, float *d32, double *partials)
+                                                    const double *__restrict__ b, float *d32, double *partials,
+                                                    const double *rscale)
 {
     int i, j;
     double sq = 0.0;
+    const double rs = rscale[0];
     if (cell_ij(L, i, j)) {
         for (int k = blockIdx.y + 1; k <= L.nz; k += gridDim.y) {
             const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
             double dg, of;
             row<double, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
             double rv = b[c] - (dg * u[c] + of);
-            d32[c] = (float)rv;
+            d32[c] = (float)(rv * rs);
             sq += rv * rv;
         }
     }
@@ -1083,11 +1110,13 @@ void defect_step(mps_ctx *c, double *u, bool do_cycle, const double *b = nullptr
     if (!b) b = c->b0;
     int np;
     if (v.fast && level_op(c, 0) != OP_STENCIL && !c->has_vel) {
-        np = launch_residual<double, float>(c, c->L64, 0, u, b, F[0].b, c->partials, level_op(c, 0));
+        np = launch_residual<double, float>(c, c->L64, 0, u, b, F[0].b, c->partials, level_op(c, 0), false,
+                                            c->d_dcscale);
     } else {
         with_op(level_op(c, 0), c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
             hipLaunchKernelGGL((k_defect32<decltype(O)::value, decltype(V)::value, decltype(R)::value>),
-                               red_grid(g), block(), 0, c->stream, v, (const double *)u, b, F[0].b, c->partials);
+                               red_grid(g), block(), 0, c->stream, v, (const double *)u, b, F[0].b, c->partials,
+                               (const double *)c->d_dcscale);
         });
         dim3 gr = red_grid(g);
         np = (int)(gr.x * gr.y);
@@ -1098,7 +1127,7 @@ void defect_step(mps_ctx *c, double *u, bool do_cycle, const double *b = nullptr
     vcycle<float>(c, F, 0, c->bottom32, true, true);
     Lev<double> v64 = view(c, c->L64, 0);
     hipLaunchKernelGGL((k_add_interior<double, float>), cells_grid(g), block(), 0, c->stream, v64, u,
-                       (const float *)F[0].u);
+                       (const float *)F[0].u, (const double *)c->d_dcscale);
 }
 
 double measure(const mps_ctx *c, double sumsq, double bnorm2)
@@ -1130,7 +1159,7 @@ void outer_step_body(mps_ctx *c, double *u, double *b, bool homog, bool dc)
         vcycle<float>(c, c->L32, 0, c->bottom32, true, true);
         Lev<double> v64 = view(c, c->L64, 0);
         hipLaunchKernelGGL((k_add_interior<double, float>), cells_grid(c->geom[0]), block(), 0, c->stream, v64, u,
-                           (const float *)c->L32[0].u);
+                           (const float *)c->L32[0].u, (const double *)c->d_dcscale);
         defect_step(c, u, false, b);
     } else {
         fp64_cycle(c, u, b, homog);
@@ -1179,11 +1208,18 @@ void outer_step(mps_ctx *c, double *u, double *b, bool homog, bool dc)
 int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, double &meas)
 {
     double bn2 = 0.0;
-    if (c->opts.tol_kind == MPS_TOL_REL_L2) {
-        b_sumsq(c, b);
-        bn2 = read_scalar(c);
-    }
     const bool dc = c->opts.precision == MPS_PREC_FP32_DC;
+    if (c->opts.tol_kind == MPS_TOL_REL_L2 || dc) {
+        b_sumsq(c, b);
+        // the FP32 defects of this solve are scaled by the binade of b's rms
+        // (they stay between about 1 and the tolerance times that), so right-
+        // hand sides far outside FP32's range, e.g. adjoint ones, still
+        // correct in FP32
+        if (dc)
+            hipLaunchKernelGGL(k_dc_scale, dim3(1), dim3(64), 0, c->stream, (const double *)c->d_scalar,
+                               (double)c->geom[0].n(), c->d_dcscale);
+        if (c->opts.tol_kind == MPS_TOL_REL_L2) bn2 = read_scalar(c);
+    }
     int it = 0;
     meas = 0.0;
     if (dc) {
@@ -1305,6 +1341,8 @@ void build_bottom_inverse(mps_ctx *c)
     if (!c->L32.empty()) c->L32[lb].inv = c->dinv;
 }
 
+void free_gmres(GmresBufs &G);
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1358,6 +1396,7 @@ int mps_destroy(mps_ctx *c)
     dfree(c->b0);
     dfree(c->partials);
     dfree(c->d_scalar);
+    dfree(c->d_dcscale);
     if (c->h_scalar) (void)hipHostFree(c->h_scalar);
     dfree(c->d_status);
     dfree(c->bottom64);
@@ -1368,8 +1407,10 @@ int mps_destroy(mps_ctx *c)
     dfree(c->dinv);
     dfree(c->dinvT);
     dfree(c->gbuf);
+    free_gmres(c->gm);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev_in) (void)hipEventDestroy(c->ev_in);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MPS_OK;
@@ -1409,6 +1450,7 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
         }
         HIPCHK(hipEventCreate(&c->ev0));
         HIPCHK(hipEventCreate(&c->ev1));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
         alloc_levels<double>(c, c->L64, true);
         if (c->opts.precision == MPS_PREC_FP32_DC) alloc_levels<float>(c, c->L32, true);
         const Geom &g0 = c->geom[0];
@@ -1419,6 +1461,8 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
         c->max_partials = residual_max_partials(g0);
         c->partials = dalloc<double>(c->max_partials);
         c->d_scalar = dalloc<double>(4);
+        c->d_dcscale = dalloc<double>(2);
+        fill<double>(c, c->d_dcscale, 2, 1.0);
         HIPCHK(hipHostMalloc((void **)&c->h_scalar, 4 * sizeof(double), hipHostMallocDefault));
         c->d_status = dalloc<int>(1);
         fill<int>(c, c->d_status, 1, 0);
@@ -1545,6 +1589,7 @@ int mps_set_coeffs(mps_ctx *c, const double *dcoeff, const double *velx, const d
     if (timederiv && !(dt != 0.0)) return fail(MPS_ERR_ARG, "timederiv with dt = 0");
     try {
         HIPCHK(hipSetDevice(c->device));
+        if (int st = after_null_stream(c)) return st;
         auto &L = c->L64;
         const long long ng = c->geom[0].ng();
         c->has_vel = velx || vely || velz;
@@ -1796,14 +1841,6 @@ int mps_last_timing(mps_ctx *c, double *total_ms, int *ncycles)
 // ======================================================= Krylov and adjoint
 namespace {
 
-struct GmresBufs {
-    int m = 0;
-    std::vector<double *> V, Z;
-    double *w = nullptr, *bg = nullptr, *dh = nullptr, *dpart = nullptr;
-    double *hh = nullptr;  // pinned host mirror of dh
-    int nblocks = 0;
-};
-
 // lazily built transposed hierarchy (explicit A^T stencils on every level)
 void ensure_transpose(mps_ctx *c)
 {
@@ -1911,6 +1948,45 @@ double multiaxpy(mps_ctx *c, GmresBufs &G, double *w, double *const *vs, int nv,
     return read_scalar(c);
 }
 
+void free_gmres(GmresBufs &G)
+{
+    for (auto p : G.V) dfree(p);
+    for (auto p : G.Z) dfree(p);
+    dfree(G.w);
+    dfree(G.bg);
+    dfree(G.dh);
+    dfree(G.dpart);
+    if (G.hh) (void)hipHostFree(G.hh);
+    G = GmresBufs{};
+}
+
+// the context's FGMRES(m) vectors, (re)allocated when m grows. Every vector
+// starts with zero ghosts and the Krylov kernels write interiors only, so
+// the ghosts stay zero from call to call.
+GmresBufs &gmres_bufs(mps_ctx *c, int m)
+{
+    GmresBufs &G = c->gm;
+    if (G.m >= m) return G;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    free_gmres(G);
+    const Geom &g = c->geom[0];
+    const dim3 gr = red_grid(g);
+    G.nblocks = (int)(gr.x * gr.y);
+    for (int q = 0; q <= m; ++q) G.V.push_back(dalloc<double>(g.ng()));
+    for (int q = 0; q < m; ++q) G.Z.push_back(dalloc<double>(g.ng()));
+    G.w = dalloc<double>(g.ng());
+    G.bg = dalloc<double>(g.ng());
+    G.dh = dalloc<double>(kMaxVec);
+    G.dpart = dalloc<double>((size_t)G.nblocks * kMaxVec);
+    HIPCHK(hipHostMalloc((void **)&G.hh, kMaxVec * sizeof(double), hipHostMallocDefault));
+    for (auto p : G.V) fill<double>(c, p, g.ng(), 0.0);
+    for (auto p : G.Z) fill<double>(c, p, g.ng(), 0.0);
+    fill<double>(c, G.w, g.ng(), 0.0);
+    fill<double>(c, G.bg, g.ng(), 0.0);
+    G.m = m;
+    return G;
+}
+
 void scale_into(mps_ctx *c, GmresBufs &G, const double *w, double *v, double s)
 {
     G.hh[0] = s;
@@ -1964,30 +2040,7 @@ int mps_gmres(mps_ctx *c, const double *b, double *x, int m, int max_restarts, d
         // ---------------- FGMRES(m), right-preconditioned by one V-cycle
         if (m + 1 > kMaxVec) return fail(MPS_ERR_ARG, "m at most %d", kMaxVec - 1);
         if (!c->mg_ok) return fail(MPS_ERR_ARG, "no multigrid hierarchy for this grid: %s", c->mg_error.c_str());
-        GmresBufs G;
-        G.m = m;
-        dim3 gr = red_grid(g);
-        G.nblocks = (int)(gr.x * gr.y);
-        for (int q = 0; q <= m; ++q) G.V.push_back(dalloc<double>(g.ng()));
-        for (int q = 0; q < m; ++q) G.Z.push_back(dalloc<double>(g.ng()));
-        G.w = dalloc<double>(g.ng());
-        G.bg = dalloc<double>(g.ng());
-        G.dh = dalloc<double>(kMaxVec);
-        G.dpart = dalloc<double>((size_t)G.nblocks * kMaxVec);
-        HIPCHK(hipHostMalloc((void **)&G.hh, kMaxVec * sizeof(double), hipHostMallocDefault));
-        auto cleanup = [&]() {
-            for (auto p : G.V) dfree(p);
-            for (auto p : G.Z) dfree(p);
-            dfree(G.w);
-            dfree(G.bg);
-            dfree(G.dh);
-            dfree(G.dpart);
-            if (G.hh) (void)hipHostFree(G.hh);
-        };
-        for (auto p : G.V) fill<double>(c, p, g.ng(), 0.0);
-        for (auto p : G.Z) fill<double>(c, p, g.ng(), 0.0);
-        fill<double>(c, G.w, g.ng(), 0.0);
-        fill<double>(c, G.bg, g.ng(), 0.0);
+        GmresBufs &G = gmres_bufs(c, m);
         const double *bd = stage_interior(c, b);
         Lev<double> v0 = view(c, c->L64, 0);
         hipLaunchKernelGGL(k_interior_in<double>, cells_grid(g), block(), 0, c->stream, v0, bd, G.bg);
@@ -2078,7 +2131,6 @@ int mps_gmres(mps_ctx *c, const double *b, double *x, int m, int max_restarts, d
         if (relres) *relres = bnorm > 0 ? beta / bnorm : beta;
         unbind_solution(c, x, staged);
         HIPCHK(hipStreamSynchronize(c->stream));
-        cleanup();
         if (status == MPS_ERR_NAN) return fail(MPS_ERR_NAN, "NaN or Inf in GMRES");
         if (status == MPS_NOT_CONVERGED)
             fail(MPS_NOT_CONVERGED, "GMRES: relative residual %g above %g", bnorm > 0 ? beta / bnorm : beta, tol);

@@ -390,20 +390,30 @@ __global__ void __launch_bounds__(kBlock) k_colour(Lev<T> L, T *__restrict__ u, 
 }
 
 // r = b - (diag*u + off); writes r (type R, ghosted layout, nullable) and the
-// block's partial sum of r^2 (nullable)
+// block's partial sum of r^2 (nullable). When R is narrower than T, the
+// stored r is scaled by rscale[0] (a power of two, nullable = 1) so that
+// residuals of any magnitude land in R's range; the sum is of r itself.
 template <typename T, typename R, int OP, bool VEL, bool REAC>
 __global__ void __launch_bounds__(kBlock) k_residual(Lev<T> L, const T *__restrict__ u, const T *__restrict__ b,
-                                                    R *r, double *partials)
+                                                    R *r, double *partials, const double *rscale = nullptr)
 {
     int i, j;
     double sq = 0.0;
+    T rs = T(1);
+    if constexpr (sizeof(R) < sizeof(T)) {
+        if (rscale) rs = (T)rscale[0];
+    }
     if (cell_ij(L, i, j)) {
         for (int k = blockIdx.y + 1; k <= L.nz; k += gridDim.y) {
             const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
             T dg, of;
             row<T, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
             T rv = b[c] - (dg * u[c] + of);
-            if (r) r[c] = (R)rv;
+            if constexpr (sizeof(R) < sizeof(T)) {
+                if (r) r[c] = (R)(rv * rs);
+            } else {
+                if (r) r[c] = (R)rv;
+            }
             sq += (double)rv * (double)rv;
         }
     }
@@ -708,14 +718,33 @@ __global__ void __launch_bounds__(kBlock) k_convert(const Ti *in, To *out, long 
     for (; t < n; t += stride) out[t] = (To)in[t];
 }
 
-// interior correction u += e (defect correction: FP64 u, FP32 e)
+// interior correction u += e * escale[1] (defect correction: FP64 u, FP32 e
+// computed for a right-hand side scaled by escale[0] = 1 / escale[1], both
+// powers of two; escale nullable = no scaling)
 template <typename T, typename E>
-__global__ void __launch_bounds__(kBlock) k_add_interior(Lev<T> L, T *u, const E *e)
+__global__ void __launch_bounds__(kBlock) k_add_interior(Lev<T> L, T *u, const E *e, const double *escale)
 {
     int i, j, k;
     long long c;
     if (!cell_of(L, i, j, k, c)) return;
-    u[c] = u[c] + (T)e[c];
+    const T s = escale ? (T)escale[1] : T(1);
+    u[c] = u[c] + (T)e[c] * s;
+}
+
+// the defect-correction scale pair (2^-e, 2^e) with 2^e the binade of the rms
+// sqrt(sumsq / n); (1, 1) for a zero or non-finite sum
+__global__ void k_dc_scale(const double *sumsq, double n, double *scale)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double rms = sqrt(sumsq[0] / n);
+    if (rms > 0.0 && isfinite(rms)) {
+        const int e = ilogb(rms);
+        scale[0] = ldexp(1.0, -e);
+        scale[1] = ldexp(1.0, e);
+    } else {
+        scale[0] = 1.0;
+        scale[1] = 1.0;
+    }
 }
 
 // interior copy between a caller's interior array (nx,ny,nz) and the

@@ -630,11 +630,12 @@ namespace mps {
 // costs one barrier, and RB cells per lane put RB times the loads in flight
 // per barrier. Values outside the interior are staged as zeros and their
 // face factors are zero. R is the stored type of r (float for the FP32
-// defect of the FP64 iterate); r is nullable.
+// defect of the FP64 iterate, stored scaled by rscale[0], a power of two;
+// rscale nullable = 1); r is nullable.
 template <typename T, typename R, int OP, bool REAC, int TX, int TY, int RB>
 __global__ void __launch_bounds__(TX *TY) k_residual_planes(Lev<T> L, const T *__restrict__ u,
                                                            const T *__restrict__ b, R *__restrict__ r,
-                                                           double *partials, int kz)
+                                                           double *partials, int kz, const double *rscale)
 {
 #pragma clang fp contract(fast)
     constexpr int HY = TY * RB;
@@ -653,6 +654,10 @@ __global__ void __launch_bounds__(TX *TY) k_residual_planes(Lev<T> L, const T *_
     const int nx = L.nx, ny = L.ny, nz = L.nz, sy = L.sy;
     const long long sz = L.sz;
     const int i = i0 + tx;
+    T rs = T(1);
+    if constexpr (sizeof(R) < sizeof(T)) {
+        if (rscale) rs = (T)rscale[0];
+    }
 
     int oc[RB], q[RB];
     bool in[RB];
@@ -782,7 +787,11 @@ __global__ void __launch_bounds__(TX *TY) k_residual_planes(Lev<T> L, const T *_
                 nb = nb + wzm * um[m];
                 nb = nb + wzp * up[m];
                 const T rv = bc[m] - (d * uc[m] - nb);
-                if (r) r[sz * k + oc[m]] = (R)rv;
+                if constexpr (sizeof(R) < sizeof(T)) {
+                    if (r) r[sz * k + oc[m]] = (R)(rv * rs);
+                } else {
+                    if (r) r[sz * k + oc[m]] = (R)rv;
+                }
                 sq += (double)rv * (double)rv;
             }
         }

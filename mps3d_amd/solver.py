@@ -96,6 +96,8 @@ class FieldSolver:
         M.check(self.lib.mps_create(C.byref(h), C.byref(g), C.byref(o)))
         self._h = h
         self._keep = []
+        self._ext = None        # the context's stream as a torch ExternalStream
+        self._torch_call = False
 
     # ------------------------------------------------------------ plumbing
     @property
@@ -119,6 +121,26 @@ class FieldSolver:
     def __exit__(self, *a):
         self.close()
 
+    def order_torch_in(self):
+        """Orders the context's stream after torch's current stream (device-
+        side, no host wait): tensors torch is still writing are complete
+        before the library reads them. The library itself only orders after
+        the null stream."""
+        import torch
+        if self._ext is None:
+            self._ext = torch.cuda.ExternalStream(self.lib.mps_stream(self._h))
+        self._ext.wait_stream(torch.cuda.current_stream())
+        self._torch_call = True
+
+    def order_torch_out(self):
+        """Orders torch's current stream after the context's stream (the
+        library returns with its stream synchronized; this keeps torch work
+        queued later correct on any stream)."""
+        import torch
+        if self._torch_call and self._ext is not None:
+            torch.cuda.current_stream().wait_stream(self._ext)
+        self._torch_call = False
+
     def _ptr(self, a, kind, writable=False):
         """Pointer to a caller array. kind: 'g' ghosted, 'i' interior, 'f' face."""
         if a is None:
@@ -130,6 +152,8 @@ class FieldSolver:
             want = {"g": int(np.prod(self.shape)), "i": int(np.prod(self.n))}.get(kind)
             if want is not None and a.numel() != want:
                 raise ValueError("device array has %d elements, expected %d" % (a.numel(), want))
+            if not self._torch_call:
+                self.order_torch_in()
             return C.c_void_p(a.data_ptr())
         arr = np.asarray(a, dtype=np.float64)
         if kind == "g":
@@ -150,6 +174,8 @@ class FieldSolver:
 
     def _release(self):
         self._keep = []
+        if self._torch_call:
+            self.order_torch_out()
 
     # ------------------------------------------------------------ set-up
     def set_bc(self, codes, faces=(0.0,) * 6):
@@ -269,7 +295,8 @@ class FieldSolver:
 
     def gmres(self, b, x, m=30, restarts=20, tol=1e-10, precond=1, raise_on_failure=False):
         """precond 0: the reference's performgmres (left, one RB sweep, small grids);
-        precond 1: FGMRES(m) right-preconditioned by one V-cycle. Returns (iters, relres, status)."""
+        precond 1: FGMRES(m) right-preconditioned by one V-cycle (the FP32 cycle under
+        precision "fp32dc"). Returns (iters, relres, status)."""
         it = C.c_int()
         rr = C.c_double()
         try:

@@ -15,6 +15,8 @@ from mps3d_amd import workloads as W
 from mps3d_amd.autograd import Operator, linear_solve
 from mps3d_amd.coupled import GlowModel, central_difference, seeded_source, source_pulse
 
+from dense_sg import dense_sg
+
 pytestmark = pytest.mark.gpu
 
 
@@ -54,67 +56,6 @@ def test_velocity_sensitivity_matches_finite_differences(gpu, flux):
     eps = 1e-6
     fd = (pairing(eps) - pairing(-eps)) / (2 * eps)
     assert abs(adj - fd) <= 1e-6 * abs(fd)
-
-
-def bernoulli_t(x):
-    small = x.abs() < 1e-3
-    xs = torch.where(small, torch.ones_like(x), x)
-    big = xs / torch.expm1(xs)
-    ser = 1.0 - 0.5 * x + x * x * (1.0 / 12.0 - x * x / 720.0)
-    return torch.where(small, ser, big)
-
-
-def dense_sg(n, h, D, vel, codes, dtinv, mean="harmonic"):
-    """Dense FP64 A(v) of the 3D SG operator (interior (nz,ny,nx) tensors,
-    row-major cell index i + nx*(j + ny*k)), built with torch ops so that
-    autograd differentiates it."""
-    nx, ny, nz = n
-    N = nx * ny * nz
-    A = torch.zeros((N, N), dtype=torch.float64, device=D.device)
-    idx = torch.arange(N, device=D.device).reshape(nz, ny, nx)
-    diag = torch.zeros((nz, ny, nx), dtype=torch.float64, device=D.device) + dtinv
-    for axis, v, lo, hi in ((2, vel[0], codes[0], codes[1]), (1, vel[1], codes[2], codes[3]),
-                            (0, vel[2], codes[4], codes[5])):
-        sl = [slice(None)] * 3
-        sr = [slice(None)] * 3
-        sl[axis] = slice(0, -1)
-        sr[axis] = slice(1, None)
-        sl, sr = tuple(sl), tuple(sr)
-        cL, cR, DL, DR = v[sl], v[sr], D[sl], D[sr]
-        ch = 0.5 * (cL + cR)
-        Dh = 2.0 * DL * DR / (DL + DR + 1e-10) if mean == "harmonic" else 0.5 * (DL + DR)
-        Pe = ch * h / Dh
-        g = Dh / h
-        wL = g * bernoulli_t(-Pe)
-        wR = -g * bernoulli_t(Pe)
-        iL, iR = idx[sl].reshape(-1), idx[sr].reshape(-1)
-        # row L: + (wL uL + wR uR)/h ; row R: - (...)/h
-        A = A.index_put((iL, iR), (wR / h).reshape(-1), accumulate=True)
-        A = A.index_put((iR, iL), (-wL / h).reshape(-1), accumulate=True)
-        dL = torch.zeros_like(diag)
-        dL[sl] = dL[sl] + wL / h
-        dL[sr] = dL[sr] - wR / h
-        diag = diag + dL
-        # boundary rows: DIRC = 2D/h^2 + upwind outflow, ZGRD = outflow
-        first = [slice(None)] * 3
-        last = [slice(None)] * 3
-        first[axis] = slice(0, 1)
-        last[axis] = slice(-1, None)
-        first, last = tuple(first), tuple(last)
-        bd = torch.zeros_like(diag)
-        vb = v[first]
-        if lo == "DIRC":
-            bd[first] = 2 * D[first] / h ** 2 + torch.where(vb < 0, -vb / h, torch.zeros_like(vb))
-        elif lo == "ZGRD":
-            bd[first] = -vb / h
-        vb = v[last]
-        if hi == "DIRC":
-            bd[last] = bd[last] + 2 * D[last] / h ** 2 + torch.where(vb > 0, vb / h, torch.zeros_like(vb))
-        elif hi == "ZGRD":
-            bd[last] = bd[last] + vb / h
-        diag = diag + bd
-    A = A + torch.diag(diag.reshape(-1))
-    return A
 
 
 def test_linear_solve_gradients_match_dense_fp64_autograd(gpu):

@@ -1,0 +1,207 @@
+"""The multi-species discharge driver (mps3d_amd.discharge) on the
+field-solve path.
+
+* 16^3 / 17^3 copies of the glow-discharge and five-species workloads
+  against the same runs with every linear solve replaced by a dense FP64
+  torch solve of the same operator (tests/dense_sg.py): all fields, and the
+  gradient of the field energy with respect to both electrode voltages
+  (hand-written adjoint with checkpointing vs torch autograd through the
+  dense solves).
+* The glow-discharge workload at full size: 128^3 over a 2 cm gap, electrons,
+  one positive ion, electron energy and Poisson, 100 implicit
+  Scharfetter-Gummel steps at dt = 1e-11 s, each species solve FGMRES(30)
+  with a V-cycle preconditioner in FP64 to 1e-10.
+* The five-species workload at full size: 257^3 over a 4 cm cube, 20 steps
+  with FP32 smoothing under FP64 refinement and GMRES(30), the error of
+  every field against the plain FP64 run, and the adjoint gradient of the
+  field energy with respect to both electrode voltages against central
+  differences (relative step 1e-6).
+"""
+import time
+
+import pytest
+import torch
+
+from mps3d_amd import discharge as DS
+
+from dense_sg import dense_sg
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+
+def make_dense(model):
+    """Replaces the model's solves by dense FP64 torch solves of the same
+    operators, so torch autograd differentiates the whole run."""
+    shp = model.shape
+    z = torch.zeros(shp, dtype=torch.float64, device="cuda")
+    one = torch.ones(shp, dtype=torch.float64, device="cuda")
+    pcodes = ("ZGRD",) * 4 + ("DIRC", "DIRC")
+    Ap = dense_sg(model.n, model.h, model.eps_r, (z, z, z), pcodes, 0.0, mean="arith")
+
+    def solve(A, rhs):
+        return torch.linalg.solve(A, rhs.reshape(-1)).reshape(shp)
+
+    def species(k, rhs, vel, x0):
+        s = model.species[k]
+        codes = (s.wall,) * 4 + (s.electrode,) * 2
+        v = vel if vel is not None else (z, z, z)
+        A = dense_sg(model.n, model.h, s.D * one, v, codes, 1.0 / model.dt,
+                     mean="harmonic" if vel is not None else "arith")
+        return solve(A, rhs)
+
+    def energy(rhs, vel, x0):
+        e = model.species[model.names.index("e")]
+        codes = (e.wall,) * 4 + (e.electrode,) * 2
+        A = dense_sg(model.n, model.h, (5.0 / 3.0) * e.D * one, vel, codes, 1.0 / model.dt)
+        return solve(A, rhs)
+
+    model.solve_poisson = lambda rhs: solve(Ap, rhs)
+    model.solve_species = species
+    model.solve_energy = energy
+    return model
+
+
+def fields(state):
+    dens, neps = state
+    return list(dens) + ([neps] if neps is not None else [])
+
+
+def test_glow_discharge_16cubed_matches_dense_solves(gpu):
+    ours, s0 = DS.glow_discharge(n=16, tol=1e-13)
+    dense, _ = DS.glow_discharge(n=16, tol=1e-13)
+    make_dense(dense)
+    V = torch.tensor([1000.0], dtype=torch.float64, device="cuda")
+    with torch.no_grad():
+        a = ours.run(V, 10, s0)
+        b = dense.run(V, 10, s0)
+    for name, x, y in zip(ours.names + ["energy"], fields(a), fields(b)):
+        assert torch.isfinite(x).all() and x.min() > 0, name
+        assert rel(x, y) <= 1e-9, (name, rel(x, y))
+    ours.close()
+    dense.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32dc"])
+def test_five_species_17cubed_matches_dense_solves_and_autograd(gpu, precision):
+    """17^3 (aligned hierarchy): fields after 6 steps, then dW/dV for both
+    electrode voltages, against dense FP64 solves differentiated by torch."""
+    ours, s0, _ = DS.five_species(n=17, precision=precision, tol=1e-13)
+    dense, _, _ = DS.five_species(n=17, precision="fp64", tol=1e-13)
+    make_dense(dense)
+    V0 = torch.tensor([800.0, -800.0], dtype=torch.float64, device="cuda")
+    with torch.no_grad():
+        a = ours.run(V0, 6, s0)
+        b = dense.run(V0, 6, s0)
+    for name, x, y in zip(ours.names, fields(a), fields(b)):
+        assert rel(x, y) <= 1e-9, (name, rel(x, y))
+
+    def grad(model, ckpt):
+        V = V0.clone().requires_grad_(True)
+        st = model.run(V, 6, s0, checkpoint_every=ckpt)
+        W = model.field_energy(st, V)
+        W.backward()
+        return float(W.detach()), V.grad.detach().clone()
+
+    Wo, go = grad(ours, 2)
+    Wd, gd = grad(dense, 0)
+    assert abs(Wo - Wd) <= 1e-10 * abs(Wd)
+    assert rel(go, gd) <= 1e-8, (go.tolist(), gd.tolist())
+    ours.close()
+    dense.close()
+
+
+@pytest.mark.timeout(900)
+def test_glow_discharge_128cubed_100_steps(gpu):
+    """The glow-discharge workload at full size, FP64, GMRES(30) with a
+    V-cycle preconditioner to 1e-10 for every species solve: every solve
+    converges (a failure raises), densities and the electron energy stay
+    positive and finite, and the mean energy stays within its bounds."""
+    model, s0 = DS.glow_discharge(n=128, tol=1e-10)
+    V = torch.tensor([1000.0], dtype=torch.float64, device="cuda")
+    with torch.no_grad():
+        t = time.perf_counter()
+        st = model.run(V, 100, s0, timed=True)
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t
+    dens, neps = st
+    for name, x in zip(model.names + ["energy"], fields(st)):
+        assert torch.isfinite(x).all() and x.min() > 0, name
+    eps = neps / dens[0]
+    counts = model.solve_counts()
+    its = {k: (min(v), max(v), sum(v) / len(v)) for k, v in counts.items()}
+    st_ms = sorted(model.step_times)
+    print("128^3 glow discharge: 100 steps in %.2f s (median step %.1f ms); mean energy %.2f..%.2f eV; "
+          "solve counts (min, max, mean) %s; e-/ion totals %.4e %.4e" % (
+              total, 1e3 * st_ms[len(st_ms) // 2], float(eps.min()), float(eps.max()), its,
+              float(dens[0].sum()), float(dens[1].sum())))
+    assert float(eps.min()) >= 0.05 and float(eps.max()) <= 500.0
+    assert max(max(v) for k, v in counts.items() if k != "poisson") <= 30 * 20
+    model.close()
+
+
+@pytest.mark.timeout(1200)
+def test_five_species_257cubed_fp32_refinement_and_adjoint(gpu):
+    """The five-species workload at full size: 20 steps with FP32 smoothing
+    under FP64 refinement vs the plain FP64 run, field by field; then the
+    gradient of the field energy with respect to both electrode voltages,
+    hand-written adjoint with checkpoints every 5 steps vs central
+    differences (relative step 1e-6) of FP64 runs solved to 1e-13."""
+    V0 = torch.tensor([800.0, -800.0], dtype=torch.float64, device="cuda")
+    runs = {}
+    for prec in ("fp64", "fp32dc"):
+        model, s0, prm = DS.five_species(n=257, precision=prec, tol=1e-10)
+        with torch.no_grad():
+            model.run(V0, 1, s0)  # warm
+            torch.cuda.synchronize()
+            model.step_times.clear()
+            t = time.perf_counter()
+            st = model.run(V0, 20, s0, timed=True)
+            phi = model.potential_of(st, V0)
+            torch.cuda.synchronize()
+            runs[prec] = (time.perf_counter() - t, fields(st) + [phi], model.solve_counts())
+        model.close()
+        del model, st, phi
+        torch.cuda.empty_cache()
+    names = ["e", "A+", "A-", "A*1", "A*2", "phi"]
+    errs = {n: rel(a, b) for n, a, b in zip(names, runs["fp32dc"][1], runs["fp64"][1])}
+    print("257^3 five species, 20 steps: FP64 %.2f s, FP32+DC %.2f s; per-field rel L2 (FP32+DC vs FP64): %s" % (
+        runs["fp64"][0], runs["fp32dc"][0], {k: "%.2e" % v for k, v in errs.items()}))
+    for prec in runs:
+        print(prec, {k: (min(v), max(v)) for k, v in runs[prec][2].items()})
+    for n_, e in errs.items():
+        assert e <= 1e-8, (n_, e)
+    del runs
+    torch.cuda.empty_cache()
+
+    model, s0, _ = DS.five_species(n=257, precision="fp64", tol=1e-13)
+    t = time.perf_counter()
+    with torch.no_grad():
+        W0 = float(model.field_energy(model.run(V0, 20, s0), V0))
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t
+    t = time.perf_counter()
+    V = V0.clone().requires_grad_(True)
+    W = model.field_energy(model.run(V, 20, s0, checkpoint_every=5), V)
+    W.backward()
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t
+    g = V.grad.detach().cpu().tolist()
+    fd = []
+    for i in range(2):
+        d = 1e-6 * abs(float(V0[i]))
+        e = torch.zeros(2, dtype=torch.float64, device="cuda")
+        e[i] = d
+        with torch.no_grad():
+            wp = float(model.field_energy(model.run(V0 + e, 20, s0), V0 + e))
+            wm = float(model.field_energy(model.run(V0 - e, 20, s0), V0 - e))
+        fd.append((wp - wm) / (2 * d))
+    print("field energy %.6e J; dW/dV adjoint %s, central differences %s; forward %.2f s, forward+adjoint %.2f s"
+          % (W0, g, fd, t_fwd, t_adj))
+    for a, b in zip(g, fd):
+        assert abs(a - b) <= 1e-6 * abs(b), (a, b)
+    assert abs(float(W.detach()) - W0) <= 1e-12 * abs(W0)
+    model.close()

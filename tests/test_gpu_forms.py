@@ -94,6 +94,30 @@ def test_fp32_defect_correction_matches_fp64(gpu, hier, n):
     assert abs(sol["fp32dc"][1] - sol["fp64"][1]) <= 2
 
 
+@pytest.mark.parametrize("scale", [1e-46, 1e-30, 1e30, 1e45])
+@pytest.mark.parametrize("tol_kind", ["rel", "rms"])
+def test_fp32_defect_correction_outside_fp32_range(gpu, scale, tol_kind):
+    """Right-hand sides far below or above FP32's range (adjoint sources of
+    tiny objectives, say): the FP32 defects are scaled by a power of two per
+    solve, so the FP32+DC solve of s*b takes the cycles of the solve of b and
+    returns s times its solution (to the 1e-10 tolerance)."""
+    c = W.varcoef_poisson(33)
+    rms_b = float(np.sqrt(np.mean(c.source ** 2)))
+    out = {}
+    for s_ in (1.0, scale):
+        tol = 1e-10 if tol_kind == "rel" else 1e-10 * s_ * rms_b
+        with FieldSolver(c.n, c.length, hierarchy="aligned", precision="fp32dc", max_cycles=40, tol=tol,
+                         tol_kind=tol_kind) as s:
+            s.set_bc(c.bc_codes, [0.0] * 6)
+            s.set_coeffs(c.dcoeff, None, None, None, None, s_ * c.source)
+            u = np.zeros(s.shape, order="F")
+            k, r, h, st = s.solve(u, raise_on_failure=True)
+            out[s_] = (u[1:-1, 1:-1, 1:-1].copy(), k)
+    a, b = out[scale][0] / scale, out[1.0][0]
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-9
+    assert abs(out[scale][1] - out[1.0][1]) <= 1
+
+
 @pytest.mark.parametrize("flux", ["upwind", "sg"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32dc"])
 def test_fgmres_with_vcycle_preconditioner(gpu, flux, prec):
