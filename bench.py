@@ -36,7 +36,32 @@ PEAK_HBM_GBPS = 8000.0
 BYTES_PER_CELL_CYCLE = {"fp64": 302.9, "fp32dc": 199.4}
 
 
-def gpu_solves(n, steps, warmup, length=0.02, tol=1e-10):
+def plain_start_ms(case, hier, prec, steps, tol):
+    """ms per solve with V-cycles from the zero guess throughout (no
+    full-multigrid first pass), for comparison."""
+    import torch
+
+    from mps3d_amd import FieldSolver, to_device
+    s = FieldSolver(case.n, case.length, hierarchy=hier, precision=prec, max_cycles=80, tol=tol,
+                    tol_kind="rel", arith="fast", start="guess")
+    s.set_case(case)
+    u = to_device(np.zeros(s.shape))
+    u.zero_()
+    s.solve(u)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        u.zero_()
+        k, r, _, st = s.solve(u, raise_on_failure=True)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    s.close()
+    del u
+    torch.cuda.empty_cache()
+    return {"ms_per_solve": round(ms, 3), "cycles": int(k)}
+
+
+def gpu_solves(n, steps, warmup, length=0.02, tol=1e-10, start="fmg"):
     import torch
 
     from mps3d_amd import FieldSolver, from_device, to_device
@@ -47,8 +72,9 @@ def gpu_solves(n, steps, warmup, length=0.02, tol=1e-10):
     out = {}
     sols = {}
     for prec in ("fp32dc", "fp64"):
+        st_ = start if hier == "aligned" else "guess"
         s = FieldSolver(case.n, case.length, hierarchy=hier, precision=prec, max_cycles=80, tol=tol,
-                        tol_kind="rel", arith="fast")
+                        tol_kind="rel", arith="fast", start=st_)
         s.set_case(case)
         u = to_device(np.zeros(s.shape))
         for _ in range(warmup):
@@ -99,6 +125,8 @@ def gpu_solves(n, steps, warmup, length=0.02, tol=1e-10):
         s.close()
         del u
         torch.cuda.empty_cache()
+        if st_ != "guess":
+            ent["plain_start"] = plain_start_ms(case, hier, prec, steps, tol)
     err = float(np.linalg.norm(sols["fp32dc"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
     return case, hier, out, err
 
@@ -222,11 +250,13 @@ def main():
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "fp32+fp64", "data": "synthetic",
         "config": {"workload": "varcoef_poisson_%d" % args.n, "n": args.n, "domain_m": 0.02,
-                   "hierarchy": hier, "smoother": "RB-GS V(2,2)", "tol_rel_l2": 1e-10, "seed": 20261202,
-                   "arith": "fast"},
+                   "hierarchy": hier, "smoother": "RB-GS V(2,2)",
+                   "start": "full multigrid" if hier == "aligned" else "zero guess", "tol_rel_l2": 1e-10,
+                   "seed": 20261202, "arith": "fast"},
         "fp64_ms": round(f64["ms_per_solve"], 3),
         "speedup_fp32dc_vs_fp64": round(f64["ms_per_solve"] / dc["ms_per_solve"], 3),
         "cycles": {"fp32dc": dc["cycles"], "fp64": f64["cycles"]},
+        "plain_start": {"fp32dc": dc.get("plain_start"), "fp64": f64.get("plain_start")},
         "relres": {"fp32dc": dc["relres"], "fp64": f64["relres"]},
         "fp32dc_vs_fp64_rel_l2": err,
         "unknowns_per_s": {"fp32dc": dc["unknowns_per_s"], "fp64": f64["unknowns_per_s"]},

@@ -90,6 +90,17 @@ enum {
     MPS_TOL_RMS = 0,      /* RMS of the absolute residual (globalvars.f90:136-159) */
     MPS_TOL_REL_L2 = 1    /* ||b - A u||_2 / ||b||_2                              */
 };
+enum {
+    MPS_START_GUESS = 0,  /* every iteration of mps_solve is one V-cycle (or
+                             defect-correction step) from the current iterate,
+                             as the reference's caller loops                */
+    MPS_START_FMG = 1     /* MPS_HIER_ALIGNED only: the first iteration is a
+                             full-multigrid pass on the initial defect (its
+                             right-hand side restricted to every level,
+                             coarse-to-fine nested iteration with trilinear
+                             interpolation and one V-cycle per level), the
+                             rest are V-cycles. No reference counterpart.  */
+};
 
 typedef struct {
     int nx, ny, nz;             /* global cells (g_Nx, g_Ny, g_Nz)             */
@@ -109,12 +120,14 @@ typedef struct {
     double tol;                 /* stopping tolerance for mps_solve            */
     int tol_kind;               /* MPS_TOL_*                                   */
     int arith;                  /* MPS_ARITH_* for the solver's own rows       */
+    int start;                  /* MPS_START_*                                 */
 } mps_opts;
 
 typedef struct mps_ctx mps_ctx;
 
 /* Fills reference defaults: REF hierarchy, FP64, upwind, nu1=nu2=2,
- * max_cycles=20, tol=1e-11 RMS (globalvars.f90:29), exact arithmetic. */
+ * max_cycles=20, tol=1e-11 RMS (globalvars.f90:29), exact arithmetic,
+ * V-cycles from the caller's guess. */
 void mps_default_opts(mps_opts *opts);
 
 /* Allocates a context for one grid on the current HIP device. Replaces the

@@ -129,6 +129,15 @@ struct LevelBuf {
     int op = -1;  // operator form override (>= 0), e.g. transposed stencils
 };
 
+// one outer solve step captured as a HIP graph, with what it was captured
+// for (rebuilt when the pointers, the kind of step or the problem change)
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    const void *u = nullptr, *b = nullptr;
+    int kind = -1;
+    unsigned gen = ~0u;
+};
+
 // FGMRES(m) work vectors of the finest level, kept by the context between
 // calls (allocated on the first call with a given m or more)
 struct GmresBufs {
@@ -176,14 +185,13 @@ struct mps_ctx {
     double *dinv = nullptr;              // dense inverse of the coarsest operator
     double *dinvT = nullptr;             // and of its transpose
     double *gbuf = nullptr;              // ghosted right-hand side of adjoint solves
+    double *fmg_r = nullptr, *fmg_e = nullptr;  // FP64 defect and correction of a full-multigrid start
     bool exact_now = false;              // reference-mirroring call in progress
-    // one outer solve step captured as a HIP graph and replayed every cycle
-    // (see outer_step); rebuilt when its pointers, kind or the problem change
+    // outer solve steps captured as HIP graphs and replayed (see
+    // outer_step): [0] the V-cycle step, [1] the full-multigrid first step
     bool graphs = true;                  // MPS3D_NO_GRAPHS=1 turns replay off
-    hipGraphExec_t step_graph = nullptr;
-    const void *graph_u = nullptr, *graph_b = nullptr;
-    int graph_kind = -1;
-    unsigned gen = 0, graph_gen = ~0u;   // gen: bumped by set_bc / set_coeffs
+    StepGraph step[2];
+    unsigned gen = 0;                    // bumped by set_bc / set_coeffs
     bool mg_ok = true;                   // a multigrid hierarchy exists
     std::string mg_error;                // why not, when it does not
     GmresBufs gm;                        // FGMRES work vectors (mps_gmres)
@@ -553,7 +561,7 @@ dim3 residual_planes_grid(const Geom &g, int &kz)
 constexpr int kRestCX = 32, kRestCY = 8, kProlTX = MPS_PROL_TX, kProlTY = MPS_PROL_TY;
 
 template <typename T>
-void launch_restrict_fw(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l)
+void launch_restrict_fw(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *src = nullptr)
 {
     const Geom &gc = c->geom[l + 1];
     Lev<T> vf = view(c, L, l), vc = view(c, L, l + 1);
@@ -562,7 +570,7 @@ void launch_restrict_fw(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l)
     dim3 gr((unsigned)((gc.nx + kRestCX - 1) / kRestCX), (unsigned)((gc.ny + kRestCY - 1) / kRestCY),
             (unsigned)((gc.nz + kz - 1) / kz));
     hipLaunchKernelGGL((k_restrict_fw_planes<T, kRestCX, kRestCY>), gr, dim3(kRestCX * kRestCY), 0, c->stream, vf,
-                       vc, W, (const T *)L[l].r, L[l + 1].b, kz);
+                       vc, W, src ? src : (const T *)L[l].r, L[l + 1].b, kz);
 }
 
 template <typename T>
@@ -849,6 +857,29 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
         launch_prolong_tri<T>(c, L, l);
     }
     smooth_level<T>(c, L, l, c->opts.nu2, op);
+}
+
+// Full multigrid for A e = L[0].b on the aligned hierarchy (homogeneous
+// rows, correction form): the right-hand side is restricted to every level
+// down to the first one the V-cycle solves as a whole (the LDS-resident
+// levels, else the coarsest), that level is solved by one V-cycle from zero,
+// and on the way up each level starts from the trilinear interpolation of
+// the coarser solution and runs one V-cycle. e ends in L[0].u. A level's
+// right-hand side is still intact when its turn comes: the V-cycles of the
+// coarser levels overwrite only the right-hand sides below them.
+template <typename T>
+void fmg(mps_ctx *c, std::vector<LevelBuf<T>> &L, T *bwork)
+{
+    const int nl = c->nlev();
+    int top = lds_cycle_from<T>(c, L);
+    if (top < 0) top = nl - 1;
+    for (int l = 0; l < top; ++l) launch_restrict_fw<T>(c, L, l, L[l].b);
+    vcycle<T>(c, L, top, bwork, true, true);
+    for (int l = top - 1; l >= 0; --l) {
+        fill<T>(c, L[l].u, c->geom[l].ng(), T(0));
+        launch_prolong_tri<T>(c, L, l);
+        vcycle<T>(c, L, l, bwork, true, false);
+    }
 }
 
 // ---------------------------------------------------------- coefficients
@@ -1166,40 +1197,89 @@ void outer_step_body(mps_ctx *c, double *u, double *b, bool homog, bool dc)
     }
 }
 
-void outer_step(mps_ctx *c, double *u, double *b, bool homog, bool dc)
+// The first iteration of a solve with MPS_START_FMG: a full-multigrid pass
+// on the current defect instead of one V-cycle, then, as outer_step_body,
+// the FP64 residual of the corrected iterate into d_scalar. The FP64 form
+// needs fmg_buffers first.
+void fmg_step_body(mps_ctx *c, double *u, double *b, bool dc)
 {
-    if (!c->graphs) {
-        outer_step_body(c, u, b, homog, dc);
+    const Geom &g = c->geom[0];
+    Lev<double> v64 = view(c, c->L64, 0);
+    if (dc) {
+        // the defect (scaled FP32) is in L32[0].b from the previous defect_step
+        fmg<float>(c, c->L32, c->bottom32);
+        hipLaunchKernelGGL((k_add_interior<double, float>), cells_grid(g), block(), 0, c->stream, v64, u,
+                           (const float *)c->L32[0].u, (const double *)c->d_dcscale);
+        defect_step(c, u, false, b);
         return;
     }
+    auto &L = c->L64;
+    launch_residual<double, double>(c, L, 0, u, b, c->fmg_r, nullptr, level_op(c, 0));
+    double *ub = L[0].u, *bb = L[0].b;
+    L[0].u = c->fmg_e;
+    L[0].b = c->fmg_r;
+    fmg<double>(c, L, c->bottom64);
+    L[0].u = ub;
+    L[0].b = bb;
+    hipLaunchKernelGGL((k_add_interior<double, double>), cells_grid(g), block(), 0, c->stream, v64, u,
+                       (const double *)c->fmg_e, (const double *)nullptr);
+    residual_sumsq<double>(c, L, u, b, level_op(c, 0));
+}
+
+// the FP64 defect and correction buffers of a full-multigrid start
+void fmg_buffers(mps_ctx *c)
+{
+    const Geom &g = c->geom[0];
+    for (double **p : {&c->fmg_r, &c->fmg_e}) {
+        if (!*p) {
+            *p = dalloc<double>(g.ng());
+            fill<double>(c, *p, g.ng(), 0.0);  // ghost layers stay zero: interiors only are written
+        }
+    }
+}
+
+// One outer step (fmg_first: the full-multigrid first step), replayed from
+// a HIP graph captured on first use.
+void outer_step(mps_ctx *c, double *u, double *b, bool homog, bool dc, bool fmg_first = false)
+{
+    if (fmg_first && !dc) fmg_buffers(c);
+    auto body = [&]() {
+        if (fmg_first) fmg_step_body(c, u, b, dc);
+        else outer_step_body(c, u, b, homog, dc);
+    };
+    if (!c->graphs) {
+        body();
+        return;
+    }
+    StepGraph &G = c->step[fmg_first ? 1 : 0];
     const int kind = (dc ? 2 : 0) + (homog ? 1 : 0) + (c->exact_now ? 4 : 0);
-    if (!c->step_graph || c->graph_u != u || c->graph_b != b || c->graph_kind != kind || c->graph_gen != c->gen) {
-        if (c->step_graph) {
-            HIPCHK(hipGraphExecDestroy(c->step_graph));
-            c->step_graph = nullptr;
+    if (!G.exec || G.u != u || G.b != b || G.kind != kind || G.gen != c->gen) {
+        if (G.exec) {
+            HIPCHK(hipGraphExecDestroy(G.exec));
+            G.exec = nullptr;
         }
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         try {
-            outer_step_body(c, u, b, homog, dc);
+            body();
         } catch (...) {
             (void)hipStreamEndCapture(c->stream, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
         HIPCHK(hipStreamEndCapture(c->stream, &g));
-        const hipError_t e = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
+        const hipError_t e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) {
-            c->step_graph = nullptr;
+            G.exec = nullptr;
             throw HipError{e, "hipGraphInstantiate"};
         }
-        c->graph_u = u;
-        c->graph_b = b;
-        c->graph_kind = kind;
-        c->graph_gen = c->gen;
+        G.u = u;
+        G.b = b;
+        G.kind = kind;
+        G.gen = c->gen;
     }
-    HIPCHK(hipGraphLaunch(c->step_graph, c->stream));
+    HIPCHK(hipGraphLaunch(G.exec, c->stream));
 }
 
 // The solve loop shared by the forward and the self-adjoint solves: V-cycles
@@ -1230,7 +1310,7 @@ int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, do
             for (it = 0; it < c->opts.max_cycles;) {
                 // the FP32 cycle on the defect, the FP64 correction, then the
                 // FP64 residual of the corrected iterate (the next defect)
-                outer_step(c, u, b, homog, true);
+                outer_step(c, u, b, homog, true, it == 0 && c->opts.start == MPS_START_FMG);
                 meas = measure(c, read_scalar(c), bn2);
                 if (history) history[it] = meas;
                 ++it;
@@ -1240,7 +1320,7 @@ int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, do
         }
     } else {
         for (it = 0; it < c->opts.max_cycles;) {
-            outer_step(c, u, b, homog, false);
+            outer_step(c, u, b, homog, false, it == 0 && c->opts.start == MPS_START_FMG && !c->geom[0].bottom);
             meas = c->geom[0].bottom ? 0.0 : measure(c, read_scalar(c), bn2);
             if (history) history[it] = meas;
             ++it;
@@ -1360,6 +1440,7 @@ void mps_default_opts(mps_opts *o)
     o->tol = 1e-11;
     o->tol_kind = MPS_TOL_RMS;
     o->arith = MPS_ARITH_EXACT;
+    o->start = MPS_START_GUESS;
 }
 
 const char *mps_error_string(int s)
@@ -1385,7 +1466,8 @@ int mps_destroy(mps_ctx *c)
     if (!c) return MPS_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
+    for (auto &G : c->step)
+        if (G.exec) (void)hipGraphExecDestroy(G.exec);
     if (!c->L64.empty()) c->L64[0].b = nullptr;  // b0 is freed below
     if (!c->LT.empty()) c->LT[0].u = nullptr;
     c->L64.size() ? free_levels(c->L64) : void();
@@ -1407,6 +1489,8 @@ int mps_destroy(mps_ctx *c)
     dfree(c->dinv);
     dfree(c->dinvT);
     dfree(c->gbuf);
+    dfree(c->fmg_r);
+    dfree(c->fmg_e);
     free_gmres(c->gm);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1435,6 +1519,11 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
     if (c->opts.nu1 < 0 || c->opts.nu2 < 0 || c->opts.max_cycles < 1) {
         delete c;
         return fail(MPS_ERR_ARG, "bad options");
+    }
+    if (c->opts.start != MPS_START_GUESS &&
+        (c->opts.start != MPS_START_FMG || c->opts.hierarchy != MPS_HIER_ALIGNED)) {
+        delete c;
+        return fail(MPS_ERR_ARG, "start must be MPS_START_GUESS, or MPS_START_FMG with MPS_HIER_ALIGNED");
     }
     int st = build_geometry(c);
     if (st != MPS_OK) {

@@ -21,6 +21,7 @@ module mps3d_gpu
     integer(c_int), parameter :: MPS_FLUX_UPWIND = 0, MPS_FLUX_SG = 1
     integer(c_int), parameter :: MPS_TOL_RMS = 0, MPS_TOL_REL_L2 = 1
     integer(c_int), parameter :: MPS_ARITH_EXACT = 0, MPS_ARITH_FAST = 1
+    integer(c_int), parameter :: MPS_START_GUESS = 0, MPS_START_FMG = 1
 
     type, bind(C) :: mps_grid
         integer(c_int) :: nx, ny, nz
@@ -31,7 +32,7 @@ module mps3d_gpu
     type, bind(C) :: mps_opts
         integer(c_int) :: hierarchy, precision, flux, nu1, nu2, max_cycles
         real(c_double) :: tol
-        integer(c_int) :: tol_kind, arith
+        integer(c_int) :: tol_kind, arith, start
     end type mps_opts
 
     interface

@@ -66,7 +66,8 @@ class FieldSolver:
     """One grid on one GPU (an ``mps_ctx``)."""
 
     def __init__(self, n, length=(1.0, 1.0, 1.0), p=(1, 1, 1), hierarchy="ref", precision="fp64",
-                 flux="upwind", nu1=2, nu2=2, max_cycles=20, tol=1e-11, tol_kind="rms", arith="exact"):
+                 flux="upwind", nu1=2, nu2=2, max_cycles=20, tol=1e-11, tol_kind="rms", arith="exact",
+                 start="guess"):
         self.lib = M.load()
         self.n = tuple(int(v) for v in (n if not np.isscalar(n) else (n, n, n)))
         self.length = tuple(float(v) for v in (length if not np.isscalar(length) else (length,) * 3))
@@ -81,6 +82,7 @@ class FieldSolver:
         o.tol = float(tol)
         o.tol_kind = _TOL[tol_kind] if isinstance(tol_kind, str) else int(tol_kind)
         o.arith = {"exact": M.ARITH_EXACT, "fast": M.ARITH_FAST}[arith] if isinstance(arith, str) else int(arith)
+        o.start = {"guess": 0, "fmg": 1}[start] if isinstance(start, str) else int(start)
         self.opts = o
         self.max_cycles = int(max_cycles)
         h = C.c_void_p()

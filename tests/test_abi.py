@@ -36,6 +36,7 @@ def test_default_options_are_the_reference_constants():
     # nsmoothsteps = 2 (mgridsteps.f90:473), 20 V-cycles, ERRTOLR = 1e-11
     assert (o.nu1, o.nu2, o.max_cycles, o.tol) == (2, 2, 20, 1e-11)
     assert (o.hierarchy, o.precision, o.flux, o.tol_kind) == (M.HIER_REF, M.PREC_FP64, M.FLUX_UPWIND, M.TOL_RMS)
+    assert (o.arith, o.start) == (0, 0)  # exact rows, V-cycles from the caller's guess
 
 
 def test_error_strings():

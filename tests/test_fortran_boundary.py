@@ -18,10 +18,14 @@ DRIVER = os.path.join(ROOT, "tests", "fortran", "_build", "laplace_gpu")
 
 
 def _built():
-    if not os.path.exists(DRIVER):
-        if shutil.which("amdflang") is None and not os.path.exists("/opt/rocm/bin/amdflang"):
-            pytest.skip("no Fortran compiler and no prebuilt driver")
+    """The driver, brought up to date with its sources when the compiler is
+    here (make rebuilds nothing when it is current: a driver built against
+    an older mps_opts layout would corrupt its stack)."""
+    have_fc = shutil.which("amdflang") is not None or os.path.exists("/opt/rocm/bin/amdflang")
+    if have_fc:
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "mps3d_amd", "fortran")], check=True)
+    elif not os.path.exists(DRIVER):
+        pytest.skip("no Fortran compiler and no prebuilt driver")
     return DRIVER
 
 

@@ -118,6 +118,53 @@ def test_fp32_defect_correction_outside_fp32_range(gpu, scale, tol_kind):
     assert abs(out[scale][1] - out[1.0][1]) <= 1
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32dc"])
+@pytest.mark.parametrize("n", [33, 65])
+def test_fmg_start(gpu, prec, n):
+    """MPS_START_FMG: the first iteration is a full-multigrid pass on the
+    initial defect. It reaches the same tolerance in fewer iterations, to the
+    solution of a tight plain solve; repeated solves (graph replays) give the
+    same bits; a warm start from a perturbed solution converges too."""
+    c = W.varcoef_poisson(n)
+    with FieldSolver(c.n, c.length, hierarchy="aligned", max_cycles=80, tol=1e-13, tol_kind="rel") as s:
+        s.set_case(c)
+        ref = np.zeros(s.shape, order="F")
+        s.solve(ref, raise_on_failure=True)
+    out = {}
+    for start in ("guess", "fmg"):
+        with FieldSolver(c.n, c.length, hierarchy="aligned", precision=prec, max_cycles=60, tol=1e-10,
+                         tol_kind="rel", arith="fast", start=start) as s:
+            s.set_case(c)
+            runs = []
+            for _ in range(3):
+                u = np.zeros(s.shape, order="F")
+                k, r, h, st = s.solve(u, raise_on_failure=True)
+                runs.append((u, k, h))
+            assert all(np.array_equal(runs[0][0], x[0]) for x in runs[1:])
+            u, k, h = runs[0]
+            ax, _, _ = s.find_AX(u)
+            b = s.find_bvec(None, True)
+            assert np.linalg.norm(b - ax) / np.linalg.norm(b) <= 1e-10
+            out[start] = (u, k, h)
+            # warm start: the solution plus a rough perturbation
+            w = u.copy(order="F")
+            w[1:-1, 1:-1, 1:-1] *= 1.0 + 1e-3 * np.random.default_rng(5).standard_normal(c.n)
+            kw, rw, _, _ = s.solve(w, raise_on_failure=True)
+            assert rw <= 1e-10
+    ug, kg, hg = out["guess"]
+    uf, kf, hf = out["fmg"]
+    assert kf <= kg - 1
+    assert hf[0] < hg[1]  # the full-multigrid pass beats two V-cycles from zero
+    for u in (ug, uf):
+        assert np.linalg.norm(u[1:-1, 1:-1, 1:-1] - ref[1:-1, 1:-1, 1:-1]) / np.linalg.norm(ref[1:-1, 1:-1, 1:-1]) < 1e-8
+
+
+def test_fmg_start_needs_the_aligned_hierarchy(gpu):
+    from mps3d_amd import _lib as M
+    with pytest.raises(M.MpsError):
+        FieldSolver((16, 16, 16), hierarchy="ref", start="fmg")
+
+
 @pytest.mark.parametrize("flux", ["upwind", "sg"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32dc"])
 def test_fgmres_with_vcycle_preconditioner(gpu, flux, prec):
