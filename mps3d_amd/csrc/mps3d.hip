@@ -867,6 +867,25 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
 // the coarser solution and runs one V-cycle. e ends in L[0].u. A level's
 // right-hand side is still intact when its turn comes: the V-cycles of the
 // coarser levels overwrite only the right-hand sides below them.
+// The coarse levels' cycles of the full-multigrid pass smooth once before
+// and once after (their errors are corrected again by the finest level's
+// cycle): measured at 257^3, the pass reaches the same defect (4.7e-5
+// relative) as with the context's two sweeps, 0.16 ms sooner.
+#ifndef MPS_FMG_NU1C
+#define MPS_FMG_NU1C 1   // pre-/post-smoothing of the coarse levels' cycles (-1: the context's)
+#endif
+#ifndef MPS_FMG_NU2C
+#define MPS_FMG_NU2C 1
+#endif
+#ifndef MPS_FMG_NU10
+#define MPS_FMG_NU10 -1  // and of the finest level's cycle
+#endif
+#ifndef MPS_FMG_NU20
+#define MPS_FMG_NU20 -1
+#endif
+#ifndef MPS_FMG_CYCC
+#define MPS_FMG_CYCC 1   // V-cycles per coarse level
+#endif
 template <typename T>
 void fmg(mps_ctx *c, std::vector<LevelBuf<T>> &L, T *bwork)
 {
@@ -875,10 +894,16 @@ void fmg(mps_ctx *c, std::vector<LevelBuf<T>> &L, T *bwork)
     if (top < 0) top = nl - 1;
     for (int l = 0; l < top; ++l) launch_restrict_fw<T>(c, L, l, L[l].b);
     vcycle<T>(c, L, top, bwork, true, true);
+    const int nu1 = c->opts.nu1, nu2 = c->opts.nu2;
     for (int l = top - 1; l >= 0; --l) {
         fill<T>(c, L[l].u, c->geom[l].ng(), T(0));
         launch_prolong_tri<T>(c, L, l);
-        vcycle<T>(c, L, l, bwork, true, false);
+        const int a = l ? MPS_FMG_NU1C : MPS_FMG_NU10, b = l ? MPS_FMG_NU2C : MPS_FMG_NU20;
+        c->opts.nu1 = a >= 0 ? a : nu1;
+        c->opts.nu2 = b >= 0 ? b : nu2;
+        for (int q = 0; q < (l ? MPS_FMG_CYCC : 1); ++q) vcycle<T>(c, L, l, bwork, true, false);
+        c->opts.nu1 = nu1;
+        c->opts.nu2 = nu2;
     }
 }
 

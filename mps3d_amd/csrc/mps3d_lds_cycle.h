@@ -17,8 +17,11 @@
 
 namespace mps {
 
-constexpr int kLdsMaxLev = 4;      // levels one launch can hold
-constexpr int kLdsBlock = 1024;    // lanes of the workgroup
+#ifndef MPS_LDS_BLOCK
+#define MPS_LDS_BLOCK 1024
+#endif
+constexpr int kLdsMaxLev = 4;              // levels one launch can hold
+constexpr int kLdsBlock = MPS_LDS_BLOCK;   // lanes of the workgroup
 
 template <typename T>
 struct LdsLevels {
@@ -169,22 +172,30 @@ __global__ void __launch_bounds__(kLdsBlock) k_lds_cycle(LdsLevels<T> S)
         for (int t = tid; t < nc; t += nt) {
             int I, J, K;
             lds_ijk(c, t, I, J, K);
+            // the cell's nine weights first, all loads in flight together
+            // (a zero weight marks a fine neighbour outside the interior)
+            double wx[3], wy[3], wz[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                wx[d] = W.wx[3 * (I - 1) + d];
+                wy[d] = W.wy[3 * (J - 1) + d];
+                wz[d] = W.wz[3 * (K - 1) + d];
+            }
             double acc = 0.0;
-            for (int dk = -1; dk <= 1; ++dk) {
-                const double wk = W.wz[3 * (K - 1) + dk + 1];
-                if (wk == 0.0) continue;
-                const int fk = 2 * K - 1 + dk;
-                for (int dj = -1; dj <= 1; ++dj) {
-                    const double wjk = wk * W.wy[3 * (J - 1) + dj + 1];
-                    if (wjk == 0.0) continue;
-                    const int fj = 2 * J - 1 + dj;
+#pragma unroll
+            for (int dk = 0; dk < 3; ++dk) {
+                const int fk = 2 * K - 2 + dk;
+#pragma unroll
+                for (int dj = 0; dj < 3; ++dj) {
+                    const int fj = 2 * J - 2 + dj;
                     double row_ = 0.0;
-                    for (int di = -1; di <= 1; ++di) {
-                        const double wi = W.wx[3 * (I - 1) + di + 1];
-                        const int fi = 2 * I - 1 + di;
-                        if (wi != 0.0) row_ += wi * (double)v.r[(fi - 1) + v.nx * ((fj - 1) + v.ny * (fk - 1))];
+#pragma unroll
+                    for (int di = 0; di < 3; ++di) {
+                        const int fi = 2 * I - 2 + di;
+                        if (wx[di] != 0.0 && wy[dj] != 0.0 && wz[dk] != 0.0)
+                            row_ += wx[di] * (double)v.r[(fi - 1) + v.nx * ((fj - 1) + v.ny * (fk - 1))];
                     }
-                    acc += wjk * row_;
+                    acc += (wz[dk] * wy[dj]) * row_;
                 }
             }
             c.b[t] = (T)acc;
