@@ -573,6 +573,61 @@ void launch_restrict_fw(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T 
                        vc, W, src ? src : (const T *)L[l].r, L[l + 1].b, kz);
 }
 
+// Residual and restriction in one pass (k_resrestrict_planes): aligned
+// hierarchy, rows without convection, fields addressed with 32-bit plane
+// offsets. MPS_FUSED_RESTRICT selects where: bit 1 FP32 face-factor rows,
+// bit 2 FP64 face-factor rows, bit 4 explicit rows.
+#ifndef MPS_FUSED_RESTRICT
+#define MPS_FUSED_RESTRICT 1
+#endif
+#ifndef MPS_RR_CY
+#define MPS_RR_CY 8
+#endif
+#ifndef MPS_RR_TARGET
+#define MPS_RR_TARGET 2048
+#endif
+#ifndef MPS_RR_KZ
+#define MPS_RR_KZ 16
+#endif
+#ifndef MPS_RR_M
+#define MPS_RR_M 1
+#endif
+constexpr int kRRCX = 32, kRRCY = MPS_RR_CY, kRRM = MPS_RR_M;
+
+template <typename T>
+bool resrestrict_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op)
+{
+    if (c->opts.hierarchy != MPS_HIER_ALIGNED || c->has_vel) return false;
+    if (c->geom[l].ng() >= (1LL << 31)) return false;
+    if (op == OP_STENCIL) return (MPS_FUSED_RESTRICT & 4) != 0;
+    return view(c, L, l).fast && (MPS_FUSED_RESTRICT & (sizeof(T) == 4 ? 1 : 2)) != 0;
+}
+
+template <typename T>
+void launch_resrestrict(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op)
+{
+    const Geom &gc = c->geom[l + 1];
+    Lev<T> vf = view(c, L, l), vc = view(c, L, l + 1);
+    AlignedW W{L[l + 1].w[0], L[l + 1].w[1], L[l + 1].w[2]};
+    const int kz = z_chunk(gc, kRRCX, kRRCY, MPS_RR_TARGET, MPS_RR_KZ);
+    dim3 gr((unsigned)((gc.nx + kRRCX - 1) / kRRCX), (unsigned)((gc.ny + kRRCY - 1) / kRRCY),
+            (unsigned)((gc.nz + kz - 1) / kz));
+    auto go = [&](auto O, auto R) {
+        hipLaunchKernelGGL((k_resrestrict_planes<T, decltype(O)::value, decltype(R)::value, kRRCX, kRRCY, kRRM>),
+                           gr, dim3(kRRCX * kRRCY * kRRM), 0, c->stream, vf, vc, W, (const T *)L[l].u,
+                           (const T *)L[l].b, L[l + 1].b, kz);
+    };
+    if (op == OP_STENCIL) {
+        go(IntC<OP_STENCIL>{}, BoolC<false>{});
+    } else if (op == OP_REF) {
+        if (c->has_reac) go(IntC<OP_REF>{}, BoolC<true>{});
+        else go(IntC<OP_REF>{}, BoolC<false>{});
+    } else {
+        if (c->has_reac) go(IntC<OP_ALIGNED>{}, BoolC<true>{});
+        else go(IntC<OP_ALIGNED>{}, BoolC<false>{});
+    }
+}
+
 template <typename T>
 void launch_prolong_tri(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l)
 {
@@ -835,7 +890,8 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
         if (!zg) fill<T>(c, L[l].u, g.ng(), T(0));
     }
     smooth_level<T>(c, L, l, c->opts.nu1, op, zg);
-    launch_residual<T, T>(c, L, l, L[l].u, L[l].b, L[l].r, nullptr, op);
+    const bool fused = resrestrict_ok<T>(c, L, l, op);
+    if (!fused) launch_residual<T, T>(c, L, l, L[l].u, L[l].b, L[l].r, nullptr, op);
     const Geom &gc = c->geom[l + 1];
     Lev<T> vf = view(c, L, l), vc = view(c, L, l + 1);
     const int opc = level_op(c, L, l + 1);
@@ -852,7 +908,8 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
         hipLaunchKernelGGL(k_prolong1_correct<T>, cells_grid(gc), block(), 0, c->stream, vf, vc, L[l].u,
                            (const T *)L[l + 1].u, (const T *)L[l + 1].uold, G);
     } else {
-        launch_restrict_fw<T>(c, L, l);
+        if (fused) launch_resrestrict<T>(c, L, l, op);
+        else launch_restrict_fw<T>(c, L, l);
         vcycle<T>(c, L, l + 1, bwork, homog, true);
         launch_prolong_tri<T>(c, L, l);
     }

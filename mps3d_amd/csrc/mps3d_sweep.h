@@ -898,6 +898,242 @@ __global__ void __launch_bounds__(CX *CY) k_restrict_fw_planes(Lev<T> F, Lev<T> 
     }
 }
 
+// Residual and full-weighting restriction in one pass (aligned hierarchy,
+// rows without convection): the coarse right-hand side cf = R (b - A u) with
+// the fine residual formed in LDS and never stored. A block owns a CX x CY
+// tile of coarse cells and marches coarse planes K; step K forms the fine
+// residual planes 2K-1 and 2K on the tile's (2CX+1) x (2CY+1) fine region
+// from four u (and D) planes with a one-cell halo kept in a ring, then the
+// x/y sums of the weights as k_restrict_fw_planes, carrying plane 2K's sum
+// to step K+1. Rows are the fast face-factor rows of k_residual_planes, or
+// explicit 7-point rows (OP_STENCIL, coefficients read per cell), with the
+// same arithmetic. Fine residuals outside the interior are zero (their
+// weights are zero). Global loads of step K+1 are issued before step K's
+// residuals are formed. M lanes per coarse cell share the staging and the
+// residuals (the first CX*CY lanes also form the coarse values).
+template <typename T, int OP, bool REAC, int CX, int CY, int M>
+__global__ void __launch_bounds__(CX *CY *M) k_resrestrict_planes(Lev<T> F, Lev<T> C, AlignedW W,
+                                                                 const T *__restrict__ u, const T *__restrict__ b,
+                                                                 T *__restrict__ cf, int kz)
+{
+#pragma clang fp contract(fast)
+    constexpr bool STN = OP == OP_STENCIL;
+    constexpr int FW = 2 * CX + 1, FH = 2 * CY + 1, FP = FW * FH;
+    constexpr int UW = FW + 2, UH = FH + 2, UP = UW * UH;
+    constexpr int NT = CX * CY * M;
+    constexpr int NU = (UP + NT - 1) / NT;
+    constexpr int NF = (FP + NT - 1) / NT;
+    __shared__ T su[4][UP];
+    __shared__ T sD[STN ? 1 : 4][STN ? 1 : UP];
+    __shared__ T sr[2][FP];
+
+    const int tid = threadIdx.x, tx = tid % CX, ty = tid / CX;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int I0 = bx * CX + 1, J0 = by * CY + 1;
+    const int Kb = bz * kz + 1, Ke = min(Kb + kz - 1, C.nz);
+    const int I = I0 + tx, J = J0 + ty;
+    const bool in = tid < CX * CY && I <= C.nx && J <= C.ny;
+    const int nx = F.nx, ny = F.ny, nz = F.nz, sy = F.sy;
+    const int szi = (int)F.sz;
+
+    // staging offsets of the u/D region (fine cells 2I0-3 .., 2J0-3 ..;
+    // -1 outside the interior: staged as zeros)
+    int ou[NU];
+#pragma unroll
+    for (int r = 0; r < NU; ++r) {
+        const int q = tid + r * NT;
+        ou[r] = -1;
+        if (q < UP) {
+            const int gi = 2 * I0 - 3 + q % UW, gj = 2 * J0 - 3 + q / UW;
+            if (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny) ou[r] = gi + sy * gj;
+        }
+    }
+    // this lane's residual cells: fine region index tid + r NT, its place in
+    // the u region, its field offset (-1: none) and x/y factors
+    int of[NF], qu[NF];
+    T fxm[NF], fxp[NF], fym[NF], fyp[NF], gxy[NF];
+#pragma unroll
+    for (int r = 0; r < NF; ++r) {
+        const int q = tid + r * NT;
+        of[r] = -1;
+        qu[r] = UW + 1;
+        fxm[r] = fxp[r] = fym[r] = fyp[r] = gxy[r] = T(0);
+        if (q < FP) {
+            const int fi = q % FW, fj = q / FW;
+            const int gi = 2 * I0 - 2 + fi, gj = 2 * J0 - 2 + fj;
+            qu[r] = (fi + 1) + UW * (fj + 1);
+            if (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny) {
+                of[r] = gi + sy * gj;
+                if constexpr (!STN) {
+                    const bool ex = OP == OP_ALIGNED && (gi == 1 || gi == nx);
+                    const bool ey = OP == OP_ALIGNED && (gj == 1 || gj == ny);
+                    const T hx = ex ? F.hxb : F.hx, hy = ey ? F.hyb : F.hy;
+                    fxm[r] = gi > 1 ? hx : T(0);
+                    fxp[r] = gi < nx ? hx : T(0);
+                    fym[r] = gj > 1 ? hy : T(0);
+                    fyp[r] = gj < ny ? hy : T(0);
+                    const T gx = ex ? F.gxb : F.gx, gy = ey ? F.gyb : F.gy;
+                    T g = T(0);
+                    if (gi == 1 && F.bc[LEFT] == BC_DIRC) g += gx;
+                    if (gi == nx && F.bc[RIGHT] == BC_DIRC) g += gx;
+                    if (gj == 1 && F.bc[BOTTOM] == BC_DIRC) g += gy;
+                    if (gj == ny && F.bc[TOP] == BC_DIRC) g += gy;
+                    gxy[r] = g;
+                }
+            }
+        }
+    }
+    double wx0 = 0, wx1 = 0, wx2 = 0, wy0 = 0, wy1 = 0, wy2 = 0;
+    if (in) {
+        wx0 = W.wx[3 * (I - 1)], wx1 = W.wx[3 * (I - 1) + 1], wx2 = W.wx[3 * (I - 1) + 2];
+        wy0 = W.wy[3 * (J - 1)], wy1 = W.wy[3 * (J - 1) + 1], wy2 = W.wy[3 * (J - 1) + 2];
+    }
+    const int qc = (2 * tx + 1) + FW * (2 * ty + 1);
+
+    auto plane = [&](int p) { return min(max(p, 0), nz + 1) * szi; };
+    auto slot = [](int p) { return (p + 4) & 3; };
+    // register sets: u/D of plane p and b of plane p - 1 (the residual plane
+    // that plane p completes); reac of plane p - 1 likewise
+    T pu[2][NU], pD[2][STN ? 1 : NU], pb[2][NF], pq[2][REAC && !STN ? NF : 1];
+    bool okp[2];
+    auto gload = [&](int p, int x) {
+        const int pk = plane(p), pkb = plane(p - 1);
+        okp[x] = p >= 1 && p <= nz;
+#pragma unroll
+        for (int r = 0; r < NU; ++r) {
+            const int o = pk + (ou[r] >= 0 ? ou[r] : 0);
+            pu[x][r] = u[o];
+            if constexpr (!STN) pD[x][r] = F.D[o];
+        }
+#pragma unroll
+        for (int r = 0; r < NF; ++r) {
+            const int o = pkb + (of[r] >= 0 ? of[r] : 0);
+            pb[x][r] = b[o];
+            if constexpr (REAC && !STN) pq[x][r] = F.reac[o];
+        }
+    };
+    auto lstore = [&](int p, int x) {
+        const int sl = slot(p);
+#pragma unroll
+        for (int r = 0; r < NU; ++r) {
+            const int q = tid + r * NT;
+            if (r * NT + NT <= UP || q < UP) {
+                const bool on = okp[x] && ou[r] >= 0;
+                su[sl][q] = on ? pu[x][r] : T(0);
+                if constexpr (!STN) sD[sl][q] = on ? pD[x][r] : T(0);
+            }
+        }
+    };
+    // residual plane p into sr[dst], with b (and reac) of plane p from set x
+    auto residual = [&](int p, int dst, const T(&bp)[NF], const T(&qp)[REAC && !STN ? NF : 1]) {
+        const bool pin = p >= 1 && p <= nz;
+        const T *Um = su[slot(p - 1)], *U0 = su[slot(p)], *Up = su[slot(p + 1)];
+        T fzm = T(0), fzp = T(0), gz = T(0);
+        if constexpr (!STN) {
+            const bool ez = OP == OP_ALIGNED && (p == 1 || p == nz);
+            const T hz = ez ? F.hzb : F.hz;
+            fzm = p > 1 ? hz : T(0);
+            fzp = p < nz ? hz : T(0);
+            if (p == 1 && F.bc[BACK] == BC_DIRC) gz += ez ? F.gzb : F.gz;
+            if (p == nz && F.bc[FRONT] == BC_DIRC) gz += ez ? F.gzb : F.gz;
+        }
+#pragma unroll
+        for (int r = 0; r < NF; ++r) {
+            const int qf = tid + r * NT;
+            if (r * NT + NT <= FP || qf < FP) {
+                T rv = T(0);
+                if (pin && of[r] >= 0) {
+                    const int q = qu[r];
+                    if constexpr (STN) {
+                        const long long c = (long long)plane(p) + of[r];
+                        T o = T(0);
+                        o = o + F.st[ST_XM][c] * U0[q - 1];
+                        o = o + F.st[ST_XP][c] * U0[q + 1];
+                        o = o + F.st[ST_YM][c] * U0[q - UW];
+                        o = o + F.st[ST_YP][c] * U0[q + UW];
+                        o = o + F.st[ST_ZM][c] * Um[q];
+                        o = o + F.st[ST_ZP][c] * Up[q];
+                        rv = bp[r] - (F.st[ST_C][c] * U0[q] + o);
+                    } else {
+                        const T *Dm = sD[slot(p - 1)], *D0 = sD[slot(p)], *Dp = sD[slot(p + 1)];
+                        const T Dc = D0[q];
+                        const T wxm = (D0[q - 1] + Dc) * fxm[r], wxp = (Dc + D0[q + 1]) * fxp[r];
+                        const T wym = (D0[q - UW] + Dc) * fym[r], wyp = (Dc + D0[q + UW]) * fyp[r];
+                        const T wzm = (Dm[q] + Dc) * fzm, wzp = (Dc + Dp[q]) * fzp;
+                        T d = ((wxm + wxp) + (wym + wyp)) + (wzm + wzp);
+                        d = d + Dc * (gxy[r] + gz) + F.dtinv;
+                        if constexpr (REAC) d = d - qp[r];
+                        T nb = wxm * U0[q - 1];
+                        nb = nb + wxp * U0[q + 1];
+                        nb = nb + wym * U0[q - UW];
+                        nb = nb + wyp * U0[q + UW];
+                        nb = nb + wzm * Um[q];
+                        nb = nb + wzp * Up[q];
+                        rv = bp[r] - (d * U0[q] - nb);
+                    }
+                }
+                sr[dst][qf] = rv;
+            }
+        }
+    };
+    auto xysum = [&](int sl) {
+        const T *s_ = sr[sl];
+        const double r0 = (wx0 * (double)s_[qc - FW - 1] + wx1 * (double)s_[qc - FW]) + wx2 * (double)s_[qc - FW + 1];
+        const double r1 = (wx0 * (double)s_[qc - 1] + wx1 * (double)s_[qc]) + wx2 * (double)s_[qc + 1];
+        const double r2 = (wx0 * (double)s_[qc + FW - 1] + wx1 * (double)s_[qc + FW]) + wx2 * (double)s_[qc + FW + 1];
+        return (wy0 * r0 + wy1 * r1) + wy2 * r2;
+    };
+
+    // warm-up: u/D planes 2Kb-3 .. 2Kb-1 and the residual of plane 2Kb-2
+    const int f0 = 2 * Kb - 2;
+    gload(f0 - 1, 0);
+    lstore(f0 - 1, 0);
+    gload(f0, 1);
+    lstore(f0, 1);
+    gload(f0 + 1, 0);
+    lstore(f0 + 1, 0);
+    T bw[NF], qw[REAC && !STN ? NF : 1];
+#pragma unroll
+    for (int r = 0; r < NF; ++r) bw[r] = pb[0][r];
+    if constexpr (REAC && !STN) {
+#pragma unroll
+        for (int r = 0; r < NF; ++r) qw[r] = pq[0][r];
+    }
+    gload(f0 + 2, 0);
+    gload(f0 + 3, 1);
+    __syncthreads();
+    residual(f0, 0, bw, qw);
+    __syncthreads();
+    double sprev = in ? xysum(0) : 0.0;
+    for (int K = Kb; K <= Ke; ++K) {
+        const int p1 = 2 * K - 1, p2 = 2 * K;
+        lstore(p2, 0);
+        lstore(p2 + 1, 1);
+        T b1[NF], b2[NF], q1[REAC && !STN ? NF : 1], q2[REAC && !STN ? NF : 1];
+#pragma unroll
+        for (int r = 0; r < NF; ++r) b1[r] = pb[0][r], b2[r] = pb[1][r];
+        if constexpr (REAC && !STN) {
+#pragma unroll
+            for (int r = 0; r < NF; ++r) q1[r] = pq[0][r], q2[r] = pq[1][r];
+        }
+        __syncthreads();
+        if (K < Ke) {
+            gload(p2 + 2, 0);
+            gload(p2 + 3, 1);
+        }
+        residual(p1, 0, b1, q1);
+        residual(p2, 1, b2, q2);
+        __syncthreads();
+        if (in) {
+            const double s1 = xysum(0), s2 = xysum(1);
+            const double *wz = W.wz + 3 * (K - 1);
+            cf[I + (long long)C.sy * J + C.sz * K] = (T)((wz[0] * sprev + wz[1] * s1) + wz[2] * s2);
+            sprev = s2;
+        }
+    }
+}
+
 // Trilinear prolongation + correction on the aligned hierarchy, marching
 // fine z-planes in odd/even pairs: each lane interpolates the coarse grid
 // in x and y once per coarse plane (fine cell f takes coarse (f+1)/2 when f
