@@ -190,6 +190,7 @@ struct mps_ctx {
     // outer solve steps captured as HIP graphs and replayed (see
     // outer_step): [0] the V-cycle step, [1] the full-multigrid first step
     bool graphs = true;                  // MPS3D_NO_GRAPHS=1 turns replay off
+    bool fused_rr = true;                // MPS3D_NO_FUSED_RESTRICT=1: residual and restriction apart
     StepGraph step[2];
     unsigned gen = 0;                    // bumped by set_bc / set_coeffs
     bool mg_ok = true;                   // a multigrid hierarchy exists
@@ -597,7 +598,7 @@ constexpr int kRRCX = 32, kRRCY = MPS_RR_CY, kRRM = MPS_RR_M;
 template <typename T>
 bool resrestrict_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op)
 {
-    if (c->opts.hierarchy != MPS_HIER_ALIGNED || c->has_vel) return false;
+    if (!c->fused_rr || c->opts.hierarchy != MPS_HIER_ALIGNED || c->has_vel) return false;
     if (c->geom[l].ng() >= (1LL << 31)) return false;
     if (op == OP_STENCIL) return (MPS_FUSED_RESTRICT & 4) != 0;
     return view(c, L, l).fast && (MPS_FUSED_RESTRICT & (sizeof(T) == 4 ? 1 : 2)) != 0;
@@ -1618,6 +1619,8 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
         {
             const char *ng = std::getenv("MPS3D_NO_GRAPHS");
             c->graphs = !(ng && ng[0] == '1');
+            const char *nf = std::getenv("MPS3D_NO_FUSED_RESTRICT");
+            c->fused_rr = !(nf && nf[0] == '1');
         }
         HIPCHK(hipEventCreate(&c->ev0));
         HIPCHK(hipEventCreate(&c->ev1));

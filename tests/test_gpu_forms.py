@@ -159,6 +159,29 @@ def test_fmg_start(gpu, prec, n):
         assert np.linalg.norm(u[1:-1, 1:-1, 1:-1] - ref[1:-1, 1:-1, 1:-1]) / np.linalg.norm(ref[1:-1, 1:-1, 1:-1]) < 1e-8
 
 
+@pytest.mark.parametrize("n", [33, 65])
+def test_fused_residual_restriction_matches_the_two_kernels(gpu, n, monkeypatch):
+    """The FP32 cycle's finest level forms its residual and restricts it in
+    one kernel (k_resrestrict_planes). With the fusion switched off
+    (MPS3D_NO_FUSED_RESTRICT=1, read at mps_create) the residual and the
+    restriction run as two kernels: every defect of the history agrees to
+    FP32 rounding of the inner cycle, and so does the solution."""
+    c = W.varcoef_poisson(n)
+    out = {}
+    for off in ("0", "1"):
+        monkeypatch.setenv("MPS3D_NO_FUSED_RESTRICT", off)
+        with FieldSolver(c.n, c.length, hierarchy="aligned", precision="fp32dc", max_cycles=40, tol=1e-11,
+                         tol_kind="rel", arith="fast", start="fmg") as s:
+            s.set_case(c)
+            u = np.zeros(s.shape, order="F")
+            k, r, h, st = s.solve(u, raise_on_failure=True)
+            out[off] = (u[1:-1, 1:-1, 1:-1].copy(), k, h)
+    (ua, ka, ha), (ub, kb, hb) = out["0"], out["1"]
+    assert ka == kb
+    assert np.allclose(ha, hb, rtol=1e-3, atol=0)
+    assert np.linalg.norm(ua - ub) / np.linalg.norm(ub) < 1e-10
+
+
 def test_fmg_start_needs_the_aligned_hierarchy(gpu):
     from mps3d_amd import _lib as M
     with pytest.raises(M.MpsError):
