@@ -165,7 +165,7 @@ def measured_traffic(kernel, n):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get("traffic_bytes") if d.get("kernel") == kernel else None
+    return d.get("traffic_bytes") if str(d.get("kernel", "")).split("<")[0] == kernel else None
 
 
 def cpu_baseline(n_gpu, cycles_gpu, tol=1e-10):

@@ -17,6 +17,7 @@ counter bytes, separately for reads and writes.
 
 Usage:
     python -m mps3d_amd.profiling stats <kernel_stats.csv> [<kernel_trace.csv>]
+    python -m mps3d_amd.profiling solve <kernel_trace.csv> [--index I] [--marker NAME]
     python -m mps3d_amd.profiling traffic <fetch.csv> <write.csv> --kernel NAME [--grid THREADS]
     python -m mps3d_amd.profiling table <fetch.csv> <write.csv> NAME [NAME ...]
 """
@@ -56,6 +57,33 @@ def stats_table(stats_csv, trace_csv=None, top=25):
         for (name, g), d in ranked:
             out.append("| `%s` | %d | %d | %.2f | %.3f |" % (
                 _short(name, 60), g, len(d), sum(d) / len(d) / 1e3, sum(d) / 1e6))
+    return "\n".join(out)
+
+
+def solve_table(trace_csv, index=1, marker="k_dc_scale", top=12):
+    """Markdown table of one solve out of a kernel trace: the kernels from
+    the index-th launch of `marker` (one per FP32+DC solve) up to the first
+    idle gap above 300 us, by kernel and launch shape, with the solve's span
+    (first start to last end) and the time inside kernels."""
+    rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
+    marks = [i for i, r in enumerate(rows) if marker in r["Kernel_Name"]]
+    if len(marks) <= index:
+        raise SystemExit("%d launches of %s in %s" % (len(marks), marker, trace_csv))
+    seg = rows[marks[index]:marks[index + 1] if index + 1 < len(marks) else len(rows)]
+    for i in range(1, len(seg)):
+        if int(seg[i]["Start_Timestamp"]) - int(seg[i - 1]["End_Timestamp"]) > 300000:
+            seg = seg[:i]
+            break
+    span = (int(seg[-1]["End_Timestamp"]) - int(seg[0]["Start_Timestamp"])) / 1e6
+    busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in seg) / 1e6
+    agg = defaultdict(list)
+    for r in seg:
+        agg[(_short(r["Kernel_Name"].split("(")[0], 70), _grid_total(r))].append(
+            int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    out = ["%d kernels, span %.3f ms, %.3f ms in kernels" % (len(seg), span, busy), "",
+           "| kernel | grid (threads) | launches | total us | avg us |", "|---|---|---|---|---|"]
+    for (name, g), d in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        out.append("| `%s` | %d | %d | %.1f | %.1f |" % (name, g, len(d), sum(d) / 1e3, sum(d) / len(d) / 1e3))
     return "\n".join(out)
 
 
@@ -138,6 +166,10 @@ def main(argv=None):
     t.add_argument("write_csv")
     t.add_argument("--kernel", required=True)
     t.add_argument("--grid", type=int, default=None, help="total grid size in threads (one launch shape)")
+    so = sub.add_parser("solve")
+    so.add_argument("trace_csv")
+    so.add_argument("--index", type=int, default=1, help="which launch of the marker starts the solve")
+    so.add_argument("--marker", default="k_dc_scale")
     tt = sub.add_parser("table")
     tt.add_argument("fetch_csv")
     tt.add_argument("write_csv")
@@ -147,6 +179,8 @@ def main(argv=None):
         print(traffic_table(a.fetch_csv, a.write_csv, a.kernels))
     elif a.cmd == "stats":
         print(stats_table(a.stats_csv, a.trace_csv))
+    elif a.cmd == "solve":
+        print(solve_table(a.trace_csv, a.index, a.marker))
     else:
         print(json.dumps(traffic(a.fetch_csv, a.write_csv, a.kernel, a.grid), indent=1))
     return 0

@@ -30,3 +30,32 @@ def test_traffic_scales_by_the_calibration_copy(tmp_path):
     assert t["fetch"]["bytes"] == 500.0 * 1024 * 2
     assert t["write"]["bytes"] == 100.0 * 1024
     assert t["traffic_bytes"] == t["fetch"]["bytes"] + t["write"]["bytes"]
+
+
+def test_solve_table_cuts_one_solve_out_of_a_trace(tmp_path):
+    """Kernels from the second marker launch up to the first idle gap above
+    300 us, grouped by name and grid, with span and in-kernel time."""
+    path = tmp_path / "trace.csv"
+    rows = []
+    t = 0
+
+    def launch(name, grid, dur, gap=1000):
+        nonlocal t
+        t += gap
+        rows.append([name + "(args)", grid, 1, 1, t, t + dur])
+        t += dur
+
+    for solve in range(3):
+        launch("k_dc_scale", 64, 2000, gap=10_000_000)
+        for _ in range(2):
+            launch("k_sweep_fast<float>", 1000, 90_000)
+        launch("k_add_interior", 5000, 60_000)
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Kernel_Name", "Grid_Size_X", "Grid_Size_Y", "Grid_Size_Z", "Start_Timestamp", "End_Timestamp"])
+        w.writerows(rows)
+    out = Pf.solve_table(str(path), index=1)
+    # 2 + 90 + 90 + 60 us in kernels, 3 gaps of 1 us, the 10 ms gap excluded
+    assert out.startswith("4 kernels, span 0.245 ms, 0.242 ms in kernels")
+    assert "| `k_sweep_fast<float>` | 1000 | 2 | 180.0 | 90.0 |" in out
+    assert "| `k_add_interior` | 5000 | 1 | 60.0 | 60.0 |" in out
