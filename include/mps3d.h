@@ -164,6 +164,17 @@ int mps_vcycle(mps_ctx *ctx, double *soln, const double *mgridsterm, double *res
  * each cycle. Returns MPS_OK or MPS_NOT_CONVERGED. */
 int mps_solve(mps_ctx *ctx, double *soln, int *ncycles, double *resnorm, double *history);
 
+/* The solve of mps_solve for a right-hand side given directly: A x = b with
+ * b the whole right-hand side (interior array: the source and any boundary
+ * terms, as find_bvec returns them) and homogeneous boundary data in the
+ * cycle; the context's source and faces are not used. For callers whose
+ * right-hand side changes from call to call while the coefficients stay,
+ * e.g. the Poisson solve of every time step (poisson_solve's loop,
+ * poissoneq.f90:427-450, with only the space charge changed): no
+ * coefficient set-up per call. Stops as mps_solve. */
+int mps_solve_rhs(mps_ctx *ctx, const double *b, double *soln, int *ncycles, double *resnorm,
+                  double *history);
+
 /* find_AX (discretepde.f90:672-1119): AX, diag and offdiag (interior arrays,
  * each nullable) for the ghosted input x. */
 int mps_apply(mps_ctx *ctx, const double *x, double *ax, double *diag, double *offdiag);

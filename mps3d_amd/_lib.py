@@ -27,7 +27,7 @@ ARITH_EXACT, ARITH_FAST = 0, 1
 EXPORTS = (
     "mps_default_opts", "mps_create", "mps_destroy", "mps_set_bc", "mps_set_bc_str", "mps_set_coeffs",
     "mps_vcycle", "mps_solve", "mps_apply", "mps_apply_transpose", "mps_bvec", "mps_smooth",
-    "mps_fill_ghosts", "mps_gmres", "mps_solve_transpose", "mps_last_timing", "mps_stream",
+    "mps_fill_ghosts", "mps_gmres", "mps_solve_transpose", "mps_last_timing", "mps_stream", "mps_solve_rhs",
     "mps_error_string", "mps_last_error", "mps_time_kernel", "mps_grad_velocity",
 )
 
@@ -71,6 +71,7 @@ def load():
     L.mps_set_coeffs.argtypes = [_vp] + [_vp] * 6 + [C.c_int, C.c_double]
     L.mps_vcycle.argtypes = [_vp, _vp, _vp, _dp]
     L.mps_solve.argtypes = [_vp, _vp, C.POINTER(C.c_int), _dp, _vp]
+    L.mps_solve_rhs.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_int), _dp, _vp]
     L.mps_apply.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.mps_apply_transpose.argtypes = [_vp, _vp, _vp]
     L.mps_bvec.argtypes = [_vp, _vp, C.c_int, _vp]

@@ -41,14 +41,17 @@ class Operator:
 
     def __init__(self, n, length, dcoeff, bc_codes, flux="upwind", reac=None, timederiv=False, dt=1.0,
                  hierarchy="ref", p=(1, 1, 1), tol=1e-13, max_cycles=80, precision="fp64", cycles=None,
-                 method="vcycle", gmres_m=30, gmres_restarts=20):
+                 method="vcycle", gmres_m=30, gmres_restarts=20, start="guess", arith="exact"):
         """cycles: a fixed number of V-cycles per solve instead of a
         tolerance, which makes the discrete solution map a smooth function
         of the inputs (needed when finite differences probe it).
         method: "vcycle" iterates V-cycles (mps_solve); "gmres" runs
         FGMRES(gmres_m) preconditioned by one V-cycle (mps_gmres, precond 1)
         to the same relative tolerance. The adjoint solve iterates V-cycles
-        of the transposed operator either way."""
+        of the transposed operator either way. start: "fmg" begins each
+        V-cycle solve with a full-multigrid pass (aligned hierarchy); arith:
+        the solver's own rows in the reference's rounding ("exact") or with
+        precomputed factors ("fast", the same operator)."""
         self.n = tuple(int(v) for v in n)
         if method not in ("vcycle", "gmres"):
             raise ValueError("method must be 'vcycle' or 'gmres'")
@@ -60,34 +63,39 @@ class Operator:
         if self.fixed:
             tol, max_cycles = 0.0, int(cycles)
         self.solver = FieldSolver(self.n, length, p=p, hierarchy=hierarchy, precision=precision, flux=flux,
-                                  max_cycles=max_cycles, tol=tol, tol_kind="rel")
+                                  max_cycles=max_cycles, tol=tol, tol_kind="rel", start=start, arith=arith)
         self.solver.set_bc(bc_codes, (0.0,) * 6)
         self.D = pad(dcoeff.to(torch.float64))
         self.reac = None if reac is None else pad(reac.to(torch.float64))
         self.timederiv, self.dt = bool(timederiv), float(dt)
         self._zero = torch.zeros_like(self.D)
+        self._static = False  # the coefficients without velocities are set
+        self._rhs = None
         self.cycles = []
 
     def configure(self, vel=None, rhs=None):
         """Sets the operator for velocities vel (3 interior tensors or None)
-        and the right-hand side rhs (interior, or None for zero)."""
+        and the right-hand side rhs (interior, or None for zero). Without
+        velocities the operator is the same every time, so its coefficients
+        (and the hierarchy built from them) are set once; the right-hand
+        side goes to the solve itself."""
         if vel is None:
-            vg = (None, None, None)
+            if not self._static:
+                self.solver.set_coeffs(self.D, None, None, None, self.reac, self._zero, self.timederiv, self.dt)
+                self._static = True
         else:
             vg = tuple(pad(v.detach()) if v is not None else self._zero for v in vel)
-        src = self._zero if rhs is None else pad(rhs.detach())
-        self._keep = (vg, src)
+            self.solver.set_coeffs(self.D, *vg, self.reac, self._zero, self.timederiv, self.dt)
+            self._static = False
         self._rhs = None if rhs is None else rhs.detach().contiguous()
-        self.solver.set_coeffs(self.D, *vg, self.reac, src, self.timederiv, self.dt)
 
     def solve(self, x0=None):
         x = torch.zeros_like(self.D) if x0 is None else pad(x0.detach())
+        b = self._rhs if self._rhs is not None else torch.zeros(self.n[::-1], dtype=torch.float64, device=x.device)
         if self.method == "gmres":
-            b = self._rhs if self._rhs is not None else torch.zeros(self.n[::-1], dtype=torch.float64,
-                                                                    device=x.device)
             k, r, st = self.solver.gmres(b, x, m=self.gmres_m, restarts=self.gmres_restarts, tol=self.tol)
         else:
-            k, r, _, st = self.solver.solve(x)
+            k, r, _, st = self.solver.solve_rhs(b, x)
         if st != M.OK and not (self.fixed and st == M.NOT_CONVERGED):
             raise M.MpsError(st, "forward solve did not converge (%g after %d %s)" % (
                 r, k, "iterations" if self.method == "gmres" else "cycles"))

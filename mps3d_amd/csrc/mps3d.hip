@@ -785,9 +785,12 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
 template <typename T>
 bool fused_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op)
 {
-    // (the fused kernels address planes with 32-bit offsets; explicit rows
-    // are fused only without convection, i.e. the conductance coarse rows)
-    return !c->has_vel && L[l].u2 && c->geom[l].ng() < (1LL << 31);
+    // (the fused kernels address planes with 32-bit offsets). Explicit rows
+    // -- conductance coarse rows, Scharfetter-Gummel rows with or without
+    // drift, transposed rows -- carry everything a Gauss-Seidel update
+    // needs; rows formed from coefficients are fused only without
+    // convection (the fused kernels have no upwind terms)
+    return (op == OP_STENCIL || !c->has_vel) && L[l].u2 && c->geom[l].ng() < (1LL << 31);
 }
 
 // nu red-black sweeps on level l: pairs of fused sweeps u -> u2 -> u where
@@ -1979,6 +1982,50 @@ int mps_solve(mps_ctx *c, double *soln, int *ncycles, double *resnorm, double *h
         compute_b0(c, nullptr);
         double meas = 0.0;
         const int it = solve_core(c, u, c->b0, false, history, meas);
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_scalar + 1, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        unbind_solution(c, soln, staged);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last_ms = ms;
+        c->last_cycles = it;
+        if (ncycles) *ncycles = it;
+        if (resnorm) *resnorm = meas;
+        int bstat;
+        std::memcpy(&bstat, c->h_scalar + 1, sizeof(int));
+        if (bstat != 0) return fail(MPS_ERR_NAN, "NaN or Inf in the coarse-grid GMRES");
+        if (!std::isfinite(meas)) return fail(MPS_ERR_NAN, "residual is not finite");
+        status = meas <= c->opts.tol ? MPS_OK : MPS_NOT_CONVERGED;
+        if (status == MPS_NOT_CONVERGED) fail(MPS_NOT_CONVERGED, "tolerance %g not met after %d cycles (%g)",
+                                              c->opts.tol, it, meas);
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return status;
+}
+
+int mps_solve_rhs(mps_ctx *c, const double *bsrc, double *soln, int *ncycles, double *resnorm, double *history)
+{
+    int st = guard_mg(c);
+    if (st) return st;
+    if (!bsrc || !soln) return fail(MPS_ERR_ARG, "null argument");
+    int status = MPS_NOT_CONVERGED;
+    try {
+        const Geom &g = c->geom[0];
+        bool staged;
+        double *u = bind_solution(c, soln, staged);
+        if (!c->gbuf) {
+            c->gbuf = dalloc<double>(g.ng());
+            fill<double>(c, c->gbuf, g.ng(), 0.0);  // ghost layers stay zero
+        }
+        const double *bd = stage_interior(c, bsrc);
+        Lev<double> v = view(c, c->L64, 0);
+        hipLaunchKernelGGL(k_interior_in<double>, cells_grid(g), block(), 0, c->stream, v, bd, c->gbuf);
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        fill<int>(c, c->d_status, 1, 0);
+        double meas = 0.0;
+        const int it = solve_core(c, u, c->gbuf, true, history, meas);
         HIPCHK(hipEventRecord(c->ev1, c->stream));
         HIPCHK(hipMemcpyAsync(c->h_scalar + 1, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         unbind_solution(c, soln, staged);

@@ -141,10 +141,11 @@ class Discharge:
         ones = torch.ones(shape, dtype=torch.float64, device=self.dev)
         self.eps_r = ones if eps_r is None else eps_r.to(self.dev, torch.float64)
         L3 = (self.L,) * 3
-        common = dict(hierarchy=hierarchy, p=p, precision=precision)
+        common = dict(hierarchy=hierarchy, p=p, precision=precision, arith="fast")
         ptol = tol if poisson_tol is None else poisson_tol
         self.poisson = Operator(self.n, L3, self.eps_r, ("ZGRD",) * 4 + ("DIRC", "DIRC"), flux="upwind",
-                                tol=ptol, **common)
+                                tol=ptol, start="fmg" if hierarchy == "aligned" else "guess", **common)
+        self._phi = None  # the last potential: the next Poisson solve starts from it
         self.ops = []
         for s in self.species:
             codes = (s.wall,) * 4 + (s.electrode,) * 2
@@ -167,7 +168,9 @@ class Discharge:
         return g[:, None, None] * g[None, :, None] * g[None, None, :]
 
     def solve_poisson(self, rhs):
-        return linear_solve(self.poisson, rhs)
+        phi = linear_solve(self.poisson, rhs, x0=self._phi)
+        self._phi = phi.detach()
+        return phi
 
     def solve_species(self, k, rhs, vel, x0):
         return linear_solve(self.ops[k], rhs, vel, x0=x0)

@@ -81,6 +81,14 @@ module mps3d_gpu
             real(c_double), intent(out) :: resnorm
         end function mps_solve
 
+        integer(c_int) function mps_solve_rhs(ctx, b, soln, ncycles, resnorm, history) &
+                bind(C, name='mps_solve_rhs')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, b, soln, history
+            integer(c_int), intent(out) :: ncycles
+            real(c_double), intent(out) :: resnorm
+        end function mps_solve_rhs
+
         integer(c_int) function mps_fill_ghosts(ctx, soln) bind(C, name='mps_fill_ghosts')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, soln
@@ -188,6 +196,23 @@ contains
         status = mps_solve(ctx, c_loc(solnvar), n, resnorm, c_loc(history))
         ncycles = n
     end subroutine gpu_solve
+
+    ! the same solve for a right-hand side b (lx,ly,lz) given directly, with
+    ! the coefficients of the last gpu_set_problem: for a Poisson solve per
+    ! time step whose space charge changes and whose permittivity does not
+    subroutine gpu_solve_rhs(ctx, b, solnvar, lx, ly, lz, ncycles, resnorm, history, status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: lx, ly, lz
+        real*8, intent(in), target :: b(lx, ly, lz)
+        real*8, intent(inout), target :: solnvar(0:lx+1, 0:ly+1, 0:lz+1)
+        integer, intent(out) :: ncycles
+        real*8, intent(out) :: resnorm
+        real*8, intent(out), target :: history(*)
+        integer, intent(out) :: status
+        integer(c_int) :: n
+        status = mps_solve_rhs(ctx, c_loc(b), c_loc(solnvar), n, resnorm, c_loc(history))
+        ncycles = n
+    end subroutine gpu_solve_rhs
 
     ! text of the library's last error
     function gpu_last_error() result(msg)

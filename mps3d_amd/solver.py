@@ -239,6 +239,21 @@ class FieldSolver:
         M.check(st, allow=(M.OK,) if raise_on_failure else (M.OK, M.NOT_CONVERGED))
         return n.value, r.value, hist[:n.value].copy(), st
 
+    def solve_rhs(self, b, soln, raise_on_failure=False):
+        """A x = b for a right-hand side b (interior) given directly, with the
+        coefficients of the last set_coeffs (their source and the faces are
+        not used). Returns (ncycles, resnorm, history, status)."""
+        n = C.c_int()
+        r = C.c_double()
+        hist = np.zeros(self.max_cycles)
+        try:
+            st = self.lib.mps_solve_rhs(self._h, self._ptr(b, "i"), self._ptr(soln, "g", True), C.byref(n),
+                                        C.byref(r), C.c_void_p(hist.ctypes.data))
+        finally:
+            self._release()
+        M.check(st, allow=(M.OK,) if raise_on_failure else (M.OK, M.NOT_CONVERGED))
+        return n.value, r.value, hist[:n.value].copy(), st
+
     def last_timing(self):
         ms = C.c_double()
         n = C.c_int()
