@@ -6,7 +6,8 @@
 ! cell centres (tests/Laplace_test/README:1-4).
 !
 ! Part 1 loops gpu_perform_vcycle the way generic_pde/pdedefs.f90:378-402
-! loops perform_vcycle; part 2 makes the same solve with one gpu_solve.
+! loops perform_vcycle; part 2 makes the same solve with one gpu_solve;
+! part 3 with gpu_solve_rhs and the right-hand side passed directly.
 ! Exit status: 0 pass, 1 fail, 2 no usable GPU.
 program laplace_gpu
     use mps3d_gpu
@@ -85,6 +86,25 @@ program laplace_gpu
         err = max(err, maxval(abs(u(i, 1:n, 1:n) - x)))
     enddo
     print '(a,i3,a,es12.4,a,es12.4)', ' gpu_solve: cycles', ncyc, ' resnorm', resnorm, ' max |phi - x| =', err
+    if (err > 1.d-10) call exit(1)
+
+    ! part 3: the same system with its right-hand side passed directly (what
+    ! find_bvec returns for the deck: the DIRC term 2 D g / h^2 of the x = 1
+    ! face, D = g = 1, h = 1/n), the coefficients as set above
+    sterm = 0.d0
+    sterm(n, :, :) = 2.d0 * dble(n) * dble(n)
+    u = 0.d0
+    call gpu_solve_rhs(ctx, sterm, u, n, n, n, ncyc, resnorm, hist, st)
+    if (st /= MPS_OK .and. st /= MPS_NOT_CONVERGED) then
+        print *, 'laplace_gpu: solve_rhs failed ', st, trim(gpu_last_error())
+        call exit(1)
+    endif
+    err = 0.d0
+    do i = 1, n
+        x = (i - 0.5d0) / n
+        err = max(err, maxval(abs(u(i, 1:n, 1:n) - x)))
+    enddo
+    print '(a,i3,a,es12.4,a,es12.4)', ' gpu_solve_rhs: cycles', ncyc, ' resnorm', resnorm, ' max |phi - x| =', err
     if (err > 1.d-10) call exit(1)
     st = mps_destroy(ctx)
     print *, 'laplace_gpu: PASS'

@@ -182,6 +182,38 @@ def test_fused_residual_restriction_matches_the_two_kernels(gpu, n, monkeypatch)
     assert np.linalg.norm(ua - ub) / np.linalg.norm(ub) < 1e-10
 
 
+@pytest.mark.parametrize("hier,n,prec", [("aligned", 33, "fp64"), ("aligned", 33, "fp32dc"), ("ref", 32, "fp64")])
+def test_solve_rhs_matches_solve(gpu, hier, n, prec):
+    """mps_solve_rhs with find_bvec's right-hand side (source and boundary
+    terms) is mps_solve of the same problem: on the aligned hierarchy (a
+    correction scheme, boundary data only in b) the same iterates; with the
+    reference FAS cycle, whose coarse ghosts use the boundary faces that
+    mps_solve_rhs treats as zero (they are in b), the same solution of the
+    same system by a slightly different path. A second right-hand side needs
+    no new coefficients."""
+    c = W.varcoef_poisson(n)
+    with FieldSolver(c.n, c.length, hierarchy=hier, precision=prec, max_cycles=40, tol=1e-11, tol_kind="rel") as s:
+        s.set_case(c)
+        b = s.find_bvec(None, True)
+        u1 = np.zeros(s.shape, order="F")
+        k1, r1, h1, _ = s.solve(u1, raise_on_failure=True)
+        u2 = np.zeros(s.shape, order="F")
+        k2, r2, h2, _ = s.solve_rhs(b, u2, raise_on_failure=True)
+        if hier == "aligned":
+            assert k1 == k2
+            assert np.allclose(h1, h2, rtol=1e-12, atol=0)
+            assert np.abs(u1[1:-1, 1:-1, 1:-1] - u2[1:-1, 1:-1, 1:-1]).max() <= 1e-12 * np.abs(u1).max()
+        else:
+            assert abs(k1 - k2) <= 1 and r2 <= 1e-11
+            assert np.allclose(h1, h2, rtol=1e-3, atol=0)
+            assert np.abs(u1[1:-1, 1:-1, 1:-1] - u2[1:-1, 1:-1, 1:-1]).max() <= 1e-9 * np.abs(u1).max()
+        # linearity in the right-hand side: 3 b gives 3 u
+        u3 = np.zeros(s.shape, order="F")
+        s.solve_rhs(3.0 * b, u3, raise_on_failure=True)
+        ref = u2[1:-1, 1:-1, 1:-1]
+        assert np.linalg.norm(u3[1:-1, 1:-1, 1:-1] - 3.0 * ref) / np.linalg.norm(3.0 * ref) < 1e-9
+
+
 def test_fmg_start_needs_the_aligned_hierarchy(gpu):
     from mps3d_amd import _lib as M
     with pytest.raises(M.MpsError):
