@@ -794,15 +794,24 @@ bool fused_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op
 }
 
 // nu red-black sweeps on level l: pairs of fused sweeps u -> u2 -> u where
-// the operator has no convection, in-place colour passes otherwise
+// the fused kernels apply, in-place colour passes otherwise. swap_odd (the
+// cycle's pre- and post-smoothing counts add up to an even number): an odd
+// last sweep is a fused sweep u -> u2 after which the level's u and u2
+// swap roles, swapped back by the other odd smoothing of the same cycle.
 template <typename T>
-void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op, bool zero_first = false)
+void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op, bool zero_first = false,
+                  bool swap_odd = false)
 {
     int s = 0;
     if (fused_ok(c, L, l, op)) {
         for (; s + 2 <= nu; s += 2) {
             launch_sweep<T>(c, L, l, L[l].u, L[l].u2, op, zero_first && s == 0);
             launch_sweep<T>(c, L, l, L[l].u2, L[l].u, op);
+        }
+        if (s < nu && swap_odd) {
+            launch_sweep<T>(c, L, l, L[l].u, L[l].u2, op, zero_first && s == 0);
+            std::swap(L[l].u, L[l].u2);
+            ++s;
         }
     }
     for (; s < nu; ++s) {
@@ -818,7 +827,8 @@ void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op
 template <typename T>
 bool zero_first_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op)
 {
-    return c->opts.nu1 >= 2 && fused_ok(c, L, l, op) && (op == OP_STENCIL || view(c, L, l).fast);
+    const bool first_fused = c->opts.nu1 >= 2 || (c->opts.nu1 == 1 && c->opts.nu2 % 2 == 1);
+    return first_fused && fused_ok(c, L, l, op) && (op == OP_STENCIL || view(c, L, l).fast);
 }
 
 // ---------------------------------------------- LDS-resident small levels
@@ -893,7 +903,10 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
         zg = zero_first_ok<T>(c, L, l, op);
         if (!zg) fill<T>(c, L[l].u, g.ng(), T(0));
     }
-    smooth_level<T>(c, L, l, c->opts.nu1, op, zg);
+    // odd smoothing counts on both sides: one swap each, so the level's
+    // buffers are back in place when the cycle returns
+    const bool swap_odd = c->opts.nu1 % 2 == 1 && c->opts.nu2 % 2 == 1;
+    smooth_level<T>(c, L, l, c->opts.nu1, op, zg, swap_odd);
     const bool fused = resrestrict_ok<T>(c, L, l, op);
     if (!fused) launch_residual<T, T>(c, L, l, L[l].u, L[l].b, L[l].r, nullptr, op);
     const Geom &gc = c->geom[l + 1];
@@ -917,7 +930,7 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
         vcycle<T>(c, L, l + 1, bwork, homog, true);
         launch_prolong_tri<T>(c, L, l);
     }
-    smooth_level<T>(c, L, l, c->opts.nu2, op);
+    smooth_level<T>(c, L, l, c->opts.nu2, op, false, swap_odd);
 }
 
 // Full multigrid for A e = L[0].b on the aligned hierarchy (homogeneous
