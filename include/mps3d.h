@@ -195,6 +195,17 @@ int mps_smooth(mps_ctx *ctx, double *soln, const double *mgridsterm, int niter, 
  * 2 g - u, other codes ghost = u, on the six faces of soln. */
 int mps_fill_ghosts(mps_ctx *ctx, double *soln);
 
+/* update_efield (plasma_pde/poissoneq.f90:565-680): the ghost values of phi
+ * (ghosted, in/out) from the context's boundary conditions as
+ * update_boundary_ghostvalues sets them, then the cell-centre field
+ * E = -scale * 0.5 (phi(+1) - phi(-1)) / d per direction, with the
+ * reference's minmod limiter (poissoneq.f90:549-563, 613-656) when limiter
+ * != 0 (the reference's default is off), and zero-gradient ghost faces of
+ * each component. efield is the reference's efield(0:nx+1,0:ny+1,0:nz+1,3)
+ * (three ghosted components one after the other; ghost edges and corners
+ * are not written, as in the reference); scale = potscale / lscale. */
+int mps_efield(mps_ctx *ctx, double *phi, double *efield, double scale, int limiter);
+
 /* performgmres (gmres.f90:197-434) on the finest level: restarted GMRES(m)
  * for A x = b (b interior, x ghosted in/out).
  *   precond = 0: the reference's method, left-preconditioned by one

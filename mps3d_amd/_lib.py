@@ -28,6 +28,7 @@ EXPORTS = (
     "mps_default_opts", "mps_create", "mps_destroy", "mps_set_bc", "mps_set_bc_str", "mps_set_coeffs",
     "mps_vcycle", "mps_solve", "mps_apply", "mps_apply_transpose", "mps_bvec", "mps_smooth",
     "mps_fill_ghosts", "mps_gmres", "mps_solve_transpose", "mps_last_timing", "mps_stream", "mps_solve_rhs",
+    "mps_efield",
     "mps_error_string", "mps_last_error", "mps_time_kernel", "mps_grad_velocity",
 )
 
@@ -77,6 +78,7 @@ def load():
     L.mps_bvec.argtypes = [_vp, _vp, C.c_int, _vp]
     L.mps_smooth.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
     L.mps_fill_ghosts.argtypes = [_vp, _vp]
+    L.mps_efield.argtypes = [_vp, _vp, _vp, C.c_double, C.c_int]
     L.mps_gmres.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), _dp]
     L.mps_solve_transpose.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_int), _dp]
     L.mps_last_timing.argtypes = [_vp, _dp, C.POINTER(C.c_int)]

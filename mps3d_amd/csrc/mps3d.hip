@@ -1955,6 +1955,47 @@ int mps_fill_ghosts(mps_ctx *c, double *soln)
     return MPS_OK;
 }
 
+int mps_efield(mps_ctx *c, double *phi, double *efield, double scale, int limiter)
+{
+    int st = guard(c, false);
+    if (st) return st;
+    if (!c->bc_set) return fail(MPS_ERR_STATE, "set boundary conditions first");
+    if (!phi || !efield) return fail(MPS_ERR_ARG, "null argument");
+    try {
+        const Geom &g = c->geom[0];
+        const long long ng = g.ng();
+        bool staged;
+        double *u = bind_solution(c, phi, staged);
+        const bool dev_out = is_device_ptr(efield);
+        double *E = dev_out ? efield : dalloc<double>(3 * ng);
+        if (!dev_out) HIPCHK(hipMemcpyAsync(E, efield, 3 * ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        Lev<double> v = view(c, c->L64, 0);
+        Faces F{};
+        for (int f = 0; f < 6; ++f) F.f[f] = c->faces0[f];
+        const long long nt = (long long)g.ny * g.nz + (long long)g.nx * g.nz + (long long)g.nx * g.ny;
+        const dim3 gg((unsigned)((nt + kBlock - 1) / kBlock));
+        hipLaunchKernelGGL(k_ghosts, gg, block(), 0, c->stream, v, u, F);
+        hipLaunchKernelGGL(k_efield, cells_grid(g), block(), 0, c->stream, v, (const double *)u, E, E + ng,
+                           E + 2 * ng, scale, limiter);
+        // zero-gradient ghost faces of each component (update_efield's
+        // efield_bc_codes = 'ZGRD', poissoneq.f90:580-583, 670-678)
+        Lev<double> z = v;
+        for (int f = 0; f < 6; ++f) z.bc[f] = BC_ZGRD;
+        const Faces none{};
+        for (int q = 0; q < 3; ++q) hipLaunchKernelGGL(k_ghosts, gg, block(), 0, c->stream, z, E + q * ng, none);
+        unbind_solution(c, phi, staged);
+        if (!dev_out) {
+            HIPCHK(hipMemcpyAsync(efield, E, 3 * ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            dfree(E);
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
+}
+
 int mps_vcycle(mps_ctx *c, double *soln, const double *sterm, double *resnorm)
 {
     int st = guard_mg(c);

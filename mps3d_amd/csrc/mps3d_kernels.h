@@ -806,6 +806,38 @@ __global__ void __launch_bounds__(kBlock) k_ghosts(Lev<double> L, double *u, Fac
     }
 }
 
+// update_efield (plasma_pde/poissoneq.f90:565-680) on the interior: the
+// central difference 0.5 (phi(+1) - phi(-1)) / d per direction, the minmod
+// limiter (minmodlimiter, :549-563) when limiter != 0, E = -E * scale; phi's
+// ghost values set beforehand. Same operation order as the reference.
+__device__ __forceinline__ double minmod_phi(double ntr, double dtr)
+{
+    if (dtr != 0.0) {
+        const double r = ntr / dtr;
+        const double m = r < 1.0 ? r : 1.0;
+        return m > 0.0 ? m : 0.0;
+    }
+    return 1.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_efield(Lev<double> L, const double *__restrict__ phi, double *ex,
+                                                  double *ey, double *ez, double scale, int limiter)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    const long long st[3] = {1, L.sy, L.sz};
+    const double d[3] = {L.dx, L.dy, L.dz};
+    double *E[3] = {ex, ey, ez};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double a = phi[c + st[q]], b = phi[c - st[q]], p0 = phi[c];
+        double e = 0.5 * (a - b) / d[q];
+        if (limiter) e = minmod_phi(p0 - b, a - p0) * e;
+        E[q][c] = -e * scale;
+    }
+}
+
 // explicit stencil of an OP_REF / OP_ALIGNED operator; transpose != 0 writes
 // the transposed rows (A^T: the coefficient of u(i-1) in row i becomes the
 // coefficient of u(i+1) in row i-1, and so on)

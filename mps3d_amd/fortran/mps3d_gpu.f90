@@ -89,6 +89,13 @@ module mps3d_gpu
             real(c_double), intent(out) :: resnorm
         end function mps_solve_rhs
 
+        integer(c_int) function mps_efield(ctx, phi, efield, scale, limiter) bind(C, name='mps_efield')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, phi, efield
+            real(c_double), value :: scale
+            integer(c_int), value :: limiter
+        end function mps_efield
+
         integer(c_int) function mps_fill_ghosts(ctx, soln) bind(C, name='mps_fill_ghosts')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, soln
@@ -213,6 +220,23 @@ contains
         status = mps_solve_rhs(ctx, c_loc(b), c_loc(solnvar), n, resnorm, c_loc(history))
         ncycles = n
     end subroutine gpu_solve_rhs
+
+    ! update_efield (plasma_pde/poissoneq.f90:565-680) on the device:
+    ! poissonsoln's ghosts from the context's boundary conditions, then
+    ! efield = -grad(poissonsoln) * potscale/lscale with zero-gradient ghosts
+    subroutine gpu_update_efield(ctx, poissonsoln, efield, lx, ly, lz, potscale, lscale, limiter, status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: lx, ly, lz
+        real*8, intent(inout), target :: poissonsoln(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(inout), target :: efield(0:lx+1, 0:ly+1, 0:lz+1, 3)
+        real*8, intent(in) :: potscale, lscale
+        logical, intent(in) :: limiter
+        integer, intent(out) :: status
+        integer(c_int) :: lim
+        lim = 0
+        if (limiter) lim = 1
+        status = mps_efield(ctx, c_loc(poissonsoln), c_loc(efield), potscale / lscale, lim)
+    end subroutine gpu_update_efield
 
     ! text of the library's last error
     function gpu_last_error() result(msg)

@@ -239,6 +239,24 @@ class FieldSolver:
         M.check(st, allow=(M.OK,) if raise_on_failure else (M.OK, M.NOT_CONVERGED))
         return n.value, r.value, hist[:n.value].copy(), st
 
+    def efield(self, phi, scale=1.0, limiter=False, out=None):
+        """update_efield: sets phi's ghosts (in place) and returns E as a
+        ghosted (nx+2, ny+2, nz+2, 3) array (host, Fortran order) or into
+        `out`, a device tensor of 3 (nz+2, ny+2, nx+2) components."""
+        nx, ny, nz = self.n
+        E = out if out is not None else np.zeros((nx + 2, ny + 2, nz + 2, 3), order="F")
+        try:
+            if _is_torch(E):
+                if E.numel() != 3 * int(np.prod(self.shape)):
+                    raise ValueError("efield needs 3 ghosted components")
+                ep = self._ptr(E, None, True)
+            else:
+                ep = C.c_void_p(E.ctypes.data)
+            M.check(self.lib.mps_efield(self._h, self._ptr(phi, "g", True), ep, float(scale), int(bool(limiter))))
+        finally:
+            self._release()
+        return E
+
     def solve_rhs(self, b, soln, raise_on_failure=False):
         """A x = b for a right-hand side b (interior) given directly, with the
         coefficients of the last set_coeffs (their source and the faces are

@@ -777,6 +777,48 @@ int orc_gmres(const orc_problem *p, const double *b, double *x, int maxkspdim, i
     return level_gmres(&L, b, x, maxkspdim, nrestarts, tol);
 }
 
+/* minmodlimiter, plasma_pde/poissoneq.f90:549-563 */
+static double minmod_phi(double ntr, double dtr)
+{
+    if (dtr != 0.0) {
+        double r = ntr / dtr;
+        double m = r < 1.0 ? r : 1.0;
+        return m > 0.0 ? m : 0.0;
+    }
+    return 1.0;
+}
+
+/* update_efield, plasma_pde/poissoneq.f90:565-680: the ghost values of phi
+ * from the boundary conditions (update_boundary_ghostvalues, in place), the
+ * central difference 0.5 (phi(+1) - phi(-1)) / d per direction (:592-606),
+ * the minmod limiter when asked (:613-656), E = -E * scale (:660), and the
+ * zero-gradient ghost faces of each component (:662-678; ghost edges and
+ * corners are left as they were). efield: three ghosted components one after
+ * the other, as efield(:,:,:,NDIM). */
+void orc_efield(const orc_problem *p, double *phi, double *efield, double scale, int limiter)
+{
+    lev_t L = finest(p);
+    level_ghosts(&L, phi);
+    size_t ng = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    double *E[3] = {efield, efield + ng, efield + 2 * ng};
+    const size_t sy = (size_t)(L.nx + 2), sz = sy * (size_t)(L.ny + 2);
+    const size_t st[3] = {1, sy, sz};
+    const double d[3] = {L.dx, L.dy, L.dz};
+    for (int k = 1; k <= L.nz; ++k)
+        for (int j = 1; j <= L.ny; ++j)
+            for (int i = 1; i <= L.nx; ++i) {
+                size_t c = IDX(&L, i, j, k);
+                for (int q = 0; q < 3; ++q) {
+                    double e = 0.5 * (phi[c + st[q]] - phi[c - st[q]]) / d[q];
+                    if (limiter) e = minmod_phi(phi[c] - phi[c - st[q]], phi[c + st[q]] - phi[c]) * e;
+                    E[q][c] = -e * scale;
+                }
+            }
+    lev_t Z = L;
+    for (int f = 0; f < 6; ++f) Z.bc[f] = ORC_BC_ZGRD;
+    for (int q = 0; q < 3; ++q) level_ghosts(&Z, E[q]);
+}
+
 /* find_sg_exponential_flux, mps1d/source/convdiff.f90:10-64 */
 void orc_sg_flux(double cL, double cR, double DL, double DR, double uL, double uR, double dx,
                  double *wL, double *wR, double *flux)

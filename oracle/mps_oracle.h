@@ -54,6 +54,9 @@ void orc_restrict(int nx, int ny, int nz, const double *fine, double *coarse);
 void orc_prolong1(int cx, int cy, int cz, const double *coarse_g, double *fine);
 /* update_boundary_ghostvalues with the problem's BC codes and face arrays. */
 void orc_ghosts(const orc_problem *p, double *u);
+
+/* update_efield, plasma_pde/poissoneq.f90:565-680 (efield: 3 ghosted components) */
+void orc_efield(const orc_problem *p, double *phi, double *efield, double scale, int limiter);
 /* restrict_boundary for one face array of (a,b) -> (a/2,b/2). */
 void orc_restrict_face(int a, int b, const double *f, double *c);
 /* One perform_vcycle on the finest level; returns the RMS residual. */

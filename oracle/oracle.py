@@ -52,6 +52,7 @@ def lib():
         L.orc_restrict.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.orc_prolong1.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.orc_ghosts.argtypes = [P, _dp]
+        L.orc_efield.argtypes = [P, _dp, _dp, C.c_double, C.c_int]
         L.orc_restrict_face.argtypes = [C.c_int, C.c_int, _dp, _dp]
         L.orc_vcycle.argtypes = [P, _dp, _dp]
         L.orc_vcycle.restype = C.c_double
@@ -146,6 +147,17 @@ def ghosts(p: Problem, u):
     u = np.array(u, dtype=np.float64, order="F", copy=True)
     lib().orc_ghosts(p.ref, _ptr(u))
     return u
+
+
+def efield(p: Problem, phi, scale=1.0, limiter=False):
+    """update_efield (plasma_pde/poissoneq.f90:565-680): returns (phi with its
+    ghost values set, E as a (nx+2, ny+2, nz+2, 3) Fortran array; ghost edges
+    and corners of E are zero here, undefined in the reference)."""
+    phi = np.array(phi, dtype=np.float64, order="F", copy=True)
+    nx, ny, nz = p.n
+    E = np.zeros((nx + 2, ny + 2, nz + 2, 3), order="F")
+    lib().orc_efield(p.ref, _ptr(phi), _ptr(E), float(scale), int(bool(limiter)))
+    return phi, E
 
 
 def restrict(fine):

@@ -188,3 +188,61 @@ def test_brute_force_assembly_detects_planted_mistakes(mutation):
     else:
         B = dense_from_faces(n, h, D, v, np.zeros(n), ("FLUX",) + codes[1:], 0.0)
     assert not np.allclose(A, B, rtol=1e-10, atol=1e-10)
+
+
+def _efield_problem(n, length, codes, faces=(0.0,) * 6):
+    return O.Problem(n, length, (1, 1, 1), codes, faces)
+
+
+def test_efield_of_linear_potentials_is_exact():
+    """update_efield (plasma_pde/poissoneq.f90:565-680) on phi = a + bx + cy
+    + dz: with DIRC faces carrying phi's wall values the ghost 2g - phi_1 is
+    the linear extrapolation, so E = -scale (b, c, d) in every cell; with
+    zero-gradient faces the boundary cells see half the slope."""
+    n, L = (7, 5, 4), (0.7, 0.4, 0.6)
+    h = [Li / ni for Li, ni in zip(L, n)]
+    a, b, c, d = 0.3, 2.0, -1.5, 0.75
+    x, y, z = [(np.arange(ni) + 0.5) * hi for ni, hi in zip(n, h)]
+    phi = a + b * x[:, None, None] + c * y[None, :, None] + d * z[None, None, :]
+
+    def wall(xv, yv, zv):
+        return a + b * xv + c * yv + d * zv
+
+    faces = [wall(0.0, y[:, None], z[None, :]), wall(L[0], y[:, None], z[None, :]),
+             wall(x[:, None], 0.0, z[None, :]), wall(x[:, None], L[1], z[None, :]),
+             wall(x[:, None], y[None, :], 0.0), wall(x[:, None], y[None, :], L[2])]
+    scale = 1.7
+    P = _efield_problem(n, L, ("DIRC",) * 6, faces)
+    _, E = O.efield(P, O.ghosted(phi), scale)
+    for q, slope in enumerate((b, c, d)):
+        assert np.allclose(E[1:-1, 1:-1, 1:-1, q], -scale * slope, rtol=1e-12, atol=0)
+    P = _efield_problem(n, L, ("ZGRD",) * 6)
+    _, E = O.efield(P, O.ghosted(phi), scale)
+    Ex = E[1:-1, 1:-1, 1:-1, 0]
+    assert np.allclose(Ex[1:-1], -scale * b, rtol=1e-12)
+    assert np.allclose(Ex[0], -scale * b / 2, rtol=1e-12) and np.allclose(Ex[-1], -scale * b / 2, rtol=1e-12)
+    # ghost faces of E copy the first interior cell
+    assert np.array_equal(E[0, 1:-1, 1:-1, 0], E[1, 1:-1, 1:-1, 0])
+    assert np.array_equal(E[1:-1, -1, 1:-1, 1], E[1:-1, -2, 1:-1, 1])
+    assert np.array_equal(E[1:-1, 1:-1, 0, 2], E[1:-1, 1:-1, 1, 2])
+
+
+def test_efield_central_difference_is_exact_for_quadratics_and_limited_at_kinks():
+    n, L = (9, 3, 3), (0.9, 0.3, 0.3)
+    h = L[0] / n[0]
+    x = (np.arange(n[0]) + 0.5) * h
+    P = _efield_problem(n, L, ("ZGRD",) * 6)
+    phi = np.broadcast_to((x ** 2)[:, None, None], n).copy()
+    _, E = O.efield(P, O.ghosted(phi), 1.0)
+    assert np.allclose(E[2:-2, 1, 1, 0], -2.0 * x[1:-1], rtol=1e-12)
+    # minmod: r = (x_i^2 - x_{i-1}^2) / (x_{i+1}^2 - x_i^2) = (2x_i - h) / (2x_i + h)
+    _, El = O.efield(P, O.ghosted(phi), 1.0, limiter=True)
+    r = (2 * x[1:-1] - h) / (2 * x[1:-1] + h)
+    assert np.allclose(El[2:-2, 1, 1, 0], -2.0 * x[1:-1] * r, rtol=1e-12)
+    # an asymmetric kink at cell 4: slopes -1 and 2, central difference 0.5,
+    # ratio of the one-sided differences -1/2 < 0, so the limiter zeroes E
+    x0 = x[4]
+    phi = np.broadcast_to(np.maximum(-(x - x0), 2 * (x - x0))[:, None, None], n).copy()
+    _, E = O.efield(P, O.ghosted(phi), 1.0)
+    _, El = O.efield(P, O.ghosted(phi), 1.0, limiter=True)
+    assert np.isclose(E[5, 1, 1, 0], -0.5, rtol=1e-12) and El[5, 1, 1, 0] == 0.0
