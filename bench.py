@@ -156,6 +156,66 @@ def calibration_copy():
     return gbps
 
 
+def step_kernels(n, reps=20):
+    """The per-step work either side of the solves, on device arrays of the
+    same grid: update_efield (mps_efield) and the transport assembly of the
+    ion_elec chemistry (electron and energy *_update_transport, Poisson's,
+    update_electrontemp). Mean wall time per call over `reps` calls (each
+    call ends with a stream synchronisation, so this is an upper bound on
+    the kernel time) and the algorithmic GB/s it implies: every array the
+    kernel reads or writes once, ghosted."""
+    import torch
+
+    from mps3d_amd import FieldSolver
+    from mps3d_amd.chemistry import etempmin_default, ion_elec
+
+    s = FieldSolver(n, 0.02)
+    s.set_bc(("ZGRD", "ZGRD", "ZGRD", "ZGRD", "DIRC", "DIRC"), (0.0, 0.0, 0.0, 0.0, 0.0, 1000.0))
+    ch = ion_elec()
+    s.set_chemistry(ch)
+    ng = int(np.prod(s.shape))
+    g = torch.Generator(device="cuda").manual_seed(20261202)
+
+    def field(m, lo, hi):
+        return torch.rand(m * ng, dtype=torch.float64, device="cuda", generator=g) * (hi - lo) + lo
+
+    phi = field(1, 0.0, 1000.0)
+    E = torch.empty(3 * ng, dtype=torch.float64, device="cuda")
+    te = field(1, 5e3, 6e4)
+    nd = torch.cat([field(1, 1e21, 3e22), field(1, 1e14, 1e16), field(1, 1e14, 1e16)])
+    soln = field(1, 0.0, 1.0)
+    eps = field(1, 1e-3, 1.0)
+    out = {k: torch.empty(m * ng, dtype=torch.float64, device="cuda")
+           for k, m in (("vel", 3), ("dcoeff", 1), ("reac", 1), ("source", 1), ("prod", 1))}
+    sc = (1.0, 1.0, 1.0, 1.0)
+    calls = {
+        # name: (call, algorithmic bytes per ghosted cell)
+        "efield": (lambda: s.efield(phi, 50.0, False, out=E), 8 * (1 + 3)),
+        "species_transport": (lambda: s.species_transport(2, E, te, nd, sc, soln=soln, dt=1e-3, out=out),
+                              8 * (3 + 1 + 2 + 1 + 3 + 1 + 1 + 1 + 1)),
+        "eenergy_transport": (lambda: s.eenergy_transport(E, te, nd, sc, lscale=1.0, soln=soln, dt=1e-3, out=out),
+                              8 * (3 + 1 + 2 + 1 + 3 + 1 + 1 + 1)),
+        "poisson_transport": (lambda: s.poisson_transport(nd[ng:2 * ng], nd[2 * ng:], sc, soln=soln, dt=1e-3,
+                                                          out=out), 8 * (1 + 1 + 1 + 3 + 1 + 1 + 1)),
+        "electron_temp": (lambda: s.electron_temp(eps, nd[ng:2 * ng], 1.0, etempmin_default(), te), 8 * 3),
+    }
+    res = {}
+    for name, (fn, bpc) in calls.items():
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        us = (time.perf_counter() - t0) * 1e6 / reps
+        res[name] = {"us_per_call": round(us, 1), "alg_bytes": bpc * ng,
+                     "alg_GBps": round(bpc * ng / (us * 1e-6) / 1e9, 1)}
+    s.close()
+    del phi, E, te, nd, soln, eps, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def measured_traffic(kernel, n):
     """HBM bytes per launch of the roofline kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, if there is
@@ -270,6 +330,7 @@ def main():
                      "copy_GBps": round(copy_gbps, 1), "frac_of_copy": round(kc["GBps"] / copy_gbps, 4)},
         "kernels": {"fp32dc": dc["kernels"], "fp64": f64["kernels"]},
     }
+    line["step_kernels"] = step_kernels(args.n)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.n, f64["cycles"])
     print(json.dumps(line), flush=True)

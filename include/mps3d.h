@@ -206,6 +206,73 @@ int mps_fill_ghosts(mps_ctx *ctx, double *soln);
  * are not written, as in the reference); scale = potscale / lscale. */
 int mps_efield(mps_ctx *ctx, double *phi, double *efield, double scale, int limiter);
 
+/* ---- Transport assembly (the step before and after each solve) ----------
+ * Per-cell coefficients and sources of the plasma equations, computed on the
+ * device in the ghosted layout mps_set_coeffs takes, for chemistries given
+ * as tables: Arrhenius rates in T_e and constant mobility and diffusion per
+ * species, the form of plasma_pde/chemistry/ion_elec_chem.f90. Inputs and
+ * outputs are host or device arrays (host arrays are staged for the call);
+ * numden is the reference's numden(0:nx+1,0:ny+1,0:nz+1,nspecies), efield
+ * and vel three ghosted components one after the other. scaling[4] =
+ * {vel, dcoeff, reac, source} scalings of the equation module
+ * (edeneq.f90:82-85: tscale/lscale, tscale/lscale^2, tscale,
+ * tscale/varscale); with timederiv != 0, source += soln/dt over the whole
+ * ghosted array, as the reference does. Output pointers are nullable. The
+ * reference's physical constants (plasma_pde/fundconstants.f90:3-9) apply. */
+typedef struct {
+    int nspecies, nreac;
+    int especnum, bgspecnum;        /* 1-based species numbers */
+    int ionspecmin, ionspecmax;     /* 1-based; ionspecmax < ionspecmin: no ions */
+    const int *reactants;           /* reactants(nreac, nspecies), Fortran order */
+    const int *products;            /* products(nreac, nspecies), Fortran order */
+    const double *k_arrh;           /* k_arrh(3, nreac): A, alpha, Ea (K) */
+    const double *elecenergy;       /* (nreac) electron energy change, eV */
+    const double *mobility;         /* (nspecies) signed mobility */
+    const double *dcoeff;           /* (nspecies) diffusion coefficient */
+    const double *molmass;          /* (nspecies) kg */
+    const double *charge;           /* (nspecies) charge number */
+} mps_chem;
+
+/* initializechemistry (ion_elec_chem.f90:104-109, with the tables of
+ * assignreactions and setspecparams, :30-102): copies the tables to the
+ * device (at most 16 species, 64 reactions, stoichiometry 0..16). */
+int mps_set_chemistry(mps_ctx *ctx, const mps_chem *chem);
+
+/* eden_update_transport (edeneq.f90:360-444) for specnum = especnum, with
+ * prod = elecprod (interior, unscaled production), and ions_update_transport
+ * (ionseq.f90:478-558) for an ion's specnum (prod NULL): vel = mu E, dcoeff =
+ * D, source = getspecproduction (ion_elec_chem.f90:111-156), all scaled;
+ * reac = 0. Pg is accepted for the reference's argument list; the table
+ * chemistries do not use it. */
+int mps_species_transport(mps_ctx *ctx, int specnum, double dt, int timederiv, const double *efield,
+                          const double *electemp, const double *numden, double Tg, double Pg,
+                          const double scaling[4], const double *soln, double *vel, double *dcoeff,
+                          double *reac, double *source, double *prod);
+
+/* eenergy_update_transport (eenergyeq.f90:679-774): vel = 5/3 mu_e E,
+ * dcoeff = 5/3 D_e, source = Joule heating (jouleheatingterm, upwinded face
+ * fluxes per axis clipped at zero, :394-555) - elastic loss (elastic_colterm,
+ * :557-612) + inelastic term (inelastic_colterm, :614-677), all scaled.
+ * lscale turns the context's cell widths into the Joule term's dx. */
+int mps_eenergy_transport(mps_ctx *ctx, double dt, int timederiv, double lscale, const double *efield,
+                          const double *electemp, const double *numden, double Tg, double Pg,
+                          const double scaling[4], const double *soln, double *vel, double *dcoeff,
+                          double *reac, double *source);
+
+/* poisson_update_transport (poissoneq.f90:335-385): source = e/eps0
+ * (sum_ions Z n - n_e), dcoeff = 1, both scaled; vel = reac = 0. iondens
+ * holds the ion species ionspecmin..ionspecmax one after the other. */
+int mps_poisson_transport(mps_ctx *ctx, double dt, int timederiv, const double *elecden,
+                          const double *iondens, const double scaling[4], const double *soln,
+                          double *vel, double *dcoeff, double *reac, double *source);
+
+/* update_electrontemp (eenergyeq.f90:842-877): electemp = eenergysoln *
+ * eenergyscale / (3/2 n_e k_B) on the interior; below etempmin the
+ * temperature is etempmin and eenergysoln (in/out) is reset to match, in the
+ * same cell (the reference's reset lands one cell off, eenergyeq.f90:868). */
+int mps_electron_temp(mps_ctx *ctx, double *eenergysoln, const double *elecden, double eenergyscale,
+                      double etempmin, double *electemp);
+
 /* performgmres (gmres.f90:197-434) on the finest level: restarted GMRES(m)
  * for A x = b (b interior, x ghosted in/out).
  *   precond = 0: the reference's method, left-preconditioned by one

@@ -28,9 +28,14 @@ EXPORTS = (
     "mps_default_opts", "mps_create", "mps_destroy", "mps_set_bc", "mps_set_bc_str", "mps_set_coeffs",
     "mps_vcycle", "mps_solve", "mps_apply", "mps_apply_transpose", "mps_bvec", "mps_smooth",
     "mps_fill_ghosts", "mps_gmres", "mps_solve_transpose", "mps_last_timing", "mps_stream", "mps_solve_rhs",
-    "mps_efield",
+    "mps_efield", "mps_set_chemistry", "mps_species_transport", "mps_eenergy_transport",
+    "mps_poisson_transport", "mps_electron_temp",
     "mps_error_string", "mps_last_error", "mps_time_kernel", "mps_grad_velocity",
 )
+
+
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
 
 
 class Grid(C.Structure):
@@ -39,14 +44,20 @@ class Grid(C.Structure):
                 ("px", C.c_int), ("py", C.c_int), ("pz", C.c_int)]
 
 
+class Chem(C.Structure):
+    """mps_chem: chemistry tables, Fortran-order arrays, 1-based species."""
+    _fields_ = [("nspecies", C.c_int), ("nreac", C.c_int), ("especnum", C.c_int), ("bgspecnum", C.c_int),
+                ("ionspecmin", C.c_int), ("ionspecmax", C.c_int),
+                ("reactants", _vp), ("products", _vp), ("k_arrh", _vp), ("elecenergy", _vp),
+                ("mobility", _vp), ("dcoeff", _vp), ("molmass", _vp), ("charge", _vp)]
+
+
 class Opts(C.Structure):
     _fields_ = [("hierarchy", C.c_int), ("precision", C.c_int), ("flux", C.c_int),
                 ("nu1", C.c_int), ("nu2", C.c_int), ("max_cycles", C.c_int),
                 ("tol", C.c_double), ("tol_kind", C.c_int), ("arith", C.c_int), ("start", C.c_int)]
 
 
-_dp = C.POINTER(C.c_double)
-_vp = C.c_void_p
 _lib = None
 
 
@@ -79,6 +90,13 @@ def load():
     L.mps_smooth.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
     L.mps_fill_ghosts.argtypes = [_vp, _vp]
     L.mps_efield.argtypes = [_vp, _vp, _vp, C.c_double, C.c_int]
+    L.mps_set_chemistry.argtypes = [_vp, _vp]
+    L.mps_species_transport.argtypes = [_vp, C.c_int, C.c_double, C.c_int, _vp, _vp, _vp, C.c_double, C.c_double,
+                                        _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mps_eenergy_transport.argtypes = [_vp, C.c_double, C.c_int, C.c_double, _vp, _vp, _vp, C.c_double,
+                                        C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mps_poisson_transport.argtypes = [_vp, C.c_double, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mps_electron_temp.argtypes = [_vp, _vp, _vp, C.c_double, C.c_double, _vp]
     L.mps_gmres.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), _dp]
     L.mps_solve_transpose.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_int), _dp]
     L.mps_last_timing.argtypes = [_vp, _dp, C.POINTER(C.c_int)]

@@ -8,6 +8,7 @@
 #include "mps3d_kernels.h"
 #include "mps3d_sweep.h"
 #include "mps3d_lds_cycle.h"
+#include "mps3d_transport.h"
 
 #include <hip/hip_runtime.h>
 
@@ -193,6 +194,8 @@ struct mps_ctx {
     bool fused_rr = true;                // MPS3D_NO_FUSED_RESTRICT=1: residual and restriction apart
     StepGraph step[2];
     unsigned gen = 0;                    // bumped by set_bc / set_coeffs
+    ChemDev *chem = nullptr;             // chemistry tables (mps_set_chemistry)
+    ChemDev chem_host{};
     bool mg_ok = true;                   // a multigrid hierarchy exists
     std::string mg_error;                // why not, when it does not
     GmresBufs gm;                        // FGMRES work vectors (mps_gmres)
@@ -1591,6 +1594,7 @@ int mps_destroy(mps_ctx *c)
     dfree(c->fmg_r);
     dfree(c->fmg_e);
     free_gmres(c->gm);
+    dfree(c->chem);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev_in) (void)hipEventDestroy(c->ev_in);
@@ -1990,6 +1994,286 @@ int mps_efield(mps_ctx *c, double *phi, double *efield, double scale, int limite
             dfree(E);
         }
         HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
+}
+
+// ------------------------------------------------------------------------
+// transport assembly (plasma_pde/*_update_transport, update_electrontemp)
+// ------------------------------------------------------------------------
+
+// Host arrays are staged through device buffers for one call: inputs are
+// copied in, outputs copied back when the call ends; device arrays are used
+// in place.
+struct Staging {
+    mps_ctx *c;
+    std::vector<void *> owned;
+    std::vector<std::pair<double *, const double *>> back;   // host dst, device src
+    std::vector<size_t> back_n;
+    explicit Staging(mps_ctx *ctx) : c(ctx) {}
+    Staging(const Staging &) = delete;
+    Staging &operator=(const Staging &) = delete;
+    ~Staging()
+    {
+        for (void *p : owned) dfree(p);
+    }
+    const double *in(const double *p, size_t n)
+    {
+        if (!p || is_device_ptr(p)) return p;
+        double *d = dalloc<double>(n);
+        owned.push_back(d);
+        HIPCHK(hipMemcpyAsync(d, p, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        return d;
+    }
+    // copy_in: the output's previous contents are kept where the kernel
+    // does not write (in/out arrays, unwritten ghost cells)
+    double *out(double *p, size_t n, bool copy_in)
+    {
+        if (!p || is_device_ptr(p)) return p;
+        double *d = dalloc<double>(n);
+        owned.push_back(d);
+        if (copy_in) HIPCHK(hipMemcpyAsync(d, p, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        back.push_back({p, d});
+        back_n.push_back(n);
+        return d;
+    }
+    void finish()
+    {
+        for (size_t q = 0; q < back.size(); ++q)
+            HIPCHK(hipMemcpyAsync(back[q].first, back[q].second, back_n[q] * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+};
+
+dim3 ghosted_grid(const Geom &g)
+{
+    return dim3((unsigned)(((g.nx + 2) * (g.ny + 2) + 255) / 256), (unsigned)(g.nz + 2));
+}
+
+int transport_guard(mps_ctx *c, const double scaling[4], int timederiv, const double *soln, double dt)
+{
+    int st = guard(c, false);
+    if (st) return st;
+    if (!c->chem) return fail(MPS_ERR_STATE, "set the chemistry first (mps_set_chemistry)");
+    if (!scaling) return fail(MPS_ERR_ARG, "null scaling");
+    if (timederiv && (!soln || dt == 0.0)) return fail(MPS_ERR_ARG, "the time term needs soln and dt != 0");
+    return MPS_OK;
+}
+
+TransportIn transport_in(const double scaling[4], int timederiv, double dt, double Tg, double Pg)
+{
+    TransportIn in{};
+    in.vs = scaling[0];
+    in.ds = scaling[1];
+    in.rs = scaling[2];
+    in.ss = scaling[3];
+    in.timederiv = timederiv ? 1 : 0;
+    in.dt = dt;
+    in.Tg = Tg;
+    in.Pg = Pg;
+    return in;
+}
+
+extern "C" int mps_set_chemistry(mps_ctx *c, const mps_chem *chem)
+{
+    int st = guard(c, false);
+    if (st) return st;
+    if (!chem) return fail(MPS_ERR_ARG, "null chemistry");
+    const int ns = chem->nspecies, nr = chem->nreac;
+    if (ns < 1 || ns > kChemMaxSpecies) return fail(MPS_ERR_ARG, "nspecies must be 1..%d", kChemMaxSpecies);
+    if (nr < 0 || nr > kChemMaxReac) return fail(MPS_ERR_ARG, "nreac must be 0..%d", kChemMaxReac);
+    auto sp_ok = [&](int s) { return s >= 1 && s <= ns; };
+    if (!sp_ok(chem->especnum) || !sp_ok(chem->bgspecnum)) return fail(MPS_ERR_ARG, "especnum/bgspecnum out of range");
+    const bool no_ions = chem->ionspecmax < chem->ionspecmin;
+    if (!no_ions && (!sp_ok(chem->ionspecmin) || !sp_ok(chem->ionspecmax)))
+        return fail(MPS_ERR_ARG, "ion species range out of range");
+    if (!chem->mobility || !chem->dcoeff || !chem->molmass || !chem->charge)
+        return fail(MPS_ERR_ARG, "null species table");
+    if (nr > 0 && (!chem->reactants || !chem->products || !chem->k_arrh || !chem->elecenergy))
+        return fail(MPS_ERR_ARG, "null reaction table");
+    ChemDev h{};
+    h.ns = ns;
+    h.nr = nr;
+    h.e = chem->especnum - 1;
+    h.bg = chem->bgspecnum - 1;
+    h.imin = no_ions ? 0 : chem->ionspecmin - 1;
+    h.imax = no_ions ? -1 : chem->ionspecmax - 1;
+    for (int r = 0; r < nr; ++r) {
+        for (int j = 0; j < ns; ++j) {
+            // reactants(nreac, nspecies) in Fortran order
+            const int a = chem->reactants[r + (size_t)nr * j], b = chem->products[r + (size_t)nr * j];
+            if (a < 0 || b < 0 || a > 16 || b > 16) return fail(MPS_ERR_ARG, "stoichiometry must be 0..16");
+            h.react[r][j] = a;
+            h.nu[r][j] = b - a;
+        }
+        h.A[r] = chem->k_arrh[3 * r];
+        h.alpha[r] = chem->k_arrh[3 * r + 1];
+        h.Ea[r] = chem->k_arrh[3 * r + 2];
+        h.eel[r] = chem->elecenergy[r];
+    }
+    for (int j = 0; j < ns; ++j) {
+        h.mob[j] = chem->mobility[j];
+        h.dco[j] = chem->dcoeff[j];
+        h.mass[j] = chem->molmass[j];
+        h.charge[j] = chem->charge[j];
+    }
+    try {
+        if (!c->chem) c->chem = dalloc<ChemDev>(1);
+        c->chem_host = h;
+        HIPCHK(hipMemcpyAsync(c->chem, &c->chem_host, sizeof(ChemDev), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
+}
+
+extern "C" int mps_species_transport(mps_ctx *c, int specnum, double dt, int timederiv, const double *efield,
+                                     const double *electemp, const double *numden, double Tg, double Pg,
+                                     const double scaling[4], const double *soln, double *vel, double *dcoeff,
+                                     double *reac, double *source, double *prod)
+{
+    int st = transport_guard(c, scaling, timederiv, soln, dt);
+    if (st) return st;
+    if (specnum < 1 || specnum > c->chem_host.ns) return fail(MPS_ERR_ARG, "specnum out of range");
+    if (!efield || !electemp || !numden) return fail(MPS_ERR_ARG, "null input");
+    try {
+        const Geom &g = c->geom[0];
+        const size_t ng = (size_t)g.ng();
+        Staging S(c);
+        TransportIn in = transport_in(scaling, timederiv, dt, Tg, Pg);
+        const double *E = S.in(efield, 3 * ng);
+        in.ex = E;
+        in.ey = E + ng;
+        in.ez = E + 2 * ng;
+        in.te = S.in(electemp, ng);
+        in.nd = S.in(numden, (size_t)c->chem_host.ns * ng);
+        in.soln = timederiv ? S.in(soln, ng) : nullptr;
+        TransportOut out{};
+        double *V = S.out(vel, 3 * ng, false);
+        if (V) {
+            out.vx = V;
+            out.vy = V + ng;
+            out.vz = V + 2 * ng;
+        }
+        out.dcoeff = S.out(dcoeff, ng, false);
+        out.reac = S.out(reac, ng, false);
+        out.source = S.out(source, ng, false);
+        out.prod = S.out(prod, ng, true);   // interior only
+        Lev<double> v = view(c, c->L64, 0);
+        hipLaunchKernelGGL(k_transport<0>, ghosted_grid(g), dim3(256), 0, c->stream, v, (const ChemDev *)c->chem,
+                           specnum - 1, in, out);
+        HIPCHK(hipGetLastError());
+        S.finish();
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
+}
+
+extern "C" int mps_eenergy_transport(mps_ctx *c, double dt, int timederiv, double lscale, const double *efield,
+                                     const double *electemp, const double *numden, double Tg, double Pg,
+                                     const double scaling[4], const double *soln, double *vel, double *dcoeff,
+                                     double *reac, double *source)
+{
+    int st = transport_guard(c, scaling, timederiv, soln, dt);
+    if (st) return st;
+    if (!efield || !electemp || !numden) return fail(MPS_ERR_ARG, "null input");
+    if (!(lscale > 0.0)) return fail(MPS_ERR_ARG, "lscale must be > 0");
+    try {
+        const Geom &g = c->geom[0];
+        const size_t ng = (size_t)g.ng();
+        Staging S(c);
+        TransportIn in = transport_in(scaling, timederiv, dt, Tg, Pg);
+        const double *E = S.in(efield, 3 * ng);
+        in.ex = E;
+        in.ey = E + ng;
+        in.ez = E + 2 * ng;
+        in.te = S.in(electemp, ng);
+        in.nd = S.in(numden, (size_t)c->chem_host.ns * ng);
+        in.soln = timederiv ? S.in(soln, ng) : nullptr;
+        in.dxl[0] = g.dx * lscale;
+        in.dxl[1] = g.dy * lscale;
+        in.dxl[2] = g.dz * lscale;
+        TransportOut out{};
+        double *V = S.out(vel, 3 * ng, false);
+        if (V) {
+            out.vx = V;
+            out.vy = V + ng;
+            out.vz = V + 2 * ng;
+        }
+        out.dcoeff = S.out(dcoeff, ng, false);
+        out.reac = S.out(reac, ng, false);
+        out.source = S.out(source, ng, false);
+        Lev<double> v = view(c, c->L64, 0);
+        hipLaunchKernelGGL(k_transport<1>, ghosted_grid(g), dim3(256), 0, c->stream, v, (const ChemDev *)c->chem,
+                           0, in, out);
+        HIPCHK(hipGetLastError());
+        S.finish();
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
+}
+
+extern "C" int mps_poisson_transport(mps_ctx *c, double dt, int timederiv, const double *elecden,
+                                     const double *iondens, const double scaling[4], const double *soln,
+                                     double *vel, double *dcoeff, double *reac, double *source)
+{
+    int st = transport_guard(c, scaling, timederiv, soln, dt);
+    if (st) return st;
+    const int nions = c->chem_host.imax - c->chem_host.imin + 1;
+    if (!elecden || (nions > 0 && !iondens)) return fail(MPS_ERR_ARG, "null input");
+    try {
+        const Geom &g = c->geom[0];
+        const size_t ng = (size_t)g.ng();
+        Staging S(c);
+        TransportIn in = transport_in(scaling, timederiv, dt, 0.0, 0.0);
+        const double *ne = S.in(elecden, ng);
+        const double *ions = nions > 0 ? S.in(iondens, (size_t)nions * ng) : nullptr;
+        in.soln = timederiv ? S.in(soln, ng) : nullptr;
+        TransportOut out{};
+        double *V = S.out(vel, 3 * ng, false);
+        if (V) {
+            out.vx = V;
+            out.vy = V + ng;
+            out.vz = V + 2 * ng;
+        }
+        out.dcoeff = S.out(dcoeff, ng, false);
+        out.reac = S.out(reac, ng, false);
+        out.source = S.out(source, ng, false);
+        Lev<double> v = view(c, c->L64, 0);
+        hipLaunchKernelGGL(k_poisson_transport, ghosted_grid(g), dim3(256), 0, c->stream, v,
+                           (const ChemDev *)c->chem, ne, ions, in, out);
+        HIPCHK(hipGetLastError());
+        S.finish();
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
+}
+
+extern "C" int mps_electron_temp(mps_ctx *c, double *eenergysoln, const double *elecden, double eenergyscale,
+                                 double etempmin, double *electemp)
+{
+    int st = guard(c, false);
+    if (st) return st;
+    if (!eenergysoln || !elecden || !electemp) return fail(MPS_ERR_ARG, "null argument");
+    try {
+        const Geom &g = c->geom[0];
+        const size_t ng = (size_t)g.ng();
+        Staging S(c);
+        const double *ne = S.in(elecden, ng);
+        double *eps = S.out(eenergysoln, ng, true);
+        double *te = S.out(electemp, ng, true);   // ghosts are not written
+        Lev<double> v = view(c, c->L64, 0);
+        hipLaunchKernelGGL(k_electron_temp, cells_grid(g), block(), 0, c->stream, v, eps, ne, eenergyscale, etempmin,
+                           te);
+        HIPCHK(hipGetLastError());
+        S.finish();
     } catch (const HipError &e) {
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
     }

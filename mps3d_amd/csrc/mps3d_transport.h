@@ -1,0 +1,254 @@
+// Transport assembly of the plasma equations on the device: the per-cell
+// coefficients and sources each implicit solve of a time step takes
+// (plasma_pde/{edeneq,ionseq,eenergyeq,poissoneq}.f90, *_update_transport,
+// update_electrontemp), for chemistries given as tables (Arrhenius rates in
+// T_e, constant mobility and diffusion per species, as
+// plasma_pde/chemistry/ion_elec_chem.f90).
+//
+// Every kernel is a one-pass, HBM-bound stream over the ghosted grid (the
+// Joule term reads the six neighbours of n_e and E as well, which the L2
+// serves): one thread per cell, lanes along x, blockIdx.y walking the
+// ghosted z planes. Arithmetic follows the reference's evaluation order
+// term by term, and the library is built without FMA contraction, so every
+// output that does not pass through exp/pow matches a sequential FP64
+// evaluation bit for bit.
+#pragma once
+
+#include "mps3d_kernels.h"
+
+namespace mps {
+
+constexpr int kChemMaxSpecies = 16;
+constexpr int kChemMaxReac = 64;
+
+// the reference's physical constants (plasma_pde/fundconstants.f90:3-9)
+constexpr double kEcharge = 1.602176e-19;
+constexpr double kEps0 = 8.854188e-12;
+constexpr double kBoltz = 1.380650e-23;
+
+// One chemistry (chem_module's tables) in device memory. Species and
+// reaction indices are 0-based here.
+struct ChemDev {
+    int ns, nr;
+    int e, bg;          // electron and background-gas species
+    int imin, imax;     // ion species range
+    int nu[kChemMaxReac][kChemMaxSpecies];      // products - reactants
+    int react[kChemMaxReac][kChemMaxSpecies];   // reactant moles
+    double A[kChemMaxReac], alpha[kChemMaxReac], Ea[kChemMaxReac];
+    double eel[kChemMaxReac];                   // electron energy change, eV
+    double mob[kChemMaxSpecies], dco[kChemMaxSpecies];
+    double mass[kChemMaxSpecies], charge[kChemMaxSpecies];
+};
+
+// x**m for an integer m >= 0 by repeated squaring (the integer power of
+// Fortran's real**integer: x**0 = 1, x**1 = x, x**2 = x*x, x**3 = x*(x*x))
+__host__ __device__ __forceinline__ double powi_sq(double x, int m)
+{
+    double r = (m & 1) ? x : 1.0;
+    while (m >>= 1) {
+        x = x * x;
+        if (m & 1) r = r * x;
+    }
+    return r;
+}
+
+// getarrhrate (ion_elec_chem.f90:260-274): A T^alpha exp(-Ea/T)
+__device__ __forceinline__ double arrh_rate(const ChemDev &C, int r, double T)
+{
+    return C.A[r] * pow(T, C.alpha[r]) * exp(-C.Ea[r] / T);
+}
+
+// product of the reactant densities of reaction r at cell c
+__device__ __forceinline__ double reactant_product(const ChemDev &C, int r, const double *__restrict__ nd,
+                                                   long long ng, long long c)
+{
+    double m = 1.0;
+    for (int j = 0; j < C.ns; ++j) {
+        const int k = C.react[r][j];
+        if (k) m = m * powi_sq(nd[(long long)j * ng + c], k);
+    }
+    return m;
+}
+
+// findjheating_eachdir (eenergyeq.f90:461-555) along one axis with stride st:
+// upwinded face fluxes mu n E - D grad n on both faces of the cell, and
+// -e <E> <Gamma>, clipped at zero
+__device__ __forceinline__ double joule_dir(const double *__restrict__ ne, const double *__restrict__ Eq,
+                                            long long c, long long st, double mu, double D, double dx)
+{
+    const double nm = ne[c - st], n0 = ne[c], np = ne[c + st];
+    const double Em = Eq[c - st], E0 = Eq[c], Ep = Eq[c + st];
+    double grad = (n0 - nm) / dx;
+    const double el = (E0 + Em) * 0.5;
+    double den = (mu * el > 0.0) ? nm : n0;
+    const double gl = mu * den * el - D * grad;
+    grad = (np - n0) / dx;
+    const double er = (E0 + Ep) * 0.5;
+    den = (mu * er > 0.0) ? n0 : np;
+    const double gr = mu * den * er - D * grad;
+    double j = -(kEcharge * 0.5 * (el + er) * 0.5 * (gl + gr));
+    if (j < 0.0) j = 0.0;
+    return j;
+}
+
+// ghosted cell of this thread: (i, j) from the flattened x-y plane, k from
+// blockIdx.y; interior = all three indices in 1..n
+struct GCell {
+    long long c;
+    bool interior;
+};
+
+template <typename T>
+__device__ __forceinline__ bool ghost_cell_of(const Lev<T> &L, GCell &g)
+{
+    const int px = L.nx + 2, py = L.ny + 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= px * py) return false;
+    const int j = t / px, i = t - j * px, k = blockIdx.y;
+    g.c = (long long)t + L.sz * k;
+    g.interior = i >= 1 && i <= L.nx && j >= 1 && j <= L.ny && k >= 1 && k <= L.nz;
+    return true;
+}
+
+struct TransportOut {
+    double *vx, *vy, *vz, *dcoeff, *reac, *source, *prod;   // each nullable
+};
+
+struct TransportIn {
+    const double *ex, *ey, *ez;   // cell-centre field
+    const double *te;             // electron temperature, K
+    const double *nd;             // ns ghosted density fields
+    const double *soln;           // the equation's current solution (time term)
+    double Tg, Pg;
+    double dt;
+    int timederiv;
+    double vs, ds, rs, ss;        // vel, dcoeff, reac, source scalings
+    double dxl[3];                // g_d * lscale per axis (Joule term)
+};
+
+// eden_update_transport (edeneq.f90:360-444) and ions_update_transport
+// (ionseq.f90:478-558), KIND 0, for species sp; eenergy_update_transport
+// (eenergyeq.f90:679-774) with jouleheatingterm, elastic_colterm and
+// inelastic_colterm (:394-677), KIND 1.
+template <int KIND>
+__global__ void __launch_bounds__(256) k_transport(Lev<double> L, const ChemDev *__restrict__ Cp, int sp,
+                                                   TransportIn in, TransportOut out)
+{
+    GCell g;
+    if (!ghost_cell_of(L, g)) return;
+    const ChemDev &C = *Cp;
+    const long long c = g.c;
+    const long long ng = L.sz * (L.nz + 2);
+    double vx = 0.0, vy = 0.0, vz = 0.0, dc = 0.0, src = 0.0;
+    if (g.interior) {
+        const double Ex = in.ex[c], Ey = in.ey[c], Ez = in.ez[c];
+        // efield_mag is not used by table chemistries (ion_elec_chem's
+        // mobility, diffusion and Arrhenius rates do not depend on it)
+        const double Te = in.te[c];
+        if (KIND == 0) {
+            const double mu = C.mob[sp];
+            vx = mu * Ex;
+            vy = mu * Ey;
+            vz = mu * Ez;
+            dc = C.dco[sp];
+            // getspecproduction (ion_elec_chem.f90:111-156)
+            double prod = 0.0;
+            for (int r = 0; r < C.nr; ++r) {
+                const int nm = C.nu[r][sp];
+                if (nm == 0) continue;
+                const double k = arrh_rate(C, r, Te);
+                prod = prod + (double)nm * k * reactant_product(C, r, in.nd, ng, c);
+            }
+            src = 0.0 + prod;
+            if (out.prod) out.prod[c] = src;
+        } else {
+            const int e = C.e;
+            const double mu = C.mob[e], D = C.dco[e];
+            const double five_by_three = 5.0 / 3.0;
+            vx = five_by_three * mu * Ex;
+            vy = five_by_three * mu * Ey;
+            vz = five_by_three * mu * Ez;
+            dc = five_by_three * D;
+            const double *ne = in.nd + (long long)e * ng;
+            const double jh = joule_dir(ne, in.ex, c, 1, mu, D, in.dxl[0]) +
+                              joule_dir(ne, in.ey, c, L.sy, mu, D, in.dxl[1]) +
+                              joule_dir(ne, in.ez, c, L.sz, mu, D, in.dxl[2]);
+            // elastic_colterm: 3 k_B n_e (T_e - T_g) (m_e / m_bg) nu, with
+            // getelectroncollisionfrequency nu = -e / m_e / mu_e
+            const double me = C.mass[e], mbg = C.mass[C.bg];
+            const double nu = -(kEcharge / me / mu);
+            const double el = 3.0 * kBoltz * ne[c] * (Te - in.Tg) * (me / mbg) * nu;
+            // inelastic_colterm with getelectroninelasticterm
+            double inel = 0.0;
+            for (int r = 0; r < C.nr; ++r) {
+                if (C.eel[r] == 0.0) continue;
+                const double k = arrh_rate(C, r, Te);
+                inel = inel + k * reactant_product(C, r, in.nd, ng, c) * C.eel[r];
+            }
+            src = jh - el + inel * kEcharge;
+        }
+    }
+    if (out.vx) out.vx[c] = vx * in.vs;
+    if (out.vy) out.vy[c] = vy * in.vs;
+    if (out.vz) out.vz[c] = vz * in.vs;
+    if (out.dcoeff) out.dcoeff[c] = dc * in.ds;
+    if (out.reac) out.reac[c] = 0.0 * in.rs;
+    if (out.source) {
+        double s = src * in.ss;
+        if (in.timederiv) s = s + in.soln[c] / in.dt;
+        out.source[c] = s;
+    }
+}
+
+// poisson_update_transport (poissoneq.f90:335-385): source = e/eps0
+// (sum_i Z_i n_i - n_e) scaled, dcoeff = 1 scaled, vel = reac = 0
+__global__ void __launch_bounds__(256) k_poisson_transport(Lev<double> L, const ChemDev *__restrict__ Cp,
+                                                           const double *__restrict__ ne,
+                                                           const double *__restrict__ ions,
+                                                           TransportIn in, TransportOut out)
+{
+    GCell g;
+    if (!ghost_cell_of(L, g)) return;
+    const ChemDev &C = *Cp;
+    const long long c = g.c;
+    const long long ng = L.sz * (L.nz + 2);
+    double src = 0.0;
+    if (g.interior) {
+        double tot = 0.0;
+        for (int s = C.imin; s <= C.imax; ++s) tot = tot + C.charge[s] * ions[(long long)(s - C.imin) * ng + c];
+        src = tot - ne[c];
+    }
+    src = src * kEcharge / kEps0;
+    if (out.vx) out.vx[c] = 0.0;
+    if (out.vy) out.vy[c] = 0.0;
+    if (out.vz) out.vz[c] = 0.0;
+    if (out.dcoeff) out.dcoeff[c] = 1.0 * in.ds;
+    if (out.reac) out.reac[c] = 0.0;
+    if (out.source) {
+        double s = src * in.ss;
+        if (in.timederiv) s = s + in.soln[c] / in.dt;
+        out.source[c] = s;
+    }
+}
+
+// update_electrontemp (eenergyeq.f90:842-877): T_e = n_eps scale /
+// (3/2 n_e k_B) on the interior, clamped below at etempmin, where the
+// energy density is reset to match the clamped temperature (in the cell
+// itself: the reference writes it one cell down-left of it, an index slip)
+__global__ void __launch_bounds__(256) k_electron_temp(Lev<double> L, double *__restrict__ eps,
+                                                       const double *__restrict__ ne, double escale, double tmin,
+                                                       double *__restrict__ te)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    const double three_by_two = 1.5;
+    double T = eps[c] * escale / (three_by_two * ne[c] * kBoltz);
+    if (T < tmin) {
+        T = tmin;
+        eps[c] = three_by_two * ne[c] * kBoltz * tmin / escale;
+    }
+    te[c] = T;
+}
+
+}  // namespace mps

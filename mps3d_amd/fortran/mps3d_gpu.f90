@@ -23,6 +23,12 @@ module mps3d_gpu
     integer(c_int), parameter :: MPS_ARITH_EXACT = 0, MPS_ARITH_FAST = 1
     integer(c_int), parameter :: MPS_START_GUESS = 0, MPS_START_FMG = 1
 
+    ! chemistry tables (mps_chem): the arrays of a chem_module, by address
+    type, bind(C) :: mps_chem
+        integer(c_int) :: nspecies, nreac, especnum, bgspecnum, ionspecmin, ionspecmax
+        type(c_ptr) :: reactants, products, k_arrh, elecenergy, mobility, dcoeff, molmass, charge
+    end type mps_chem
+
     type, bind(C) :: mps_grid
         integer(c_int) :: nx, ny, nz
         real(c_double) :: lenx, leny, lenz
@@ -95,6 +101,43 @@ module mps3d_gpu
             real(c_double), value :: scale
             integer(c_int), value :: limiter
         end function mps_efield
+
+        integer(c_int) function mps_set_chemistry(ctx, chem) bind(C, name='mps_set_chemistry')
+            import :: c_int, c_ptr, mps_chem
+            type(c_ptr), value :: ctx
+            type(mps_chem), intent(in) :: chem
+        end function mps_set_chemistry
+
+        integer(c_int) function mps_species_transport(ctx, specnum, dt, timederiv, efield, electemp, numden, &
+                Tg, Pg, scaling, soln, vel, dcoeff, reac, source, prod) bind(C, name='mps_species_transport')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, efield, electemp, numden, scaling, soln, vel, dcoeff, reac, source, prod
+            integer(c_int), value :: specnum, timederiv
+            real(c_double), value :: dt, Tg, Pg
+        end function mps_species_transport
+
+        integer(c_int) function mps_eenergy_transport(ctx, dt, timederiv, lscale, efield, electemp, numden, &
+                Tg, Pg, scaling, soln, vel, dcoeff, reac, source) bind(C, name='mps_eenergy_transport')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, efield, electemp, numden, scaling, soln, vel, dcoeff, reac, source
+            integer(c_int), value :: timederiv
+            real(c_double), value :: dt, lscale, Tg, Pg
+        end function mps_eenergy_transport
+
+        integer(c_int) function mps_poisson_transport(ctx, dt, timederiv, elecden, iondens, scaling, soln, &
+                vel, dcoeff, reac, source) bind(C, name='mps_poisson_transport')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, elecden, iondens, scaling, soln, vel, dcoeff, reac, source
+            integer(c_int), value :: timederiv
+            real(c_double), value :: dt
+        end function mps_poisson_transport
+
+        integer(c_int) function mps_electron_temp(ctx, eenergysoln, elecden, eenergyscale, etempmin, electemp) &
+                bind(C, name='mps_electron_temp')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, eenergysoln, elecden, electemp
+            real(c_double), value :: eenergyscale, etempmin
+        end function mps_electron_temp
 
         integer(c_int) function mps_fill_ghosts(ctx, soln) bind(C, name='mps_fill_ghosts')
             import :: c_int, c_ptr
@@ -237,6 +280,98 @@ contains
         if (limiter) lim = 1
         status = mps_efield(ctx, c_loc(poissonsoln), c_loc(efield), potscale / lscale, lim)
     end subroutine gpu_update_efield
+
+    ! initializechemistry: the chem_module's tables (reactants/products
+    ! (nreac, nspecies), k_arrh(3, nreac), per-species mobility, dcoeff,
+    ! molmass, charge) copied to the device
+    subroutine gpu_set_chemistry(ctx, nspecies, nreac, especnum, bgspecnum, ionspecmin, ionspecmax, &
+                                 reactants, products, k_arrh, elecenergy, mobility, dcoeff, molmass, charge, status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: nspecies, nreac, especnum, bgspecnum, ionspecmin, ionspecmax
+        integer(c_int), intent(in), target :: reactants(nreac, nspecies), products(nreac, nspecies)
+        real*8, intent(in), target :: k_arrh(3, nreac), elecenergy(nreac)
+        real*8, intent(in), target :: mobility(nspecies), dcoeff(nspecies), molmass(nspecies), charge(nspecies)
+        integer, intent(out) :: status
+        type(mps_chem) :: ch
+        ch%nspecies = nspecies
+        ch%nreac = nreac
+        ch%especnum = especnum
+        ch%bgspecnum = bgspecnum
+        ch%ionspecmin = ionspecmin
+        ch%ionspecmax = ionspecmax
+        ch%reactants = c_loc(reactants)
+        ch%products = c_loc(products)
+        ch%k_arrh = c_loc(k_arrh)
+        ch%elecenergy = c_loc(elecenergy)
+        ch%mobility = c_loc(mobility)
+        ch%dcoeff = c_loc(dcoeff)
+        ch%molmass = c_loc(molmass)
+        ch%charge = c_loc(charge)
+        status = mps_set_chemistry(ctx, ch)
+    end subroutine gpu_set_chemistry
+
+    ! eden_update_transport / ions_update_transport with the module's
+    ! vel, dcoeff, reac, source arrays and scalings passed explicitly;
+    ! prod = elecprod for the electrons (pass any array for ions: its
+    ! interior gets the unscaled production)
+    subroutine gpu_species_transport(ctx, specnum, dt, elecfield, electemp, numden, Tg, Pg, scaling, soln, &
+                                     vel, dcoeff, reac, source, prod, lx, ly, lz, nsp, status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: specnum, lx, ly, lz, nsp
+        real*8, intent(in) :: dt, Tg, Pg
+        real*8, intent(in), target :: elecfield(0:lx+1, 0:ly+1, 0:lz+1, 3), electemp(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in), target :: numden(0:lx+1, 0:ly+1, 0:lz+1, nsp), soln(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in), target :: scaling(4)
+        real*8, intent(inout), target :: vel(0:lx+1, 0:ly+1, 0:lz+1, 3), dcoeff(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(inout), target :: reac(0:lx+1, 0:ly+1, 0:lz+1), source(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(inout), target :: prod(0:lx+1, 0:ly+1, 0:lz+1)
+        integer, intent(out) :: status
+        status = mps_species_transport(ctx, specnum, dt, 1, c_loc(elecfield), c_loc(electemp), c_loc(numden), &
+            Tg, Pg, c_loc(scaling), c_loc(soln), c_loc(vel), c_loc(dcoeff), c_loc(reac), c_loc(source), c_loc(prod))
+    end subroutine gpu_species_transport
+
+    ! eenergy_update_transport (lscale: the length scale of the Joule term's dx)
+    subroutine gpu_eenergy_transport(ctx, dt, lscale, elecfield, electemp, numden, Tg, Pg, scaling, soln, &
+                                     vel, dcoeff, reac, source, lx, ly, lz, nsp, status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: lx, ly, lz, nsp
+        real*8, intent(in) :: dt, lscale, Tg, Pg
+        real*8, intent(in), target :: elecfield(0:lx+1, 0:ly+1, 0:lz+1, 3), electemp(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in), target :: numden(0:lx+1, 0:ly+1, 0:lz+1, nsp), soln(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in), target :: scaling(4)
+        real*8, intent(inout), target :: vel(0:lx+1, 0:ly+1, 0:lz+1, 3), dcoeff(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(inout), target :: reac(0:lx+1, 0:ly+1, 0:lz+1), source(0:lx+1, 0:ly+1, 0:lz+1)
+        integer, intent(out) :: status
+        status = mps_eenergy_transport(ctx, dt, 1, lscale, c_loc(elecfield), c_loc(electemp), c_loc(numden), &
+            Tg, Pg, c_loc(scaling), c_loc(soln), c_loc(vel), c_loc(dcoeff), c_loc(reac), c_loc(source))
+    end subroutine gpu_eenergy_transport
+
+    ! poisson_update_transport (iondens: the solved ions one after the other)
+    subroutine gpu_poisson_transport(ctx, dt, elecden, iondens, scaling, soln, vel, dcoeff, reac, source, &
+                                     lx, ly, lz, nions, status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: lx, ly, lz, nions
+        real*8, intent(in) :: dt
+        real*8, intent(in), target :: elecden(0:lx+1, 0:ly+1, 0:lz+1), iondens(0:lx+1, 0:ly+1, 0:lz+1, nions)
+        real*8, intent(in), target :: scaling(4), soln(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(inout), target :: vel(0:lx+1, 0:ly+1, 0:lz+1, 3), dcoeff(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(inout), target :: reac(0:lx+1, 0:ly+1, 0:lz+1), source(0:lx+1, 0:ly+1, 0:lz+1)
+        integer, intent(out) :: status
+        status = mps_poisson_transport(ctx, dt, 1, c_loc(elecden), c_loc(iondens), c_loc(scaling), c_loc(soln), &
+            c_loc(vel), c_loc(dcoeff), c_loc(reac), c_loc(source))
+    end subroutine gpu_poisson_transport
+
+    ! update_electrontemp
+    subroutine gpu_update_electrontemp(ctx, electemp, elecden, eenergysoln, eenergyscale, etempmin, &
+                                       lx, ly, lz, status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: lx, ly, lz
+        real*8, intent(inout), target :: electemp(0:lx+1, 0:ly+1, 0:lz+1), eenergysoln(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in), target :: elecden(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in) :: eenergyscale, etempmin
+        integer, intent(out) :: status
+        status = mps_electron_temp(ctx, c_loc(eenergysoln), c_loc(elecden), eenergyscale, etempmin, c_loc(electemp))
+    end subroutine gpu_update_electrontemp
 
     ! text of the library's last error
     function gpu_last_error() result(msg)

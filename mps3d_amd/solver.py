@@ -62,6 +62,14 @@ def from_device(t):
     return np.asfortranarray(t.detach().cpu().numpy().transpose(2, 1, 0))
 
 
+def _scaling(sc):
+    """{vel, dcoeff, reac, source} scalings as a double[4]."""
+    v = [float(x) for x in sc]
+    if len(v) != 4:
+        raise ValueError("scaling needs 4 values: vel, dcoeff, reac, source")
+    return (C.c_double * 4)(*v)
+
+
 class FieldSolver:
     """One grid on one GPU (an ``mps_ctx``)."""
 
@@ -100,6 +108,7 @@ class FieldSolver:
         self._keep = []
         self._ext = None        # the context's stream as a torch ExternalStream
         self._torch_call = False
+        self._chem = None       # the Chemistry of the last set_chemistry
 
     # ------------------------------------------------------------ plumbing
     @property
@@ -256,6 +265,130 @@ class FieldSolver:
         finally:
             self._release()
         return E
+
+    # ------------------------------------------------------------ transport
+    def _fields(self, a, ncomp, writable=False):
+        """Pointer to ncomp ghosted fields one after the other: a device tensor
+        of ncomp * (nx+2)(ny+2)(nz+2) float64 elements, or a Fortran-order
+        (nx+2, ny+2, nz+2[, ncomp]) numpy array (in/out arrays must already
+        be Fortran-ordered float64)."""
+        if a is None:
+            return None
+        want = ncomp * int(np.prod(self.shape))
+        if _is_torch(a):
+            import torch
+            if not a.is_cuda or a.dtype != torch.float64 or not a.is_contiguous() or a.numel() != want:
+                raise ValueError("device arrays must be contiguous float64 GPU tensors of %d elements" % want)
+            if not self._torch_call:
+                self.order_torch_in()
+            return C.c_void_p(a.data_ptr())
+        if writable:
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.f_contiguous):
+                raise ValueError("in/out arrays must be Fortran-ordered float64 numpy arrays (or device tensors)")
+            arr = a
+        else:
+            arr = np.asfortranarray(a, dtype=np.float64)
+        if arr.size != want or arr.shape[:3] != self.shape:
+            raise ValueError("expected %d ghosted field(s) of shape %s, got %s" % (ncomp, self.shape, arr.shape))
+        self._keep.append(arr)
+        return C.c_void_p(arr.ctypes.data)
+
+    def _outputs(self, out, device):
+        """vel, dcoeff, reac, source (and prod) outputs: the caller's dict
+        entries, or new host arrays / device tensors."""
+        nx, ny, nz = self.n
+        g = (nx + 2, ny + 2, nz + 2)
+        res = dict(out or {})
+        if device:
+            import torch
+            ng = int(np.prod(g))
+            dev = device
+            for k, m in (("vel", 3), ("dcoeff", 1), ("reac", 1), ("source", 1), ("prod", 1)):
+                res.setdefault(k, torch.zeros(m * ng, dtype=torch.float64, device=dev))
+        else:
+            res.setdefault("vel", np.zeros(g + (3,), order="F"))
+            for k in ("dcoeff", "reac", "source", "prod"):
+                res.setdefault(k, np.zeros(g, order="F"))
+        return res
+
+    def set_chemistry(self, chem):
+        """initializechemistry: the tables of a :class:`Chemistry` to the device."""
+        s, keep = chem.struct()
+        self._chem = chem
+        M.check(self.lib.mps_set_chemistry(self._h, C.byref(s)))
+        del keep
+
+    def _need_chem(self):
+        if self._chem is None:
+            raise M.MpsError(M.ERR_STATE, "set the chemistry first (set_chemistry)")
+        return self._chem
+
+    def species_transport(self, specnum, efield, electemp, numden, scaling, soln=None, dt=1.0, timederiv=True,
+                          Tg=300.0, Pg=101325.0, out=None):
+        """eden_update_transport / ions_update_transport for specnum (1-based):
+        returns {"vel", "dcoeff", "reac", "source", "prod"} (ghosted; vel three
+        components; prod the unscaled production, interior cells)."""
+        dev = efield.device if _is_torch(efield) else None
+        o = self._outputs(out, dev)
+        ns = self._need_chem().nspecies
+        try:
+            M.check(self.lib.mps_species_transport(
+                self._h, int(specnum), float(dt), int(bool(timederiv)), self._fields(efield, 3),
+                self._fields(electemp, 1), self._fields(numden, ns), float(Tg), float(Pg), _scaling(scaling),
+                self._fields(soln, 1) if timederiv else None, self._fields(o["vel"], 3, True),
+                self._fields(o["dcoeff"], 1, True), self._fields(o["reac"], 1, True),
+                self._fields(o["source"], 1, True), self._fields(o["prod"], 1, True)))
+        finally:
+            self._release()
+        return o
+
+    def eenergy_transport(self, efield, electemp, numden, scaling, lscale=1.0, soln=None, dt=1.0, timederiv=True,
+                          Tg=300.0, Pg=101325.0, out=None):
+        """eenergy_update_transport: {"vel", "dcoeff", "reac", "source"}."""
+        dev = efield.device if _is_torch(efield) else None
+        o = self._outputs(out, dev)
+        o.pop("prod", None)
+        ns = self._need_chem().nspecies
+        try:
+            M.check(self.lib.mps_eenergy_transport(
+                self._h, float(dt), int(bool(timederiv)), float(lscale), self._fields(efield, 3),
+                self._fields(electemp, 1), self._fields(numden, ns), float(Tg), float(Pg), _scaling(scaling),
+                self._fields(soln, 1) if timederiv else None, self._fields(o["vel"], 3, True),
+                self._fields(o["dcoeff"], 1, True), self._fields(o["reac"], 1, True),
+                self._fields(o["source"], 1, True)))
+        finally:
+            self._release()
+        return o
+
+    def poisson_transport(self, elecden, iondens, scaling, soln=None, dt=1.0, timederiv=True, out=None):
+        """poisson_update_transport: {"vel", "dcoeff", "reac", "source"};
+        iondens holds the ion species one after the other."""
+        dev = elecden.device if _is_torch(elecden) else None
+        o = self._outputs(out, dev)
+        o.pop("prod", None)
+        ch = self._need_chem()
+        nions = ch.ionspecmax - ch.ionspecmin + 1
+        try:
+            M.check(self.lib.mps_poisson_transport(
+                self._h, float(dt), int(bool(timederiv)), self._fields(elecden, 1),
+                self._fields(iondens, nions) if nions > 0 else None, _scaling(scaling),
+                self._fields(soln, 1) if timederiv else None, self._fields(o["vel"], 3, True),
+                self._fields(o["dcoeff"], 1, True), self._fields(o["reac"], 1, True),
+                self._fields(o["source"], 1, True)))
+        finally:
+            self._release()
+        return o
+
+    def electron_temp(self, eenergysoln, elecden, eenergyscale, etempmin, electemp):
+        """update_electrontemp: writes electemp's interior and clamps
+        eenergysoln (both in/out)."""
+        try:
+            M.check(self.lib.mps_electron_temp(self._h, self._fields(eenergysoln, 1, True),
+                                               self._fields(elecden, 1), float(eenergyscale), float(etempmin),
+                                               self._fields(electemp, 1, True)))
+        finally:
+            self._release()
+        return electemp
 
     def solve_rhs(self, b, soln, raise_on_failure=False):
         """A x = b for a right-hand side b (interior) given directly, with the

@@ -819,6 +819,245 @@ void orc_efield(const orc_problem *p, double *phi, double *efield, double scale,
     for (int q = 0; q < 3; ++q) level_ghosts(&Z, E[q]);
 }
 
+/* ---------------------------------------------------------------------
+ * Transport assembly, plasma_pde/{edeneq,ionseq,eenergyeq,poissoneq}.f90,
+ * with the table chemistry of plasma_pde/chemistry/ion_elec_chem.f90. Each
+ * routine follows its reference loop by loop: zero the outputs, fill the
+ * interior, scale the whole arrays, add soln/dt over the whole arrays.
+ * Constants: plasma_pde/fundconstants.f90:3-9.
+ * --------------------------------------------------------------------- */
+#define ORC_ECHARGE 1.602176e-19
+#define ORC_EPS0 8.854188e-12
+#define ORC_KBMAN 1.380650e-23
+
+/* real**integer (m >= 0) by repeated squaring */
+static double orc_powi(double x, int m)
+{
+    double r = (m & 1) ? x : 1.0;
+    while (m >>= 1) {
+        x = x * x;
+        if (m & 1) r = r * x;
+    }
+    return r;
+}
+
+/* getarrhrate, ion_elec_chem.f90:260-274 */
+static double orc_arrh(const double *k, double T)
+{
+    double A = k[0], alpha = k[1], Ea = k[2];
+    return A * pow(T, alpha) * exp(-Ea / T);
+}
+
+/* specmult of reaction r: prod_j specden(j)**reactants(r,j) */
+static double orc_specmult(const orc_chem *ch, int r, const double *specden)
+{
+    double m = 1.0;
+    for (int j = 0; j < ch->nspecies; ++j) m = m * orc_powi(specden[j], ch->reactants[r + ch->nreac * j]);
+    return m;
+}
+
+/* getspecproduction, ion_elec_chem.f90:111-156 (specnum 1-based) */
+static double orc_specproduction(const orc_chem *ch, int specnum, double Te, const double *specden)
+{
+    double specprod = 0.0;
+    for (int r = 0; r < ch->nreac; ++r) {
+        int s = specnum - 1;
+        int nmoles = ch->products[r + ch->nreac * s] - ch->reactants[r + ch->nreac * s];
+        if (nmoles != 0) {
+            double k = orc_arrh(ch->k_arrh + 3 * r, Te);
+            double specmult = orc_specmult(ch, r, specden);
+            specprod = specprod + nmoles * k * specmult;
+        }
+    }
+    return specprod;
+}
+
+/* getelectroninelasticterm, ion_elec_chem.f90:158-199 */
+static double orc_inelterm(const orc_chem *ch, double Te, const double *specden)
+{
+    double inelterm = 0.0;
+    for (int r = 0; r < ch->nreac; ++r) {
+        if (ch->elecenergy[r] != 0) {
+            double k = orc_arrh(ch->k_arrh + 3 * r, Te);
+            double specmult = orc_specmult(ch, r, specden);
+            inelterm = inelterm + k * specmult * ch->elecenergy[r];
+        }
+    }
+    return inelterm;
+}
+
+static void orc_specarray(const orc_chem *ch, const double *numden, size_t ng, size_t c, double *sa)
+{
+    for (int s = 0; s < ch->nspecies; ++s) sa[s] = numden[(size_t)s * ng + c];
+}
+
+/* the scaling and time-term tail shared by the *_update_transport routines */
+static void orc_transport_tail(size_t ng, int timederiv, double dt, const double *scaling, const double *soln,
+                               double *vel, double *dcoeff, double *reac, double *source)
+{
+    for (size_t c = 0; c < 3 * ng; ++c) vel[c] = vel[c] * scaling[0];
+    for (size_t c = 0; c < ng; ++c) {
+        dcoeff[c] = dcoeff[c] * scaling[1];
+        reac[c] = reac[c] * scaling[2];
+        source[c] = source[c] * scaling[3];
+    }
+    if (timederiv)
+        for (size_t c = 0; c < ng; ++c) source[c] = source[c] + soln[c] / dt;
+}
+
+void orc_species_transport(const orc_problem *p, const orc_chem *ch, int specnum, double dt, int timederiv,
+                           const double *efield, const double *electemp, const double *numden, double Tg,
+                           const double *scaling, const double *soln, orc_transport_out out)
+{
+    (void)Tg;
+    lev_t L = finest(p);
+    size_t ng = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    for (size_t c = 0; c < 3 * ng; ++c) out.vel[c] = 0.0;
+    for (size_t c = 0; c < ng; ++c) out.dcoeff[c] = out.reac[c] = out.source[c] = 0.0;
+    double sa[64];
+    for (int k = 1; k <= L.nz; ++k)
+        for (int j = 1; j <= L.ny; ++j)
+            for (int i = 1; i <= L.nx; ++i) {
+                size_t c = IDX(&L, i, j, k);
+                orc_specarray(ch, numden, ng, c, sa);
+                /* getspecmobility / getspecdcoeff of ion_elec_chem: constants */
+                double mu = ch->mobility[specnum - 1];
+                for (int q = 0; q < 3; ++q) out.vel[q * ng + c] = mu * efield[q * ng + c];
+                out.dcoeff[c] = ch->dcoeff[specnum - 1];
+                double specprod = orc_specproduction(ch, specnum, electemp[c], sa);
+                out.source[c] = out.source[c] + specprod;
+                if (out.prod) out.prod[c] = out.source[c];
+            }
+    orc_transport_tail(ng, timederiv, dt, scaling, soln, out.vel, out.dcoeff, out.reac, out.source);
+}
+
+/* findjheating_eachdir, eenergyeq.f90:461-555, for cell c along stride st */
+static double orc_jheat_dir(const double *ne, const double *ef, size_t c, size_t st, double mu_e, double D_e,
+                            double dx)
+{
+    double grad_ne = (ne[c] - ne[c - st]) / dx;
+    double elec_field_left = (ef[c] + ef[c - st]) * 0.5;
+    double e_den = (mu_e * elec_field_left > 0) ? ne[c - st] : ne[c];
+    double Gamma_left = (mu_e * e_den * elec_field_left - D_e * grad_ne);
+    grad_ne = (ne[c + st] - ne[c]) / dx;
+    double elec_field_right = (ef[c] + ef[c + st]) * 0.5;
+    e_den = (mu_e * elec_field_right > 0) ? ne[c] : ne[c + st];
+    double Gamma_right = (mu_e * e_den * elec_field_right - D_e * grad_ne);
+    double jheat = -(ORC_ECHARGE * 0.5 * (elec_field_left + elec_field_right) * 0.5 * (Gamma_left + Gamma_right));
+    if (jheat < 0.0) jheat = 0.0;
+    return jheat;
+}
+
+void orc_eenergy_transport(const orc_problem *p, const orc_chem *ch, double dt, int timederiv, double lscale,
+                           const double *efield, const double *electemp, const double *numden, double Tg,
+                           const double *scaling, const double *soln, orc_transport_out out)
+{
+    lev_t L = finest(p);
+    size_t ng = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    size_t n = (size_t)L.nx * L.ny * L.nz;
+    const size_t st[3] = {1, (size_t)(L.nx + 2), (size_t)(L.nx + 2) * (L.ny + 2)};
+    const double dxl[3] = {L.dx * lscale, L.dy * lscale, L.dz * lscale};
+    int e = ch->especnum - 1;
+    const double *elecden = numden + (size_t)e * ng;
+    double *jheat = calloc(n, sizeof(double)), *elastic = calloc(n, sizeof(double));
+    double *inelastic = calloc(n, sizeof(double));
+    double sa[64];
+    size_t t;
+    /* jouleheatingterm (:394-459) */
+    t = 0;
+    for (int k = 1; k <= L.nz; ++k)
+        for (int j = 1; j <= L.ny; ++j)
+            for (int i = 1; i <= L.nx; ++i, ++t) {
+                size_t c = IDX(&L, i, j, k);
+                double mu_e = ch->mobility[e], D_e = ch->dcoeff[e];
+                double jx = orc_jheat_dir(elecden, efield, c, st[0], mu_e, D_e, dxl[0]);
+                double jy = orc_jheat_dir(elecden, efield + ng, c, st[1], mu_e, D_e, dxl[1]);
+                double jz = orc_jheat_dir(elecden, efield + 2 * ng, c, st[2], mu_e, D_e, dxl[2]);
+                jheat[t] = jx + jy + jz;
+            }
+    /* inelastic_colterm (:614-677) */
+    t = 0;
+    for (int k = 1; k <= L.nz; ++k)
+        for (int j = 1; j <= L.ny; ++j)
+            for (int i = 1; i <= L.nx; ++i, ++t) {
+                size_t c = IDX(&L, i, j, k);
+                orc_specarray(ch, numden, ng, c, sa);
+                inelastic[t] = orc_inelterm(ch, electemp[c], sa) * ORC_ECHARGE;
+            }
+    /* elastic_colterm (:557-612), getelectroncollisionfrequency
+     * (ion_elec_chem.f90:244-258) */
+    double m_e = ch->molmass[e], m_bg = ch->molmass[ch->bgspecnum - 1];
+    t = 0;
+    for (int k = 1; k <= L.nz; ++k)
+        for (int j = 1; j <= L.ny; ++j)
+            for (int i = 1; i <= L.nx; ++i, ++t) {
+                size_t c = IDX(&L, i, j, k);
+                double nu = -(ORC_ECHARGE / m_e / ch->mobility[e]);
+                elastic[t] = 3.0 * ORC_KBMAN * elecden[c] * (electemp[c] - Tg) * (m_e / m_bg) * nu;
+            }
+    for (size_t c = 0; c < 3 * ng; ++c) out.vel[c] = 0.0;
+    for (size_t c = 0; c < ng; ++c) out.dcoeff[c] = out.reac[c] = out.source[c] = 0.0;
+    const double five_by_three = 5.0 / 3.0;
+    t = 0;
+    for (int k = 1; k <= L.nz; ++k)
+        for (int j = 1; j <= L.ny; ++j)
+            for (int i = 1; i <= L.nx; ++i, ++t) {
+                size_t c = IDX(&L, i, j, k);
+                double mu_e = ch->mobility[e];
+                for (int q = 0; q < 3; ++q) out.vel[q * ng + c] = five_by_three * mu_e * efield[q * ng + c];
+                out.dcoeff[c] = five_by_three * ch->dcoeff[e];
+                out.source[c] = jheat[t] - elastic[t] + inelastic[t];
+            }
+    free(jheat);
+    free(elastic);
+    free(inelastic);
+    orc_transport_tail(ng, timederiv, dt, scaling, soln, out.vel, out.dcoeff, out.reac, out.source);
+}
+
+void orc_poisson_transport(const orc_problem *p, const orc_chem *ch, double dt, int timederiv,
+                           const double *elecden, const double *iondens, const double *scaling,
+                           const double *soln, orc_transport_out out)
+{
+    lev_t L = finest(p);
+    size_t ng = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    for (size_t c = 0; c < 3 * ng; ++c) out.vel[c] = 0.0;
+    for (size_t c = 0; c < ng; ++c) out.dcoeff[c] = out.reac[c] = out.source[c] = 0.0;
+    int nions = ch->ionspecmax - ch->ionspecmin + 1;
+    for (int k = 1; k <= L.nz; ++k)
+        for (int j = 1; j <= L.ny; ++j)
+            for (int i = 1; i <= L.nx; ++i) {
+                size_t c = IDX(&L, i, j, k);
+                double eden = elecden[c], total_ionden = 0.0;
+                for (int s = 0; s < nions; ++s)
+                    total_ionden = total_ionden + ch->charge[ch->ionspecmin - 1 + s] * iondens[(size_t)s * ng + c];
+                out.source[c] = (total_ionden - eden);
+            }
+    for (size_t c = 0; c < ng; ++c) {
+        out.source[c] = out.source[c] * ORC_ECHARGE / ORC_EPS0;
+        out.dcoeff[c] = 1.0 * scaling[1];
+        out.source[c] = out.source[c] * scaling[3];
+    }
+    if (timederiv)
+        for (size_t c = 0; c < ng; ++c) out.source[c] = out.source[c] + soln[c] / dt;
+}
+
+void orc_electron_temp(const orc_problem *p, double *eenergysoln, const double *elecden, double eenergyscale,
+                       double etempmin, double *electemp)
+{
+    lev_t L = finest(p);
+    const double three_by_two = 1.5;
+    for (int k = 1; k <= L.nz; ++k)
+        for (int j = 1; j <= L.ny; ++j)
+            for (int i = 1; i <= L.nx; ++i) {
+                size_t c = IDX(&L, i, j, k);
+                electemp[c] = eenergysoln[c] * eenergyscale / (three_by_two * elecden[c] * ORC_KBMAN);
+                if (electemp[c] < etempmin) {
+                    electemp[c] = etempmin;
+                    eenergysoln[c] = three_by_two * elecden[c] * ORC_KBMAN * etempmin / eenergyscale;
+                }
+            }
+}
+
 /* find_sg_exponential_flux, mps1d/source/convdiff.f90:10-64 */
 void orc_sg_flux(double cL, double cR, double DL, double DR, double uL, double uR, double dx,
                  double *wL, double *wR, double *flux)

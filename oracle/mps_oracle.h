@@ -57,6 +57,41 @@ void orc_ghosts(const orc_problem *p, double *u);
 
 /* update_efield, plasma_pde/poissoneq.f90:565-680 (efield: 3 ghosted components) */
 void orc_efield(const orc_problem *p, double *phi, double *efield, double scale, int limiter);
+/* Chemistry tables of a chem_module (plasma_pde/chemistry/ion_elec_chem.f90:
+ * reactants/products(nreac, nspecies) and k_arrh(3, nreac) in Fortran
+ * order, 1-based species numbers). */
+typedef struct {
+    int nspecies, nreac;
+    int especnum, bgspecnum, ionspecmin, ionspecmax;
+    const int *reactants, *products;
+    const double *k_arrh, *elecenergy;
+    const double *mobility, *dcoeff, *molmass, *charge;
+} orc_chem;
+
+/* Outputs of an *_update_transport: vel (3 ghosted components), dcoeff,
+ * reac, source (ghosted), prod (interior cells of a ghosted array). */
+typedef struct {
+    double *vel, *dcoeff, *reac, *source, *prod;
+} orc_transport_out;
+
+/* eden_update_transport (plasma_pde/edeneq.f90:360-444) and
+ * ions_update_transport (plasma_pde/ionseq.f90:478-558) for species specnum;
+ * scaling = {vel, dcoeff, reac, source}. */
+void orc_species_transport(const orc_problem *p, const orc_chem *ch, int specnum, double dt, int timederiv,
+                           const double *efield, const double *electemp, const double *numden, double Tg,
+                           const double *scaling, const double *soln, orc_transport_out out);
+/* eenergy_update_transport (plasma_pde/eenergyeq.f90:394-774). */
+void orc_eenergy_transport(const orc_problem *p, const orc_chem *ch, double dt, int timederiv, double lscale,
+                           const double *efield, const double *electemp, const double *numden, double Tg,
+                           const double *scaling, const double *soln, orc_transport_out out);
+/* poisson_update_transport (plasma_pde/poissoneq.f90:335-385). */
+void orc_poisson_transport(const orc_problem *p, const orc_chem *ch, double dt, int timederiv,
+                           const double *elecden, const double *iondens, const double *scaling,
+                           const double *soln, orc_transport_out out);
+/* update_electrontemp (plasma_pde/eenergyeq.f90:842-877), the clamp's
+ * energy reset in the clamped cell itself. */
+void orc_electron_temp(const orc_problem *p, double *eenergysoln, const double *elecden, double eenergyscale,
+                       double etempmin, double *electemp);
 /* restrict_boundary for one face array of (a,b) -> (a/2,b/2). */
 void orc_restrict_face(int a, int b, const double *f, double *c);
 /* One perform_vcycle on the finest level; returns the RMS residual. */

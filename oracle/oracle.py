@@ -53,6 +53,13 @@ def lib():
         L.orc_prolong1.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.orc_ghosts.argtypes = [P, _dp]
         L.orc_efield.argtypes = [P, _dp, _dp, C.c_double, C.c_int]
+        CP, TO = C.POINTER(_Chem), _TOut
+        L.orc_species_transport.argtypes = [P, CP, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, C.c_double, _dp,
+                                            _dp, TO]
+        L.orc_eenergy_transport.argtypes = [P, CP, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double,
+                                            _dp, _dp, TO]
+        L.orc_poisson_transport.argtypes = [P, CP, C.c_double, C.c_int, _dp, _dp, _dp, _dp, TO]
+        L.orc_electron_temp.argtypes = [P, _dp, _dp, C.c_double, C.c_double, _dp]
         L.orc_restrict_face.argtypes = [C.c_int, C.c_int, _dp, _dp]
         L.orc_vcycle.argtypes = [P, _dp, _dp]
         L.orc_vcycle.restype = C.c_double
@@ -158,6 +165,93 @@ def efield(p: Problem, phi, scale=1.0, limiter=False):
     E = np.zeros((nx + 2, ny + 2, nz + 2, 3), order="F")
     lib().orc_efield(p.ref, _ptr(phi), _ptr(E), float(scale), int(bool(limiter)))
     return phi, E
+
+
+# ---------------------------------------------------------------- transport
+class _Chem(C.Structure):
+    _fields_ = [("nspecies", C.c_int), ("nreac", C.c_int), ("especnum", C.c_int), ("bgspecnum", C.c_int),
+                ("ionspecmin", C.c_int), ("ionspecmax", C.c_int),
+                ("reactants", C.c_void_p), ("products", C.c_void_p), ("k_arrh", C.c_void_p),
+                ("elecenergy", C.c_void_p), ("mobility", C.c_void_p), ("dcoeff", C.c_void_p),
+                ("molmass", C.c_void_p), ("charge", C.c_void_p)]
+
+
+class _TOut(C.Structure):
+    _fields_ = [("vel", _dp), ("dcoeff", _dp), ("reac", _dp), ("source", _dp), ("prod", _dp)]
+
+
+def _chem(ch):
+    """orc_chem for an object with the tables of a chem_module (names,
+    reactants/products (nreac, nspecies), k_arrh (nreac, 3), elecenergy,
+    mobility, dcoeff, molmass, charge, especnum, bgspecnum, ionspecmin,
+    ionspecmax)."""
+    keep = [np.asfortranarray(ch.reactants, dtype=np.int32), np.asfortranarray(ch.products, dtype=np.int32),
+            np.ascontiguousarray(ch.k_arrh, dtype=np.float64), np.ascontiguousarray(ch.elecenergy, dtype=np.float64),
+            np.ascontiguousarray(ch.mobility, dtype=np.float64), np.ascontiguousarray(ch.dcoeff, dtype=np.float64),
+            np.ascontiguousarray(ch.molmass, dtype=np.float64), np.ascontiguousarray(ch.charge, dtype=np.float64)]
+    s = _Chem(len(ch.mobility), int(np.asarray(ch.reactants).shape[0]), int(ch.especnum), int(ch.bgspecnum),
+              int(ch.ionspecmin), int(ch.ionspecmax), *[a.ctypes.data for a in keep])
+    return s, keep
+
+
+def _touts(p: Problem, prod=True):
+    nx, ny, nz = p.n
+    g = (nx + 2, ny + 2, nz + 2)
+    o = {"vel": np.zeros(g + (3,), order="F"), "dcoeff": np.zeros(g, order="F"), "reac": np.zeros(g, order="F"),
+         "source": np.zeros(g, order="F")}
+    if prod:
+        o["prod"] = np.zeros(g, order="F")
+    s = _TOut(_ptr(o["vel"]), _ptr(o["dcoeff"]), _ptr(o["reac"]), _ptr(o["source"]), _ptr(o.get("prod")))
+    return o, s
+
+
+def _f(a, ncomp=None):
+    return None if a is None else np.asfortranarray(a, dtype=np.float64)
+
+
+def species_transport(p: Problem, ch, specnum, efield, electemp, numden, scaling, soln=None, dt=1.0,
+                      timederiv=True, Tg=300.0):
+    """eden_update_transport / ions_update_transport (edeneq.f90:360-444,
+    ionseq.f90:478-558): {"vel", "dcoeff", "reac", "source", "prod"}."""
+    cs, keep = _chem(ch)
+    o, os_ = _touts(p)
+    E, T, N, S = _f(efield), _f(electemp), _f(numden), _f(soln)
+    sc = np.asarray(scaling, dtype=np.float64)
+    lib().orc_species_transport(p.ref, C.byref(cs), int(specnum), float(dt), int(bool(timederiv)), _ptr(E), _ptr(T),
+                                _ptr(N), float(Tg), _ptr(sc), _ptr(S), os_)
+    return o
+
+
+def eenergy_transport(p: Problem, ch, efield, electemp, numden, scaling, lscale=1.0, soln=None, dt=1.0,
+                      timederiv=True, Tg=300.0):
+    """eenergy_update_transport (eenergyeq.f90:394-774)."""
+    cs, keep = _chem(ch)
+    o, os_ = _touts(p, prod=False)
+    E, T, N, S = _f(efield), _f(electemp), _f(numden), _f(soln)
+    sc = np.asarray(scaling, dtype=np.float64)
+    lib().orc_eenergy_transport(p.ref, C.byref(cs), float(dt), int(bool(timederiv)), float(lscale), _ptr(E), _ptr(T),
+                                _ptr(N), float(Tg), _ptr(sc), _ptr(S), os_)
+    return o
+
+
+def poisson_transport(p: Problem, ch, elecden, iondens, scaling, soln=None, dt=1.0, timederiv=True):
+    """poisson_update_transport (poissoneq.f90:335-385)."""
+    cs, keep = _chem(ch)
+    o, os_ = _touts(p, prod=False)
+    ne, ni, S = _f(elecden), _f(iondens), _f(soln)
+    sc = np.asarray(scaling, dtype=np.float64)
+    lib().orc_poisson_transport(p.ref, C.byref(cs), float(dt), int(bool(timederiv)), _ptr(ne), _ptr(ni), _ptr(sc),
+                                _ptr(S), os_)
+    return o
+
+
+def electron_temp(p: Problem, eenergysoln, elecden, eenergyscale, etempmin, electemp=None):
+    """update_electrontemp (eenergyeq.f90:842-877): (eenergysoln, electemp)."""
+    eps = np.array(eenergysoln, dtype=np.float64, order="F", copy=True)
+    te = p.zeros() if electemp is None else np.array(electemp, dtype=np.float64, order="F", copy=True)
+    ne = _f(elecden)
+    lib().orc_electron_temp(p.ref, _ptr(eps), _ptr(ne), float(eenergyscale), float(etempmin), _ptr(te))
+    return eps, te
 
 
 def restrict(fine):

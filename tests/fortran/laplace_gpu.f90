@@ -7,13 +7,23 @@
 !
 ! Part 1 loops gpu_perform_vcycle the way generic_pde/pdedefs.f90:378-402
 ! loops perform_vcycle; part 2 makes the same solve with one gpu_solve;
-! part 3 with gpu_solve_rhs and the right-hand side passed directly.
+! part 3 with gpu_solve_rhs and the right-hand side passed directly;
+! part 4 takes the field of that potential with gpu_update_efield, as
+! plasma_pde/poissoneq.f90:549-680 does after each solve: for phi = x and
+! potscale / lscale = 2, E = (-2, 0, 0) in every cell and ghost face;
+! part 5 sets the ion_elec chemistry (plasma_pde/chemistry/ion_elec_chem.f90)
+! and assembles the electron transport in that field (vel = mu_e E) and
+! the Poisson space charge e/eps0 (n_i - n_e) on the device.
 ! Exit status: 0 pass, 1 fail, 2 no usable GPU.
 program laplace_gpu
     use mps3d_gpu
     implicit none
     integer, parameter :: n = 128, nvcycles = 20
-    real*8, allocatable, target :: u(:,:,:), vel(:,:,:,:), dc(:,:,:), reac(:,:,:), src(:,:,:), sterm(:,:,:)
+    real*8, allocatable, target :: u(:,:,:), ef(:,:,:,:), vel(:,:,:,:), tv(:,:,:,:), nd(:,:,:,:)
+    real*8, allocatable, target :: td(:,:,:), tr(:,:,:), ts(:,:,:), tp(:,:,:), te(:,:,:)
+    integer(c_int), target :: reactants(1, 3), products(1, 3)
+    real*8, allocatable, target :: dc(:,:,:), reac(:,:,:), src(:,:,:), sterm(:,:,:)
+    real*8, target :: k_arrh(3, 1), elecenergy(1), mob(3), dco(3), mass(3), chg(3), scal(4)
     real*8, allocatable, target :: left(:,:), right(:,:), bottom(:,:), top(:,:), back(:,:), front(:,:)
     real*8, target :: hist(nvcycles)
     character(len=4) :: codes(6)
@@ -22,7 +32,7 @@ program laplace_gpu
     real*8 :: resnorm, err, x, tol
     integer :: i, it, st, ncyc
 
-    allocate(u(0:n+1, 0:n+1, 0:n+1), vel(0:n+1, 0:n+1, 0:n+1, 3), dc(0:n+1, 0:n+1, 0:n+1))
+    allocate(u(0:n+1, 0:n+1, 0:n+1), ef(0:n+1, 0:n+1, 0:n+1, 3), vel(0:n+1, 0:n+1, 0:n+1, 3), dc(0:n+1, 0:n+1, 0:n+1))
     allocate(reac(0:n+1, 0:n+1, 0:n+1), src(0:n+1, 0:n+1, 0:n+1), sterm(n, n, n))
     allocate(left(n, n), right(n, n), bottom(n, n), top(n, n), back(n, n), front(n, n))
     u = 0.d0
@@ -106,6 +116,67 @@ program laplace_gpu
     enddo
     print '(a,i3,a,es12.4,a,es12.4)', ' gpu_solve_rhs: cycles', ncyc, ' resnorm', resnorm, ' max |phi - x| =', err
     if (err > 1.d-10) call exit(1)
+
+    ! part 4: E = -(potscale / lscale) grad phi, ghost faces by zero gradient
+    ef = 0.d0
+    call gpu_update_efield(ctx, u, ef, n, n, n, 4.d0, 2.d0, .false., st)
+    if (st /= MPS_OK) then
+        print *, 'laplace_gpu: update_efield failed ', st, trim(gpu_last_error())
+        call exit(1)
+    endif
+    err = max(maxval(abs(ef(0:n+1, 1:n, 1:n, 1) + 2.d0)), maxval(abs(ef(1:n, 0:n+1, 1:n, 2))), &
+              maxval(abs(ef(1:n, 1:n, 0:n+1, 3))))
+    print '(a,es12.4)', ' gpu_update_efield: max |E - (-2, 0, 0)| =', err
+    if (err > 1.d-6) call exit(1)
+
+    ! part 5: ion_elec chemistry, electron transport and Poisson space charge
+    reactants = 0
+    products = 0
+    reactants(1, 1) = 1
+    reactants(1, 2) = 1
+    products(1, 3) = 1
+    products(1, 2) = 2
+    k_arrh(:, 1) = (/ 2.5d-6 * 0.01**3, 0.d0, 278508.d0 /)
+    elecenergy = -15.578
+    mob = (/ 0.d0, -20.d0, 0.2d0 /)
+    dco = (/ 0.d0, 100.d0, 0.01d0 /)
+    mass = (/ 4.d0 * 1.672621d-27, 9.109382d-31, 4.d0 * 1.672621d-27 /)
+    chg = (/ 0.d0, -1.d0, 1.d0 /)
+    call gpu_set_chemistry(ctx, 3, 1, 2, 1, 3, 3, reactants, products, k_arrh, elecenergy, mob, dco, mass, chg, st)
+    if (st /= MPS_OK) then
+        print *, 'laplace_gpu: set_chemistry failed ', st, trim(gpu_last_error())
+        call exit(1)
+    endif
+    allocate(tv(0:n+1, 0:n+1, 0:n+1, 3), nd(0:n+1, 0:n+1, 0:n+1, 3), te(0:n+1, 0:n+1, 0:n+1))
+    allocate(td(0:n+1, 0:n+1, 0:n+1), tr(0:n+1, 0:n+1, 0:n+1), ts(0:n+1, 0:n+1, 0:n+1), tp(0:n+1, 0:n+1, 0:n+1))
+    nd(:, :, :, 1) = 1.d22
+    nd(:, :, :, 2) = 1.d15
+    nd(:, :, :, 3) = 3.d15
+    te = 2.d4
+    tp = 0.d0
+    scal = (/ 0.5d0, 0.25d0, 1.d0, 2.d0 /)
+    call gpu_species_transport(ctx, 2, 1.d0, ef, te, nd, 300.d0, 101325.d0, scal, u, tv, td, tr, ts, tp, &
+                               n, n, n, 3, st)
+    if (st /= MPS_OK) then
+        print *, 'laplace_gpu: species_transport failed ', st, trim(gpu_last_error())
+        call exit(1)
+    endif
+    ! mu_e E_x vel_scaling = -20 * (-2) * 0.5 up to the solve's error in E
+    err = max(maxval(abs(tv(1:n, 1:n, 1:n, 1) - 20.d0)), maxval(abs(td(1:n, 1:n, 1:n) - 25.d0)))
+    print '(a,es12.4)', ' gpu_species_transport: max |vel_x - 20|, |dcoeff - 25| =', err
+    if (err > 1.d-5 .or. maxval(tp(1:n, 1:n, 1:n)) <= 0.d0) call exit(1)
+    call gpu_poisson_transport(ctx, 1.d0, nd(:, :, :, 2), nd(:, :, :, 3:3), scal, u, tv, td, tr, ts, n, n, n, 1, st)
+    if (st /= MPS_OK) then
+        print *, 'laplace_gpu: poisson_transport failed ', st, trim(gpu_last_error())
+        call exit(1)
+    endif
+    err = 0.d0
+    do i = 1, n
+        err = max(err, maxval(abs(ts(i, 1:n, 1:n) - (2.d15 * 1.602176d-19 / 8.854188d-12 * 2.d0 + u(i, 1:n, 1:n))) &
+                  / (2.d15 * 1.602176d-19 / 8.854188d-12 * 2.d0)))
+    enddo
+    print '(a,es12.4)', ' gpu_poisson_transport: max rel. error of the space charge =', err
+    if (err > 1.d-14) call exit(1)
     st = mps_destroy(ctx)
     print *, 'laplace_gpu: PASS'
 end program laplace_gpu
