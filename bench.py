@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=257)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-step-kernels", action="store_true",
+                    help="time only the field and transport calls around the solves (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the whole 257^3 domain fits one MI355X "
@@ -296,6 +298,9 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(3)
     torch.cuda.set_device(0)
+    if args.only_step_kernels:
+        print(json.dumps({"n": args.n, "step_kernels": step_kernels(args.n)}), flush=True)
+        return
     case, hier, res, err = gpu_solves(args.n, args.steps, args.warmup)
     copy_gbps = calibration_copy()
     dc, f64 = res["fp32dc"], res["fp64"]

@@ -304,11 +304,14 @@ class FieldSolver:
             ng = int(np.prod(g))
             dev = device
             for k, m in (("vel", 3), ("dcoeff", 1), ("reac", 1), ("source", 1), ("prod", 1)):
-                res.setdefault(k, torch.zeros(m * ng, dtype=torch.float64, device=dev))
+                if k not in res:
+                    res[k] = torch.zeros(m * ng, dtype=torch.float64, device=dev)
         else:
-            res.setdefault("vel", np.zeros(g + (3,), order="F"))
+            if "vel" not in res:
+                res["vel"] = np.zeros(g + (3,), order="F")
             for k in ("dcoeff", "reac", "source", "prod"):
-                res.setdefault(k, np.zeros(g, order="F"))
+                if k not in res:
+                    res[k] = np.zeros(g, order="F")
         return res
 
     def set_chemistry(self, chem):
