@@ -200,6 +200,23 @@ def step_kernels(n, reps=20):
         "electron_temp": (lambda: s.electron_temp(eps, nd[ng:2 * ng], 1.0, etempmin_default(), te), 8 * 3),
     }
     res = {}
+    # writeoutput of two cell scalars from device tensors (the copy to the
+    # host included), one piece, into a scratch directory
+    import shutil
+    import tempfile
+
+    from mps3d_amd import vtk
+    td = tempfile.mkdtemp()
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        vtk.write_output(os.path.join(td, "out"), [phi, te], ("Potential (V)", "Electron_temp (K)"), (n, n, n),
+                         (0.02,) * 3, (1, 1, 1))
+        sec = time.perf_counter() - t0
+        nbytes = sum(os.path.getsize(os.path.join(r, f)) for r, _, fs in os.walk(td) for f in fs)
+        res["vtk_output"] = {"s_per_call": round(sec, 3), "bytes": nbytes, "MBps": round(nbytes / sec / 1e6, 1)}
+    finally:
+        shutil.rmtree(td, ignore_errors=True)
     for name, (fn, bpc) in calls.items():
         fn()
         torch.cuda.synchronize()

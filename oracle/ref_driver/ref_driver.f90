@@ -20,6 +20,9 @@
 !   mode 5  restriction, restrict_boundary, prolongation1
 !           (mgridlib/mgridsteps.f90:11, :55, :170)
 !   mode 6  update_boundary_ghostvalues  (mgridlib/discretepde.f90:1121)
+!   mode 7  writeoutput                  (mgridlib/cartwrite.f90:455-481):
+!           u0 and dcoeff as two cell scalars, written to vtkout.pvtr and
+!           vtkout/file_<rank>.vtr in the working directory
 !
 ! Input  (stream, little endian): int32 hdr(16), real64 rhdr(5),
 !        character(24) bc codes, the six global BC face arrays, then the
@@ -34,6 +37,7 @@ program ref_driver
     use discretepde
     use mgridsteps
     use solvergmres_module
+    use cartwrite
     implicit none
 
     character(len=512) :: infile, outfile
@@ -55,6 +59,8 @@ program ref_driver
     real*8, allocatable :: sterm(:,:,:), bvec(:,:,:), ax(:,:,:), diag(:,:,:), offdiag(:,:,:), res(:,:,:)
     real*8, allocatable :: x0(:,:,:), xs(:,:,:), coarse(:,:,:), fine(:,:,:), hist(:)
     type(boundarycondition) :: bcvals, bcvals_c
+    real*8, allocatable :: vtkvars(:,:,:,:)
+    character(len=40) :: vtknames(2)
 
     call MPI_Init(ierr)
     call get_command_argument(1, infile)
@@ -241,6 +247,16 @@ program ref_driver
         soln = gextra
         call update_boundary_ghostvalues(soln, bc_codes, bcvals, lx, ly, lz)
         if (g_myproc .eq. g_rootproc) call put_record('GHOSTED ', soln, size(soln))
+
+    case (7)
+        allocate(vtkvars(0:lx+1,0:ly+1,0:lz+1,2))
+        vtkvars = ZERO
+        vtkvars(:,:,:,1) = soln
+        vtkvars(:,:,:,2) = dcoeff
+        vtknames(1) = 'Potential (V)'
+        vtknames(2) = 'Electron_density (#/m3)'
+        call writeoutput('vtkout', vtkvars, 2, vtknames, lx, ly, lz)
+        if (g_myproc .eq. g_rootproc) call put_record('VTKDONE ', (/ 1.d0 /), 1)
 
     end select
 

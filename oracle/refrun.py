@@ -82,6 +82,30 @@ def read_output(path):
     return out
 
 
+MODE_VTK = 7
+
+
+def run_vtk(case, timeout=120):
+    """writeoutput of the reference (mode 7) on ``case`` (u0 and dcoeff as
+    the two scalars): returns (pvtr bytes, [vtr bytes of rank 0, 1, ...])."""
+    px, py, pz = case.get("p", (1, 1, 1))
+    nranks = px * py * pz
+    case = dict(case, mode=MODE_VTK)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_input(fin, case)
+        cmd = [mpiexec_path(), "-n", str(nranks), REF_DRIVER, fin, fout]
+        subprocess.run(["bash", "-c", "ulimit -s unlimited && exec \"$@\"", "_"] + cmd, check=True,
+                       timeout=timeout, cwd=td, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+        with open(os.path.join(td, "vtkout.pvtr"), "rb") as fh:
+            pvtr = fh.read()
+        vtr = []
+        for r in range(nranks):
+            with open(os.path.join(td, "vtkout", "file_%08d.vtr" % r), "rb") as fh:
+                vtr.append(fh.read())
+    return pvtr, vtr
+
+
 def run(case, timeout=600):
     """Run the reference driver on ``case``; returns a dict of float64 arrays.
 

@@ -6,7 +6,9 @@ oracle/ref_driver/*.f90) on seeded inputs and stores inputs and outputs as
 compressed .npz files. Run it in the build container (where the reference
 tree and MPICH are present):
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [vtk]
+
+(``vtk`` writes only the VTK output fixture.)
 
 The fixtures are data only: inputs, expected outputs and residual histories.
 """
@@ -55,8 +57,34 @@ def save(name, **arrays):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
-def main():
+def vtk_fixture():
+    """writeoutput (mgridlib/cartwrite.f90:455-481) on an 8x6x4 grid over a
+    2x1x2 process grid, a 6x4x10 grid over 1x2x5 and a 5x3x2 grid on one
+    rank, two seeded cell scalars each: the .pvtr and .vtr files' bytes."""
+    rng = np.random.default_rng(20261203)
+    out = {}
+    for tag, n, L, p in (("a", (8, 6, 4), (0.8, 0.3, 0.4), (2, 1, 2)),
+                         ("b", (6, 4, 10), (0.01, 0.01, 0.01), (1, 2, 5)),
+                         ("c", (5, 3, 2), (1.0, 0.7, 0.3), (1, 1, 1))):
+        u0 = rng.standard_normal(n)
+        d = rng.uniform(1e14, 1e16, n)
+        pv, vt = refrun.run_vtk(dict(n=n, len=L, p=p, bc_codes=["ZGRD"] * 6, u0=u0, dcoeff=d))
+        out[tag + "_n"] = np.array(n)
+        out[tag + "_len"] = np.array(L)
+        out[tag + "_p"] = np.array(p)
+        out[tag + "_u0"] = u0
+        out[tag + "_d"] = d
+        out[tag + "_pvtr"] = np.frombuffer(pv, dtype=np.uint8)
+        for r, v in enumerate(vt):
+            out["%s_vtr%d" % (tag, r)] = np.frombuffer(v, dtype=np.uint8)
+    save("vtk_output", **out)
+
+
+def main(argv=()):
     assert refrun.available(), "build the reference first: make -C oracle ref"
+    if "vtk" in argv:
+        vtk_fixture()
+        return
     timings = {}
 
     # --- operator, RHS and smoother on a mixed-BC random problem -------------
@@ -144,9 +172,11 @@ def main():
     rows = np.array(rows)
     save("sg1d", rows=rows, out=refrun.run_sg1d(rows))
 
+    vtk_fixture()
+
     with open(os.path.join(OUT, "ref_timings.json"), "w") as fh:
         json.dump({"host": os.uname().nodename, "nproc": os.cpu_count(), "runs": timings}, fh, indent=1)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
