@@ -230,6 +230,22 @@ def step_kernels(n, reps=20):
     s.close()
     del phi, E, te, nd, soln, eps, out
     torch.cuda.empty_cache()
+    # the whole time step (solve_manager's loop: four transport assemblies,
+    # four V-cycle solves to 1e-11, the field and the temperature) on the
+    # reference's ion_elec deck and on a 128^3 version of it
+    from mps3d_amd.plasma import ION_ELEC_DECK, PlasmaRun
+    for tag, n, reps in (("32", 32, 5), ("128", 128, 3)):
+        deck = dict(ION_ELEC_DECK)
+        deck["n"] = (n, n, n)
+        run = PlasmaRun(deck)
+        run.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.run(reps)
+        torch.cuda.synchronize()
+        res["plasma_step_%s" % tag] = {"ms_per_step": round((time.perf_counter() - t0) * 1e3 / reps, 2),
+                                       "cycles_last_step": list(run.cycles[-1]), "p": list(deck["p"])}
+        run.close()
     return res
 
 

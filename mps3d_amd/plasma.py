@@ -1,0 +1,267 @@
+"""The plasma solver's time-step loop on the device.
+
+:class:`PlasmaRun` is ``timestepping`` of plasma_pde/solve_manager.f90:363-502
+for a table chemistry (ion_elec: background gas, electrons, one ion; no
+neutral solve): per step the Poisson solve and the field, the electron
+energy solve and the electron temperature, the electron solve and the ion
+solve, each as the reference's ``*_solve`` does it -- transport assembly,
+boundary update (``*_update_bcs``), the loop of at most 20 reference V-cycles
+to an RMS residual of 1e-11, floors and ghost values -- with every field
+resident in HBM between steps. Each equation has its own context (the
+reference hierarchy on the deck's process grid, exact arithmetic), so the
+solves follow perform_vcycle's trajectory.
+
+The deck is the dictionary form of the reference's GEOM_INPUTS and
+PLASMA_INPUTS (``ION_ELEC_DECK`` below has the layout). Fields are
+float64 device tensors of shape (nz+2, ny+2, nx+2), the ghosted Fortran
+layout; :meth:`PlasmaRun.fields` returns the physical fields under the
+names of the reference's output files.
+"""
+from __future__ import annotations
+
+from .chemistry import K_BMAN, M_ELEC, etempmin_default, ion_elec, real4
+from .solver import FieldSolver
+
+ERRTOLR = 1.0e-11        # mgridlib/globalvars.f90:29
+NVCYCLES = 20            # the *_solve loops
+PI_REF = real4(3.14159265)   # plasma_pde/fundconstants.f90:8, a default-real literal
+FACES = ("left", "right", "bottom", "top", "back", "front")
+
+
+# tests/ion_elec/{GEOM_INPUTS,PLASMA_INPUTS} of the reference as a deck
+ION_ELEC_DECK = {
+    "origin": (0.0, 0.0, 0.0), "length": (0.01, 0.01, 0.01), "n": (32, 32, 32), "p": (2, 4, 1),
+    "timestep": 1.0e-11, "pressure": 133.0, "temperature": 300.0,
+    "scales": {"LENGTH": 0.01, "TIME": 1.0e-11, "ELECDEN": 1.0e14, "IONDEN": 1.0e14, "SPECDEN": 1.0e15,
+               "POT": 100.0, "ETEMP": 5800.0},
+    "init": {"N": 3.21e22, "E": 1.0e14, "I+": 1.0e14, "POT": 0.0, "ELECTEMP": 50000.0},
+    # face: codes and values of (electron density, species density, potential, electron temperature)
+    "bcs": [("left", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0),
+            ("right", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0),
+            ("bottom", "FLUX", "FLUX", "DIRC", "FLUX", 0.0, 0.0, 100.0, 0.0),
+            ("top", "FLUX", "FLUX", "DIRC", "FLUX", 0.0, 0.0, 0.0, 0.0),
+            ("back", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0),
+            ("front", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0)],
+}
+
+
+class PlasmaRun:
+    def __init__(self, deck, chem=None, device="cuda"):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(device)
+        self.chem = chem or ion_elec()
+        ch = self.chem
+        if ch.ionspecmax != ch.ionspecmin:
+            raise ValueError("one ion species (the ion_elec layout)")
+        self.n = tuple(int(v) for v in deck["n"])
+        self.p = tuple(int(v) for v in deck["p"])
+        sc = deck["scales"]
+        self.L0, self.t0 = float(sc["LENGTH"]), float(sc["TIME"])
+        self.escale_n, self.iscale = float(sc["ELECDEN"]), float(sc["IONDEN"])
+        self.potscale, self.etempscale = float(sc["POT"]), float(sc["ETEMP"])
+        # solversetup (solve_manager.f90:316)
+        self.eescale = self.escale_n * K_BMAN * self.etempscale
+        self.dt = float(deck["timestep"])
+        self.Tg, self.Pg = float(deck["temperature"]), float(deck["pressure"])
+        length = tuple(float(L) / self.L0 for L in deck["length"])   # scalelengths
+        self.length = length
+
+        def ctx():
+            s = FieldSolver(self.n, length, self.p, hierarchy="ref", precision="fp64", flux="upwind",
+                            max_cycles=NVCYCLES, tol=ERRTOLR, tol_kind="rms", arith="exact")
+            s.set_chemistry(ch)
+            return s
+
+        self.pois, self.ener, self.elec, self.ion = ctx(), ctx(), ctx(), ctx()
+        # boundary codes and parameters per equation (readsolverinputfile)
+        bcs = {b[0]: b[1:] for b in deck["bcs"]}
+        self.codes = {eq: tuple(bcs[f][q] for f in FACES) for q, eq in enumerate(("eden", "spec", "pot", "etemp"))}
+        self.params = {eq: tuple(float(bcs[f][4 + q]) for f in FACES)
+                       for q, eq in enumerate(("eden", "spec", "pot", "etemp"))}
+        # scalings of each module's *_initialize
+        tl, tl2 = self.t0 / self.L0, self.t0 / (self.L0 * self.L0)
+        self.scal = {"pot": (tl, tl2, self.t0, self.t0 / self.potscale),
+                     "eden": (tl, tl2, self.t0, self.t0 / self.escale_n),
+                     "ion": (tl, tl2, self.t0, self.t0 / self.iscale),
+                     "ener": (tl, tl2, self.t0, self.t0 / self.eescale)}
+        self.vel_scaling = tl
+        self.nemin = 1.0e11 / self.escale_n                 # eden_initialize
+        self.etempmin = etempmin_default()                  # eenergy_initialize
+        # initial state (solversetup, solve_manager.f90:294-339)
+        ini = deck["init"]
+        g = tuple(v + 2 for v in self.n)[::-1]
+        full = lambda v: torch.full(g, float(v), dtype=torch.float64, device=self.dev)  # noqa: E731
+        self.numden = torch.stack([full(ini[nm]) for nm in ch.names])
+        self.pot = full(ini["POT"])
+        self.E = torch.zeros((3,) + g, dtype=torch.float64, device=self.dev)
+        self.Te = full(ini["ELECTEMP"])
+        e = ch.especnum - 1
+        self.eps = 1.5 * self.numden[e] * K_BMAN * self.Te     # real*4 1.5 is exact
+        self.eprod = torch.zeros(g, dtype=torch.float64, device=self.dev)
+        # module solutions (scaled, ghosted): poisson_init, eden_init, ...
+        self.phi_s = self.pot / self.potscale
+        self.ne_s = self.numden[e] / self.escale_n
+        self.ni_s = self.numden[ch.ionspecmin - 1] / self.iscale
+        self.eps_s = self.eps / self.eescale
+        # *_set_bcs: face values bc_params / scale on every face
+        self.faces = {
+            "pot": [self._face_full(f, self.params["pot"][q] / self.potscale) for q, f in enumerate(FACES)],
+            "eden": [self._face_full(f, self.params["eden"][q] / self.escale_n) for q, f in enumerate(FACES)],
+            "ion": [self._face_full(f, self.params["spec"][q] / self.iscale) for q, f in enumerate(FACES)],
+            "ener": [self._face_full(f, 0.0) for f in FACES],
+        }
+        self.t = 0.0
+        self.steps = 0
+        self.cycles = []
+
+    # ------------------------------------------------------------ faces
+    def _face_shape(self, f):
+        nx, ny, nz = self.n
+        return {"left": (nz, ny), "right": (nz, ny), "bottom": (nz, nx), "top": (nz, nx),
+                "back": (ny, nx), "front": (ny, nx)}[f]
+
+    def _face_full(self, f, v):
+        return self.torch.full(self._face_shape(f), float(v), dtype=self.torch.float64, device=self.dev)
+
+    def _bc(self, G, f):
+        """Boundary-cell values of a ghosted (nz+2, ny+2, nx+2) tensor on face
+        f, as the reference's face arrays ((ny,nz), (nx,nz), (nx,ny) in
+        Fortran order)."""
+        nx, ny, nz = self.n
+        sl = {"left": (slice(1, -1), slice(1, -1), 1), "right": (slice(1, -1), slice(1, -1), nx),
+              "bottom": (slice(1, -1), 1, slice(1, -1)), "top": (slice(1, -1), ny, slice(1, -1)),
+              "back": (1, slice(1, -1), slice(1, -1)), "front": (nz, slice(1, -1), slice(1, -1))}[f]
+        return G[sl]
+
+    @staticmethod
+    def _axis(f):
+        return {"left": 0, "right": 0, "bottom": 1, "top": 1, "back": 2, "front": 2}[f]
+
+    # ------------------------------------------------------------ one step
+    def _solve(self, s, u, codes, faces, co, timederiv, dt_s):
+        s.set_bc(codes, [x.contiguous() for x in faces])
+        V = co["vel"].view(3, -1)
+        s.set_coeffs(co["dcoeff"], V[0], V[1], V[2], co["reac"], co["source"], timederiv, dt_s)
+        k, _, _, _ = s.solve(u)
+        s.fill_ghosts(u)
+        return k
+
+    def step(self):
+        torch = self.torch
+        ch = self.chem
+        e, ion = ch.especnum - 1, ch.ionspecmin - 1
+        dt_s = self.dt / self.t0
+        cyc = []
+        # poisson_solve (poissoneq.f90:387-470); the potential's faces stay
+        # (poisson_update_bcs changes only 'DZGD' electrodes)
+        co = self.pois.poisson_transport(self.numden[e].reshape(-1), self.numden[ion].reshape(-1),
+                                         self.scal["pot"], timederiv=False, dt=dt_s)
+        cyc.append(self._solve(self.pois, self.phi_s, self.codes["pot"], self.faces["pot"], co, False, dt_s))
+        self.pot = self.phi_s * self.potscale
+        # update_efield (poissoneq.f90:565-680)
+        self.pois.efield(self.phi_s, self.potscale / self.L0, False, out=self.E)
+        # eenergy_solve (eenergyeq.f90:776-840) and update_electrontemp
+        co = self.ener.eenergy_transport(self.E.reshape(-1), self.Te.reshape(-1), self.numden.reshape(-1),
+                                         self.scal["ener"], lscale=self.L0, soln=self.eps_s.reshape(-1),
+                                         dt=dt_s, Tg=self.Tg, Pg=self.Pg)
+        self._eenergy_bcs()
+        cyc.append(self._solve(self.ener, self.eps_s, self.codes["etemp"], self.faces["ener"], co, True, dt_s))
+        self.ener.electron_temp(self.eps_s, self.numden[e], self.eescale, self.etempmin, self.Te)
+        self.eps = self.eps_s * self.eescale
+        # eden_solve (edeneq.f90:446-523)
+        co = self.elec.species_transport(ch.especnum, self.E.reshape(-1), self.Te.reshape(-1),
+                                         self.numden.reshape(-1), self.scal["eden"], soln=self.ne_s.reshape(-1),
+                                         dt=dt_s, Tg=self.Tg, Pg=self.Pg, out={"prod": self.eprod.view(-1)})
+        self._eden_bcs()
+        cyc.append(self._solve(self.elec, self.ne_s, self.codes["eden"], self.faces["eden"], co, True, dt_s))
+        inner = self.ne_s[1:-1, 1:-1, 1:-1]
+        inner.copy_(torch.where(inner < self.nemin, torch.full_like(inner, self.nemin), inner))
+        self.elec.fill_ghosts(self.ne_s)
+        self.numden[e].copy_(self.ne_s * self.escale_n)
+        # ions_solve (ionseq.f90:560-629)
+        co = self.ion.species_transport(ch.ionspecmin, self.E.reshape(-1), self.Te.reshape(-1),
+                                        self.numden.reshape(-1), self.scal["ion"], soln=self.ni_s.reshape(-1),
+                                        dt=dt_s, Tg=self.Tg, Pg=self.Pg)
+        self._ion_bcs()
+        cyc.append(self._solve(self.ion, self.ni_s, self.codes["spec"], self.faces["ion"], co, True, dt_s))
+        self.numden[ion].copy_(self.ni_s * self.iscale)
+        self.t = self.t + self.dt
+        self.steps += 1
+        self.cycles.append(tuple(cyc))
+
+    def run(self, nsteps):
+        for _ in range(nsteps):
+            self.step()
+        return self
+
+    # ------------------------------------------------------------ BC updaters
+    def _const_1(self):
+        return 8.0 * K_BMAN / PI_REF / M_ELEC
+
+    def _eden_bcs(self):
+        """eden_update_bcs (edeneq.f90:203-358): FLUX faces get the thermal
+        outflow 0.25 n cbar, cbar = sqrt(8 k_B T_e / (pi m_e))."""
+        torch = self.torch
+        c1 = self._const_1()
+        for q, f in enumerate(FACES):
+            if self.codes["eden"][q] == "FLUX":
+                cbar = torch.sqrt(c1 * self._bc(self.Te, f))
+                self.faces["eden"][q] = 0.25 * self._bc(self.ne_s, f) * cbar * self.vel_scaling
+
+    def _eenergy_bcs(self):
+        """eenergy_update_bcs (eenergyeq.f90:208-392): DIRC faces 3/2 k_B n_e
+        T_wall, FLUX faces 4/3 n_eps cbar."""
+        torch = self.torch
+        c1 = self._const_1()
+        ne = self.numden[self.chem.especnum - 1]
+        for q, f in enumerate(FACES):
+            code = self.codes["etemp"][q]
+            if code == "DIRC":
+                self.faces["ener"][q] = 1.5 * K_BMAN * self._bc(ne, f) * self.params["etemp"][q] / self.eescale
+            elif code == "FLUX":
+                eb = self._bc(self.eps_s, f)
+                etemp = eb * self.eescale / (1.5 * self._bc(ne, f) * K_BMAN)
+                cbar = torch.sqrt(c1 * etemp)
+                flux = (4.0 / 3.0) * eb * cbar
+                self.faces["ener"][q] = flux * self.vel_scaling
+
+    def _ion_bcs(self):
+        """ions_update_bcs (ionseq.f90:206-476): FLUX faces carry the drift
+        outflow mu n E_normal where it leaves the domain, else zero."""
+        torch = self.torch
+        mu = float(self.chem.mobility[self.chem.ionspecmin - 1])
+        for q, f in enumerate(FACES):
+            code = self.codes["spec"][q]
+            if code == "DIRC":
+                # the reference divides by ionscale on every call (ionseq.f90:243)
+                self.faces["ion"][q] = self.faces["ion"][q] / self.iscale
+            elif code == "FLUX":
+                En = self._bc(self.E[self._axis(f)], f)
+                nb = self._bc(self.ni_s, f)
+                low = f in ("left", "bottom", "back")
+                drift = mu * nb * En * self.vel_scaling
+                out = (mu * En < 0) if low else (mu * En > 0)
+                val = -drift if low else drift
+                self.faces["ion"][q] = torch.where(out, val, torch.zeros_like(val))
+
+    # ------------------------------------------------------------ output
+    def fields(self):
+        """{reference output name: physical (nx, ny, nz) numpy array}."""
+        ch = self.chem
+
+        def host(G):
+            return G[1:-1, 1:-1, 1:-1].detach().cpu().numpy().T
+
+        out = {"%s_density (#/m3)" % nm: host(self.numden[s]) for s, nm in enumerate(ch.names)}
+        out["Potential (V)"] = host(self.pot)
+        for q, a in enumerate("XYZ"):
+            out["Electric_field_%s (V/m)" % a] = host(self.E[q])
+        out["Electron_Temp (K)"] = host(self.Te)
+        out["Electron_energy (J/m3)"] = host(self.eps)
+        out["E_prod (#/m3/s)"] = host(self.eprod)
+        return out
+
+    def close(self):
+        for s in (self.pois, self.ener, self.elec, self.ion):
+            s.close()

@@ -106,6 +106,69 @@ def run_vtk(case, timeout=120):
     return pvtr, vtr
 
 
+REF_PLASMA = os.path.join(HERE, "_ref", "mps3d_ion_elec")
+
+from mps3d_amd.plasma import ION_ELEC_DECK  # noqa: E402,F401  (the deck layout)
+
+
+def write_deck(td, deck, nsteps):
+    """GEOM_INPUTS and PLASMA_INPUTS in the layout the reference reads
+    (plasma_pde/inputs.f90:6-45, plasma_pde/solve_manager.f90:58-150), for a
+    run of exactly nsteps steps (finaltime between nsteps-1 and nsteps
+    steps) whose only outputs are output_0000000 and output_final."""
+    sep = "=" * 70
+    o, L, n, p = deck["origin"], deck["length"], deck["n"], deck["p"]
+    geom = [sep, "Geometry", sep] + ["ORIGIN_%s %r" % (a, v) for a, v in zip("XYZ", o)] + \
+        ["LENGTH_%s %r" % (a, v) for a, v in zip("XYZ", L)] + [sep, "Grid_and_processors", sep] + \
+        ["CELLS_%s %d" % (a, v) for a, v in zip("XYZ", n)] + ["PROC_%s %d" % (a, v) for a, v in zip("XYZ", p)] + [sep]
+    dt = deck["timestep"]
+    sc, ini = deck["scales"], deck["init"]
+    plasma = [sep, "Time information", sep, "timestep %r" % dt, "finaltime %r" % ((nsteps - 0.5) * dt),
+              "maxtimesteps 1000000", "outputstep 1000000", "stdoutstep 1000000",
+              sep, "Background gas parameters", sep, "PRESSURE %r" % deck["pressure"],
+              "TEMPERATURE %r" % deck["temperature"], sep, "Scales", sep] + \
+        ["%s %r" % (k, sc[k]) for k in ("LENGTH", "TIME", "ELECDEN", "IONDEN", "SPECDEN", "POT", "ETEMP")] + \
+        [sep, "Initial conditions (species,potential,electemp)", sep] + \
+        ["%s %r" % (k, v) for k, v in ini.items()] + \
+        [sep, "Boundary_conditions(elecden,specden,potential,electemp)", sep] + \
+        ["%s %s %s %s %s %r %r %r %r" % b for b in deck["bcs"]] + \
+        [sep, "Problem_specific_parameters", sep, " ".join(["0.0"] * 20)]
+    with open(os.path.join(td, "GEOM_INPUTS"), "w") as fh:
+        fh.write("\n".join(geom) + "\n")
+    with open(os.path.join(td, "PLASMA_INPUTS"), "w") as fh:
+        fh.write("\n".join(plasma) + "\n")
+
+
+def _gather(td, name, n, p):
+    """Global (nx, ny, nz) fields of output <name> from its per-rank .vtr files."""
+    from mps3d_amd import vtk
+    px, py, pz = p
+    l = (n[0] // px, n[1] // py, n[2] // pz)
+    out = {}
+    for r in range(px * py * pz):
+        pi = (r % px, (r // px) % py, r // (px * py))
+        d = vtk.read_vtr(os.path.join(td, name, "file_%08d.vtr" % r))
+        for k, v in d.items():
+            if k in ("X", "Y", "Z"):
+                continue
+            a = out.setdefault(k, np.zeros(n, order="F"))
+            a[pi[0] * l[0]:(pi[0] + 1) * l[0], pi[1] * l[1]:(pi[1] + 1) * l[1], pi[2] * l[2]:(pi[2] + 1) * l[2]] = v
+    return out
+
+
+def run_plasma(deck, nsteps, timeout=600):
+    """The reference plasma solver (oracle/_ref/mps3d_ion_elec) on ``deck``
+    for nsteps time steps: (initial fields, fields after nsteps steps), each
+    {scalar name: global (nx, ny, nz) array} as its VTK output holds them."""
+    p = deck["p"]
+    with tempfile.TemporaryDirectory() as td:
+        write_deck(td, deck, nsteps)
+        cmd = [mpiexec_path(), "-n", str(int(np.prod(p))), REF_PLASMA]
+        subprocess.run(["bash", "-c", "ulimit -s unlimited && exec \"$@\"", "_"] + cmd, check=True,
+                       timeout=timeout, cwd=td, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+        return _gather(td, "output_0000000", deck["n"], p), _gather(td, "output_final", deck["n"], p)
+
+
 def run(case, timeout=600):
     """Run the reference driver on ``case``; returns a dict of float64 arrays.
 

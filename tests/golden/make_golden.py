@@ -6,9 +6,9 @@ oracle/ref_driver/*.f90) on seeded inputs and stores inputs and outputs as
 compressed .npz files. Run it in the build container (where the reference
 tree and MPICH are present):
 
-    make -C oracle ref && python tests/golden/make_golden.py [vtk]
+    make -C oracle ref plasma && python tests/golden/make_golden.py [vtk] [plasma]
 
-(``vtk`` writes only the VTK output fixture.)
+(``vtk`` / ``plasma`` write only those fixtures.)
 
 The fixtures are data only: inputs, expected outputs and residual histories.
 """
@@ -80,10 +80,34 @@ def vtk_fixture():
     save("vtk_output", **out)
 
 
+PLASMA_STEPS = (1, 5)
+
+
+def plasma_fixture():
+    """The reference plasma solver (plasma_pde/main.f90, ion_elec chemistry)
+    on the tests/ion_elec deck at 16^3 cells over a 2x2x1 process grid: the
+    fields of its VTK output after 1 and after 5 time steps."""
+    deck = dict(refrun.ION_ELEC_DECK)
+    deck.update(n=(16, 16, 16), p=(2, 2, 1))
+    out = {"n": np.array(deck["n"]), "p": np.array(deck["p"])}
+    names = None
+    for k in PLASMA_STEPS:
+        ini, fin = refrun.run_plasma(deck, k)
+        names = sorted(fin)
+        for i, nm in enumerate(names):
+            out["s%d_f%d" % (k, i)] = fin[nm]
+    out["names"] = np.array(names)
+    out["steps"] = np.array(PLASMA_STEPS)
+    save("plasma_ion_elec16", **out)
+
+
 def main(argv=()):
     assert refrun.available(), "build the reference first: make -C oracle ref"
-    if "vtk" in argv:
-        vtk_fixture()
+    if "vtk" in argv or "plasma" in argv:
+        if "vtk" in argv:
+            vtk_fixture()
+        if "plasma" in argv:
+            plasma_fixture()
         return
     timings = {}
 
@@ -173,6 +197,7 @@ def main(argv=()):
     save("sg1d", rows=rows, out=refrun.run_sg1d(rows))
 
     vtk_fixture()
+    plasma_fixture()
 
     with open(os.path.join(OUT, "ref_timings.json"), "w") as fh:
         json.dump({"host": os.uname().nodename, "nproc": os.cpu_count(), "runs": timings}, fh, indent=1)
