@@ -12,9 +12,12 @@ namespace mps {
 // classical Gram-Schmidt, Givens rotations, lucky breakdown at 1e-25.
 //
 // One workgroup runs the whole solve. Vector updates are spread over the
-// workgroup; inner products and norms are summed by one lane in the
-// reference's memory order, so the coarse solve reproduces the reference's
-// rounding. The coarsest grid holds at most kBottomMax cells.
+// workgroup; every inner product is summed by one lane in the reference's
+// memory order (its products formed in parallel), so the coarse solve
+// reproduces the reference's rounding. The kd Gram-Schmidt products of
+// Arnoldi step kd all take the unmodified w (classical Gram-Schmidt), so
+// they run at once, one lane each, against w held in LDS. The coarsest grid
+// holds at most kBottomMax cells.
 // ------------------------------------------------------------------------
 constexpr int kBottomMax = 512;
 
@@ -48,6 +51,9 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
     T *be = sn + (m + 1);
     T *y = be + (m + 1);
     __shared__ T sh[2];
+    __shared__ int cmap[kBottomMax];   // interior cell q (i fastest) -> ghosted index
+    __shared__ T pr[kBottomMax];       // products of one inner product
+    __shared__ T wsh[kBottomMax];      // w of the current Arnoldi step, by cell
     const int tid = threadIdx.x, nt = blockDim.x;
 
     auto cidx = [&](int q, int &i, int &j, int &k) {
@@ -96,14 +102,17 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
             __syncthreads();
         }
     };
-    auto dot = [&](const T *a, const T *c2) {  // one lane, reference order
+    for (int q = tid; q < n; q += nt) {
+        int i, j, k;
+        cmap[q] = (int)cidx(q, i, j, k);
+    }
+    __syncthreads();
+    auto dot = [&](const T *a, const T *c2) {  // products in parallel, one lane sums in order
+        for (int q = tid; q < n; q += nt) pr[q] = a[cmap[q]] * c2[cmap[q]];
+        __syncthreads();
         if (tid == 0) {
             T s = T(0);
-            for (int q = 0; q < n; ++q) {
-                int i, j, k;
-                long long c = cidx(q, i, j, k);
-                s = s + a[c] * c2[c];
-            }
+            for (int q = 0; q < n; ++q) s = s + pr[q];
             sh[0] = s;
         }
         __syncthreads();
@@ -173,15 +182,23 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
             zero(av);
             apply(av, vj);
             precond(w, av);
-            for (long long q = tid; q < ng; q += nt) wj[q] = w[q];
+            // classical Gram-Schmidt: h_ii = <w, v_ii> for ii = 1..kd, one
+            // lane each, then wj = w - h_1 v_1 - h_2 v_2 - ... in that order
+            for (int q = tid; q < n; q += nt) wsh[q] = w[cmap[q]];
             __syncthreads();
-            for (int ii = 1; ii <= kd; ++ii) {  // classical Gram-Schmidt
+            for (int ii = 1 + tid; ii <= kd; ii += nt) {
                 const T *vi = V + ng * (long long)(ii - 1);
-                T h = dot(w, vi);
-                if (tid == 0) HM(ii, kd) = h;
-                for (long long q = tid; q < ng; q += nt) wj[q] = wj[q] - h * vi[q];
-                __syncthreads();
+                T h = T(0);
+                for (int q = 0; q < n; ++q) h = h + wsh[q] * vi[cmap[q]];
+                HM(ii, kd) = h;
             }
+            __syncthreads();
+            for (long long q = tid; q < ng; q += nt) {
+                T t = w[q];
+                for (int ii = 1; ii <= kd; ++ii) t = t - HM(ii, kd) * V[ng * (long long)(ii - 1) + q];
+                wj[q] = t;
+            }
+            __syncthreads();
             T hn = sqrt(dot(wj, wj));
             int nanflag = 0;
             if (hn > T(1e-25)) {
