@@ -139,8 +139,9 @@ class PlasmaRun:
         return {"left": 0, "right": 0, "bottom": 1, "top": 1, "back": 2, "front": 2}[f]
 
     # ------------------------------------------------------------ one step
-    def _solve(self, s, u, codes, faces, co, timederiv, dt_s):
-        s.set_bc(codes, [x.contiguous() for x in faces])
+    def _solve(self, s, u, codes, faces, co, timederiv, dt_s, faces_changed=True):
+        if faces_changed:
+            s.set_bc(codes, [x.contiguous() for x in faces])
         V = co["vel"].view(3, -1)
         s.set_coeffs(co["dcoeff"], V[0], V[1], V[2], co["reac"], co["source"], timederiv, dt_s)
         k, _, _, _ = s.solve(u)
@@ -157,7 +158,8 @@ class PlasmaRun:
         # (poisson_update_bcs changes only 'DZGD' electrodes)
         co = self.pois.poisson_transport(self.numden[e].reshape(-1), self.numden[ion].reshape(-1),
                                          self.scal["pot"], timederiv=False, dt=dt_s)
-        cyc.append(self._solve(self.pois, self.phi_s, self.codes["pot"], self.faces["pot"], co, False, dt_s))
+        cyc.append(self._solve(self.pois, self.phi_s, self.codes["pot"], self.faces["pot"], co, False, dt_s,
+                               faces_changed=self.steps == 0))
         self.pot = self.phi_s * self.potscale
         # update_efield (poissoneq.f90:565-680)
         self.pois.efield(self.phi_s, self.potscale / self.L0, False, out=self.E)
