@@ -20,6 +20,10 @@ namespace mps {
 // holds at most kBottomMax cells.
 // ------------------------------------------------------------------------
 constexpr int kBottomMax = 512;
+// Krylov dimensions up to this keep the Hessenberg matrix and the Givens
+// arrays in LDS (the reference's coarse solves use m = n, which is 8-64 on
+// the usual process grids); larger m keep them in the work buffer
+constexpr int kBottomLdsM = 64;
 
 template <typename T>
 __host__ __device__ inline long long bottom_work_elems(int m, long long ng)
@@ -45,8 +49,11 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
     T *w = av + ng;
     T *wj = w + ng;
     T *mb = wj + ng;
-    T *H = mb + ng;
-    T *cs = H + (long long)(m + 1) * m;
+    __shared__ T sH[(kBottomLdsM + 1) * kBottomLdsM];
+    __shared__ T sg[4 * (kBottomLdsM + 1)];
+    const bool lds = m <= kBottomLdsM;
+    T *H = lds ? sH : mb + ng;
+    T *cs = lds ? sg : H + (long long)(m + 1) * m;
     T *sn = cs + (m + 1);
     T *be = sn + (m + 1);
     T *y = be + (m + 1);
@@ -216,11 +223,17 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
             }
             if (tid == 0) {  // triangularize (gmres.f90:159-195)
                 HM(kd + 1, kd) = hn;
+                // the rotated entry ii+1 becomes the next step's entry ii:
+                // carried in a register, the chain touches memory only to
+                // load the untouched entries and store the finished ones
+                T hc = HM(1, kd);
                 for (int ii = 1; ii <= kd - 1; ++ii) {
-                    T t = cs[ii - 1] * HM(ii, kd) + sn[ii - 1] * HM(ii + 1, kd);
-                    HM(ii + 1, kd) = -sn[ii - 1] * HM(ii, kd) + cs[ii - 1] * HM(ii + 1, kd);
+                    const T hn1 = HM(ii + 1, kd), c = cs[ii - 1], s = sn[ii - 1];
+                    const T t = c * hc + s * hn1;
+                    hc = -s * hc + c * hn1;
                     HM(ii, kd) = t;
                 }
+                if (kd > 1) HM(kd, kd) = hc;
                 T rr = HM(kd, kd), ss = HM(kd + 1, kd), c, s;
                 if (rr == T(0)) {
                     c = T(0);
