@@ -324,7 +324,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the whole 257^3 domain fits one MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+              "(multi-GPU sharding is not implemented)", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
