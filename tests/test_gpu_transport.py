@@ -192,3 +192,52 @@ def test_transport_argument_errors(gpu):
             s.species_transport(4, ef, te, nd, SCALING, soln=soln)      # specnum out of range
         with pytest.raises(M.MpsError):
             s.species_transport(2, ef, te, nd, SCALING, soln=soln, dt=0.0)  # time term with dt = 0
+
+
+def test_transport_and_field_at_257_cubed(gpu):
+    """The full 257^3 grid on device tensors, through properties that do not
+    depend on size: a linear potential between DIRC plates gives the exact
+    uniform field (ghost faces included); uniform densities and temperature
+    in that field give uniform drift velocities mu E, the Arrhenius
+    production k n_N n_E at every interior cell, scaled zeros plus the time
+    term in the ghost cells, and the Poisson source of the space charge."""
+    import math
+
+    import torch
+    n = 257
+    L = 0.02
+    with FieldSolver(n, L) as s:
+        ch = ion_elec()
+        s.set_chemistry(ch)
+        V0 = 1000.0
+        # phi = V0 z / L at cell centres; DIRC walls 0 and V0
+        h = L / n
+        z = (torch.arange(n + 2, dtype=torch.float64, device="cuda") - 0.5) * h
+        phi = (V0 * z / L).view(-1, 1, 1).expand(n + 2, n + 2, n + 2).contiguous()
+        nx = n
+        faces = [0.0, 0.0, 0.0, 0.0, np.zeros((nx, nx)), np.full((nx, nx), V0)]
+        s.set_bc(("ZGRD", "ZGRD", "ZGRD", "ZGRD", "DIRC", "DIRC"), faces)
+        E = torch.zeros((3, n + 2, n + 2, n + 2), dtype=torch.float64, device="cuda")
+        s.efield(phi, 1.0, False, out=E)
+        Ez = E[2, 1:-1, 1:-1, 1:-1]
+        assert torch.allclose(Ez, torch.full_like(Ez, -V0 / L), rtol=1e-12, atol=0)
+        assert float(E[0].abs().max()) == 0.0 and float(E[1].abs().max()) == 0.0
+        g = (n + 2) ** 3
+        Ea = ch.k_arrh[0, 2]
+        te = torch.full((g,), Ea, dtype=torch.float64, device="cuda")
+        nN, nE, nI = 3.0e22, 2.0e16, 1.0e16
+        nd = torch.cat([torch.full((g,), v, dtype=torch.float64, device="cuda") for v in (nN, nE, nI)])
+        soln = torch.full((g,), 0.5, dtype=torch.float64, device="cuda")
+        o = s.species_transport(2, E.reshape(-1), te, nd, SCALING, soln=soln, dt=0.25)
+        vel = o["vel"].view(3, n + 2, n + 2, n + 2)
+        vz = vel[2, 1:-1, 1:-1, 1:-1]
+        assert torch.equal(vz, ch.mobility[1] * Ez * SCALING[0])
+        src = o["source"].view(n + 2, n + 2, n + 2)
+        prod = (ch.k_arrh[0, 0] / math.e) * nN * nE
+        inner = src[1:-1, 1:-1, 1:-1]
+        assert torch.allclose(inner, torch.full_like(inner, prod * SCALING[3] + 0.5 / 0.25), rtol=1e-14, atol=0)
+        assert float(src[0].sub(2.0).abs().max()) == 0.0 and float(src[:, 0].sub(2.0).abs().max()) == 0.0
+        p = s.poisson_transport(nd[g:2 * g], nd[2 * g:], SCALING, soln=soln, dt=0.25)
+        ps = p["source"].view(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1]
+        rho = (nI - nE) * 1.602176e-19 / 8.854188e-12 * SCALING[3] + 0.5 / 0.25
+        assert torch.allclose(ps, torch.full_like(ps, rho), rtol=1e-14, atol=0)
