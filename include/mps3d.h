@@ -231,6 +231,19 @@ typedef struct {
     const double *dcoeff;           /* (nspecies) diffusion coefficient */
     const double *molmass;          /* (nspecies) kg */
     const double *charge;           /* (nspecies) charge number */
+    /* Field-dependent transport (getspecmobility / getspecdcoeff of
+     * plasma_pde/chemistry/H2_O2_chem.f90:532-672); all four NULL = the
+     * constants above. mob_kind(s): 0 mobility(s); 1 electron fit
+     * mobility(s) / sum_j n_j / exp(a_j L^2 + b_j L + c_j), L = ln(T_e / 11604.5 K);
+     * 2 reduced ion mobility mobility(s) N_atm / sum_j n_j / K_j, N_atm =
+     * 101325 Pa / k_B / 300 K. The sums run over the partners j with
+     * mob_terms(j, s) != 0, in species order, with mob_coef(1:3, j, s) =
+     * (a, b, c) or (K, -, -). dcoeff_kind(s): 0 dcoeff(s); 1 k_B T_e mu /
+     * (Z e); 2 k_B T_g mu / (Z e). */
+    const int *mob_kind;            /* (nspecies) */
+    const int *mob_terms;           /* (nspecies, nspecies), Fortran order */
+    const double *mob_coef;         /* (3, nspecies, nspecies), Fortran order */
+    const int *dcoeff_kind;         /* (nspecies) */
 } mps_chem;
 
 /* initializechemistry (ion_elec_chem.f90:104-109, with the tables of

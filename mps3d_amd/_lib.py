@@ -49,7 +49,8 @@ class Chem(C.Structure):
     _fields_ = [("nspecies", C.c_int), ("nreac", C.c_int), ("especnum", C.c_int), ("bgspecnum", C.c_int),
                 ("ionspecmin", C.c_int), ("ionspecmax", C.c_int),
                 ("reactants", _vp), ("products", _vp), ("k_arrh", _vp), ("elecenergy", _vp),
-                ("mobility", _vp), ("dcoeff", _vp), ("molmass", _vp), ("charge", _vp)]
+                ("mobility", _vp), ("dcoeff", _vp), ("molmass", _vp), ("charge", _vp),
+                ("mob_kind", _vp), ("mob_terms", _vp), ("mob_coef", _vp), ("dcoeff_kind", _vp)]
 
 
 class Opts(C.Structure):

@@ -3,10 +3,11 @@
 A :class:`Chemistry` holds what a reference ``chem_module`` defines in its
 ``assignreactions`` and ``setspecparams`` (species numbers 1-based,
 ``reactants``/``products`` as (nreac, nspecies), ``k_arrh`` as (nreac, 3)
-rows of A, alpha, Ea): Arrhenius rates in the electron temperature and one
-constant mobility and diffusion coefficient per species, the form of
-``plasma_pde/chemistry/ion_elec_chem.f90``. ``FieldSolver.set_chemistry``
-copies it to the device.
+rows of A, alpha, Ea): Arrhenius rates in the electron temperature, and per
+species a constant mobility and diffusion coefficient (the form of
+``plasma_pde/chemistry/ion_elec_chem.f90``, :func:`ion_elec`) or the
+field-dependent models of ``plasma_pde/chemistry/H2_O2_chem.f90``
+(:func:`h2_o2`). ``FieldSolver.set_chemistry`` copies it to the device.
 """
 from __future__ import annotations
 
@@ -48,6 +49,11 @@ class Chemistry:
     bgspecnum: int
     ionspecmin: int
     ionspecmax: int
+    # field-dependent transport (None: the constants above); see mps_chem
+    mob_kind: np.ndarray | None = None     # (nspecies,) 0 constant, 1 electron fit, 2 reduced ion mobility
+    mob_terms: np.ndarray | None = None    # (nspecies partner, nspecies)
+    mob_coef: np.ndarray | None = None     # (3, nspecies partner, nspecies)
+    dcoeff_kind: np.ndarray | None = None  # (nspecies,) 0 constant, 1 k_B T_e mu/(Z e), 2 k_B T_g mu/(Z e)
 
     @property
     def nspecies(self):
@@ -59,7 +65,7 @@ class Chemistry:
 
     def arrays(self):
         """The tables as the C ABI takes them (Fortran order, int32/float64)."""
-        return {
+        out = {
             "reactants": np.asfortranarray(self.reactants, dtype=np.int32),
             "products": np.asfortranarray(self.products, dtype=np.int32),
             "k_arrh": np.ascontiguousarray(self.k_arrh, dtype=np.float64),  # k_arrh(3, nreac) in memory
@@ -69,6 +75,12 @@ class Chemistry:
             "molmass": np.ascontiguousarray(self.molmass, dtype=np.float64),
             "charge": np.ascontiguousarray(self.charge, dtype=np.float64),
         }
+        if self.mob_kind is not None:
+            out["mob_kind"] = np.ascontiguousarray(self.mob_kind, dtype=np.int32)
+            out["mob_terms"] = np.asfortranarray(self.mob_terms, dtype=np.int32)
+            out["mob_coef"] = np.asfortranarray(self.mob_coef, dtype=np.float64)
+            out["dcoeff_kind"] = np.ascontiguousarray(self.dcoeff_kind, dtype=np.int32)
+        return out
 
     def struct(self):
         """(mps_chem structure, arrays it points into: keep both alive)."""
@@ -113,6 +125,78 @@ def ion_elec():
         molmass=np.array([4.0 * M_PROT, M_ELEC, 4.0 * M_PROT]),
         charge=np.array([0.0, -1.0, 1.0]),
         especnum=2, bgspecnum=1, ionspecmin=3, ionspecmax=3)
+
+
+def h2_o2():
+    """The reference's default chemistry, plasma_pde/chemistry/H2_O2_chem.f90
+    (build/makefile:1): H2, O2 (background), E, H+, H2+, O+, O2+, O-, H, O;
+    15 Arrhenius reactions in T_e (every reaction of assignreactions has
+    isratearrh = .true.), and field-dependent transport.
+
+    The reaction tables are the doubles that assignreactions leaves in
+    reactants, products, k_arrh and elecenergy (:30-384) when compiled: its
+    constant expressions such as 6.5023d-9*(cm_to_m**3)/(eVtoK**0.48931) are
+    folded by the compiler from single-precision literals, so they are given
+    here as the values themselves, taken from the reference build
+    (oracle/ref_driver/ref_chem.f90); a test checks them against it.
+    Transport (getspecmobility, getspecdcoeff, :532-672): the electron
+    mobility is the fit -1 / sum_j n_j / exp(a_j L^2 + b_j L + c_j) over H2
+    and O2 (L = ln T_e[eV], single-precision coefficients), ion mobilities are
+    N_atm / sum_j n_j / (K_j cm^2) (O-: negative), charged species diffuse by
+    the Einstein relation (electrons at T_e, ions at T_g), H and O with
+    constants."""
+    reactants = np.array([[0, 0, 1, 0, 0, 0, 0, 0, 1, 0], [1, 0, 1, 0, 0, 0, 0, 0, 0, 0], [0, 0, 1, 0, 1, 0, 0, 0, 0, 0], [0, 0, 1, 0, 1, 0, 0, 0, 0, 0], [0, 0, 0, 0, 1, 0, 0, 0, 1, 0], [1, 0, 0, 1, 0, 0, 0, 0, 0, 0], [1, 0, 1, 0, 0, 0, 0, 0, 0, 0], [1, 0, 1, 0, 0, 0, 0, 0, 0, 0], [0, 1, 1, 0, 0, 0, 0, 0, 0, 0], [0, 1, 1, 0, 0, 0, 0, 0, 0, 0], [0, 1, 1, 0, 0, 0, 0, 0, 0, 0], [0, 0, 1, 0, 0, 0, 0, 0, 0, 1], [0, 0, 0, 0, 0, 0, 1, 1, 0, 0], [0, 0, 0, 0, 0, 1, 0, 1, 0, 0], [0, 0, 1, 0, 0, 0, 0, 1, 0, 0]], dtype=np.int32)
+    products = np.array([[0, 0, 2, 1, 0, 0, 0, 0, 0, 0], [0, 0, 2, 1, 0, 0, 0, 0, 1, 0], [0, 0, 1, 1, 0, 0, 0, 0, 1, 0], [0, 0, 2, 2, 0, 0, 0, 0, 0, 0], [1, 0, 0, 1, 0, 0, 0, 0, 0, 0], [0, 0, 0, 0, 1, 0, 0, 0, 1, 0], [0, 0, 2, 0, 1, 0, 0, 0, 0, 0], [0, 0, 1, 0, 0, 0, 0, 0, 2, 0], [0, 0, 2, 0, 0, 0, 1, 0, 0, 0], [0, 0, 1, 0, 0, 0, 0, 0, 0, 2], [0, 0, 0, 0, 0, 0, 0, 1, 0, 1], [0, 0, 2, 0, 0, 1, 0, 0, 0, 0], [0, 1, 0, 0, 0, 0, 0, 0, 0, 1], [0, 0, 0, 0, 0, 0, 0, 0, 0, 2], [0, 0, 2, 0, 0, 0, 0, 0, 0, 1]], dtype=np.int32)
+    k_arrh = np.array([
+        [6.671210258802429e-17, 0.48930999636650085, 149624.359375],
+        [4.673050547002412e-16, 0.4445599913597107, 437818.75],
+        [6.780687508393681e-14, 0.04876000061631203, 112450.8515625],
+        [1.122866241117621e-16, 0.31393998861312866, 270371.5],
+        [8.999999396502984e-16, 0.0, 0.0],
+        [1.1899999202042835e-28, 0.0, 0.0],
+        [6.269267675269048e-15, 0.17156000435352325, 232987.484375],
+        [1.8635879765620025e-08, -1.236680030822754, 146128.84375],
+        [8.354669297014652e-18, 0.5, 146216.703125],
+        [4.229999716356402e-15, 0.0, 64985.19921875],
+        [8.79999940991403e-17, 0.0, 51059.80078125],
+        [1.2852985741823762e-17, 0.699999988079071, 150858.5],
+        [1.9899998665601042e-13, 0.0, 0.0],
+        [2.6599998216331046e-13, 0.0, 0.0],
+        [1.999999865889552e-13, 0.0, 63824.75],
+    ])
+    elecenergy = np.array([-12.89365005493164, -37.72835922241211, -9.690279960632324, -23.29884910583496, 0.0, 0.0, -20.07733917236328, -12.592430114746094, -12.600000381469727, -6.400000095367432, -3.5999999046325684, -13.600000381469727, 0.0, 0.0, -5.5])
+    cm2 = real4(0.01) * real4(0.01)
+    ns = 10
+    H2, O2, E, Hp, H2p, Op, O2p, Om, H, O = range(ns)
+    mob = np.zeros(ns)
+    dco = np.zeros(ns)
+    mob_kind = np.zeros(ns, dtype=np.int32)
+    dk = np.zeros(ns, dtype=np.int32)
+    terms = np.zeros((ns, ns), dtype=np.int32)
+    coef = np.zeros((3, ns, ns))
+    # electrons: n_H2 / exp(0.061 L^2 - 0.342 L + 55.89) + n_O2 / exp(0.031 L^2 - 0.658 L + 56.71)
+    mob_kind[E], mob[E], dk[E] = 1, -1.0, 1
+    for j, (a, b, c) in ((H2, (0.061, 0.342, 55.89)), (O2, (0.031, 0.658, 56.71))):
+        terms[j, E] = 1
+        coef[:, j, E] = (real4(a), -real4(b), real4(c))
+    # ions: reduced mobilities (cm^2/V/s) in H2 and O2
+    for sp, sign, kh2, ko2 in ((Hp, 1.0, 15.0, 15.0), (H2p, 1.0, 13.0, 13.0), (Op, 1.0, 11.0, 3.5),
+                               (O2p, 1.0, 8.0, 2.1), (Om, -1.0, 24.0, 3.8)):
+        mob_kind[sp], mob[sp], dk[sp] = 2, sign, 2
+        terms[H2, sp] = terms[O2, sp] = 1
+        coef[0, H2, sp] = kh2 * cm2
+        coef[0, O2, sp] = ko2 * cm2
+    dco[H], dco[O] = real4(0.03), real4(0.0075)
+    mp = M_PROT
+    return Chemistry(
+        names=("H2", "O2", "E", "H+", "H2+", "O+", "O2+", "O-", "H", "O"),
+        reactants=reactants, products=products, k_arrh=k_arrh, elecenergy=elecenergy,
+        mobility=mob, dcoeff=dco,
+        molmass=np.array([2.0 * mp, 32.0 * mp, M_ELEC, 1.0 * mp, 2.0 * mp, 16.0 * mp, 32.0 * mp, 16.0 * mp, 1.0 * mp,
+                          16.0 * mp]),
+        charge=np.array([0.0, 0.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 0.0, 0.0]),
+        especnum=3, bgspecnum=1, ionspecmin=4, ionspecmax=8,
+        mob_kind=mob_kind, mob_terms=terms, mob_coef=coef, dcoeff_kind=dk)
 
 
 def etempmin_default():

@@ -2120,6 +2120,23 @@ extern "C" int mps_set_chemistry(mps_ctx *c, const mps_chem *chem)
         h.mass[j] = chem->molmass[j];
         h.charge[j] = chem->charge[j];
     }
+    const bool models = chem->mob_kind || chem->mob_terms || chem->mob_coef || chem->dcoeff_kind;
+    if (models && !(chem->mob_kind && chem->mob_terms && chem->mob_coef && chem->dcoeff_kind))
+        return fail(MPS_ERR_ARG, "mob_kind, mob_terms, mob_coef and dcoeff_kind go together");
+    for (int s2 = 0; models && s2 < ns; ++s2) {
+        const int mk = chem->mob_kind[s2], dk = chem->dcoeff_kind[s2];
+        if (mk < 0 || mk > 2 || dk < 0 || dk > 2) return fail(MPS_ERR_ARG, "mob_kind and dcoeff_kind must be 0..2");
+        if (dk != 0 && chem->charge[s2] == 0.0) return fail(MPS_ERR_ARG, "Einstein diffusion of a neutral species");
+        h.mkind[s2] = mk;
+        h.dkind[s2] = dk;
+        int nterms = 0;
+        for (int j = 0; j < ns; ++j) {
+            h.mterm[s2][j] = chem->mob_terms[j + (size_t)ns * s2] != 0;
+            nterms += h.mterm[s2][j];
+            for (int q = 0; q < 3; ++q) h.mco[s2][j][q] = chem->mob_coef[q + 3 * (j + (size_t)ns * s2)];
+        }
+        if (mk != 0 && nterms == 0) return fail(MPS_ERR_ARG, "a mobility fit needs at least one partner species");
+    }
     try {
         if (!c->chem) c->chem = dalloc<ChemDev>(1);
         c->chem_host = h;

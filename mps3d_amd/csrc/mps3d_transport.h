@@ -38,7 +38,46 @@ struct ChemDev {
     double eel[kChemMaxReac];                   // electron energy change, eV
     double mob[kChemMaxSpecies], dco[kChemMaxSpecies];
     double mass[kChemMaxSpecies], charge[kChemMaxSpecies];
+    // field-dependent transport: mobility and diffusion models per species
+    int mkind[kChemMaxSpecies];                     // 0 constant, 1 electron fit, 2 reduced ion mobility
+    int mterm[kChemMaxSpecies][kChemMaxSpecies];    // [s][j]: partner j enters s's sum
+    double mco[kChemMaxSpecies][kChemMaxSpecies][3];
+    int dkind[kChemMaxSpecies];                     // 0 constant, 1 Einstein with T_e, 2 with T_g
 };
+
+constexpr double kEvToK = 11604.5;                  // plasma_pde/fundconstants.f90:9
+constexpr double kNatm = 101325.0 / kBoltz / 300.0;  // Patm / k_BMAN / Troom of getspecmobility
+
+// getspecmobility (H2_O2_chem.f90:600-672; ion_elec_chem.f90:222-242 for
+// the constants) of species s at cell c
+__device__ __forceinline__ double spec_mobility(const ChemDev &C, int s, const double *__restrict__ nd, long long ng,
+                                               long long c, double Te)
+{
+    const int kind = C.mkind[s];
+    if (kind == 0) return C.mob[s];
+    double sum = 0.0;
+    bool first = true;
+    const double L = kind == 1 ? log(Te / kEvToK) : 0.0;
+    for (int j = 0; j < C.ns; ++j) {
+        if (!C.mterm[s][j]) continue;
+        const double nj = nd[(long long)j * ng + c];
+        const double inv = kind == 1 ? nj / exp(C.mco[s][j][0] * (L * L) + C.mco[s][j][1] * L + C.mco[s][j][2])
+                                     : nj / C.mco[s][j][0];
+        sum = first ? inv : sum + inv;
+        first = false;
+    }
+    const double num = kind == 2 ? C.mob[s] * kNatm : C.mob[s];
+    return num / sum;
+}
+
+// getspecdcoeff: constant, or the Einstein relation k_B T mu / (Z e)
+__device__ __forceinline__ double spec_dcoeff(const ChemDev &C, int s, double mu, double Te, double Tg)
+{
+    const int kind = C.dkind[s];
+    if (kind == 0) return C.dco[s];
+    const double T = kind == 1 ? Te : Tg;
+    return kBoltz * T * mu / (C.charge[s] * kEcharge);
+}
 
 // x**m for an integer m >= 0 by repeated squaring (the integer power of
 // Fortran's real**integer: x**0 = 1, x**1 = x, x**2 = x*x, x**3 = x*(x*x))
@@ -146,11 +185,11 @@ __global__ void __launch_bounds__(256) k_transport(Lev<double> L, const ChemDev 
         // mobility, diffusion and Arrhenius rates do not depend on it)
         const double Te = in.te[c];
         if (KIND == 0) {
-            const double mu = C.mob[sp];
+            const double mu = spec_mobility(C, sp, in.nd, ng, c, Te);
             vx = mu * Ex;
             vy = mu * Ey;
             vz = mu * Ez;
-            dc = C.dco[sp];
+            dc = spec_dcoeff(C, sp, mu, Te, in.Tg);
             // getspecproduction (ion_elec_chem.f90:111-156)
             double prod = 0.0;
             for (int r = 0; r < C.nr; ++r) {
@@ -163,7 +202,7 @@ __global__ void __launch_bounds__(256) k_transport(Lev<double> L, const ChemDev 
             if (out.prod) out.prod[c] = src;
         } else {
             const int e = C.e;
-            const double mu = C.mob[e], D = C.dco[e];
+            const double mu = spec_mobility(C, e, in.nd, ng, c, Te), D = spec_dcoeff(C, e, mu, Te, in.Tg);
             const double five_by_three = 5.0 / 3.0;
             vx = five_by_three * mu * Ex;
             vy = five_by_three * mu * Ey;

@@ -27,6 +27,8 @@ module mps3d_gpu
     type, bind(C) :: mps_chem
         integer(c_int) :: nspecies, nreac, especnum, bgspecnum, ionspecmin, ionspecmax
         type(c_ptr) :: reactants, products, k_arrh, elecenergy, mobility, dcoeff, molmass, charge
+        type(c_ptr) :: mob_kind = c_null_ptr, mob_terms = c_null_ptr, mob_coef = c_null_ptr
+        type(c_ptr) :: dcoeff_kind = c_null_ptr
     end type mps_chem
 
     type, bind(C) :: mps_grid

@@ -886,6 +886,42 @@ static double orc_inelterm(const orc_chem *ch, double Te, const double *specden)
     return inelterm;
 }
 
+/* getspecmobility (H2_O2_chem.f90:600-672; ion_elec_chem.f90:222-242):
+ * kind 1 (electrons) mobility / sum_j specarray(j)/exp(a L^2 + b L + c),
+ * L = log(Te/eVtoK); kind 2 (ions) mobility*Natm / sum_j specarray(j)/K_j,
+ * Natm = Patm/k_BMAN/Troom; kind 0 the constant. Specnum 1-based. */
+static double orc_spec_mobility(const orc_chem *ch, int specnum, const double *sa, double Te)
+{
+    int s = specnum - 1, ns = ch->nspecies;
+    int kind = ch->mob_kind ? ch->mob_kind[s] : 0;
+    if (kind == 0) return ch->mobility[s];
+    double Te_in_eV = Te / 11604.5;
+    double log_Te = log(Te_in_eV);
+    double Natm = 101325.0 / ORC_KBMAN / 300.0;
+    double mob_inv = 0.0;
+    int first = 1;
+    for (int j = 0; j < ns; ++j) {
+        if (!ch->mob_terms[j + ns * s]) continue;
+        const double *co = ch->mob_coef + 3 * (j + ns * s);
+        double inv = kind == 1 ? sa[j] / exp(co[0] * (log_Te * log_Te) + co[1] * log_Te + co[2]) : sa[j] / co[0];
+        mob_inv = first ? inv : mob_inv + inv;
+        first = 0;
+    }
+    return kind == 2 ? ch->mobility[s] * Natm / mob_inv : ch->mobility[s] / mob_inv;
+}
+
+/* getspecdcoeff: kind 1 k_BMAN*Te*mob/(Z*ECHARGE), kind 2 the same at Tg,
+ * kind 0 the constant */
+static double orc_spec_dcoeff(const orc_chem *ch, int specnum, const double *sa, double Te, double Tg)
+{
+    int s = specnum - 1;
+    int kind = ch->dcoeff_kind ? ch->dcoeff_kind[s] : 0;
+    if (kind == 0) return ch->dcoeff[s];
+    double mob = orc_spec_mobility(ch, specnum, sa, Te);
+    double T = kind == 1 ? Te : Tg;
+    return ORC_KBMAN * T * mob / (ch->charge[s] * ORC_ECHARGE);
+}
+
 static void orc_specarray(const orc_chem *ch, const double *numden, size_t ng, size_t c, double *sa)
 {
     for (int s = 0; s < ch->nspecies; ++s) sa[s] = numden[(size_t)s * ng + c];
@@ -909,7 +945,6 @@ void orc_species_transport(const orc_problem *p, const orc_chem *ch, int specnum
                            const double *efield, const double *electemp, const double *numden, double Tg,
                            const double *scaling, const double *soln, orc_transport_out out)
 {
-    (void)Tg;
     lev_t L = finest(p);
     size_t ng = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
     for (size_t c = 0; c < 3 * ng; ++c) out.vel[c] = 0.0;
@@ -920,10 +955,9 @@ void orc_species_transport(const orc_problem *p, const orc_chem *ch, int specnum
             for (int i = 1; i <= L.nx; ++i) {
                 size_t c = IDX(&L, i, j, k);
                 orc_specarray(ch, numden, ng, c, sa);
-                /* getspecmobility / getspecdcoeff of ion_elec_chem: constants */
-                double mu = ch->mobility[specnum - 1];
+                double mu = orc_spec_mobility(ch, specnum, sa, electemp[c]);
                 for (int q = 0; q < 3; ++q) out.vel[q * ng + c] = mu * efield[q * ng + c];
-                out.dcoeff[c] = ch->dcoeff[specnum - 1];
+                out.dcoeff[c] = orc_spec_dcoeff(ch, specnum, sa, electemp[c], Tg);
                 double specprod = orc_specproduction(ch, specnum, electemp[c], sa);
                 out.source[c] = out.source[c] + specprod;
                 if (out.prod) out.prod[c] = out.source[c];
@@ -969,7 +1003,9 @@ void orc_eenergy_transport(const orc_problem *p, const orc_chem *ch, double dt, 
         for (int j = 1; j <= L.ny; ++j)
             for (int i = 1; i <= L.nx; ++i, ++t) {
                 size_t c = IDX(&L, i, j, k);
-                double mu_e = ch->mobility[e], D_e = ch->dcoeff[e];
+                orc_specarray(ch, numden, ng, c, sa);
+                double mu_e = orc_spec_mobility(ch, e + 1, sa, electemp[c]);
+                double D_e = orc_spec_dcoeff(ch, e + 1, sa, electemp[c], Tg);
                 double jx = orc_jheat_dir(elecden, efield, c, st[0], mu_e, D_e, dxl[0]);
                 double jy = orc_jheat_dir(elecden, efield + ng, c, st[1], mu_e, D_e, dxl[1]);
                 double jz = orc_jheat_dir(elecden, efield + 2 * ng, c, st[2], mu_e, D_e, dxl[2]);
@@ -992,7 +1028,8 @@ void orc_eenergy_transport(const orc_problem *p, const orc_chem *ch, double dt, 
         for (int j = 1; j <= L.ny; ++j)
             for (int i = 1; i <= L.nx; ++i, ++t) {
                 size_t c = IDX(&L, i, j, k);
-                double nu = -(ORC_ECHARGE / m_e / ch->mobility[e]);
+                orc_specarray(ch, numden, ng, c, sa);
+                double nu = -(ORC_ECHARGE / m_e / orc_spec_mobility(ch, e + 1, sa, electemp[c]));
                 elastic[t] = 3.0 * ORC_KBMAN * elecden[c] * (electemp[c] - Tg) * (m_e / m_bg) * nu;
             }
     for (size_t c = 0; c < 3 * ng; ++c) out.vel[c] = 0.0;
@@ -1003,9 +1040,10 @@ void orc_eenergy_transport(const orc_problem *p, const orc_chem *ch, double dt, 
         for (int j = 1; j <= L.ny; ++j)
             for (int i = 1; i <= L.nx; ++i, ++t) {
                 size_t c = IDX(&L, i, j, k);
-                double mu_e = ch->mobility[e];
+                orc_specarray(ch, numden, ng, c, sa);
+                double mu_e = orc_spec_mobility(ch, e + 1, sa, electemp[c]);
                 for (int q = 0; q < 3; ++q) out.vel[q * ng + c] = five_by_three * mu_e * efield[q * ng + c];
-                out.dcoeff[c] = five_by_three * ch->dcoeff[e];
+                out.dcoeff[c] = five_by_three * orc_spec_dcoeff(ch, e + 1, sa, electemp[c], Tg);
                 out.source[c] = jheat[t] - elastic[t] + inelastic[t];
             }
     free(jheat);

@@ -66,6 +66,12 @@ typedef struct {
     const int *reactants, *products;
     const double *k_arrh, *elecenergy;
     const double *mobility, *dcoeff, *molmass, *charge;
+    /* field-dependent transport of H2_O2_chem.f90:532-672 (NULL: constants):
+     * mob_kind (nspecies), mob_terms (nspecies, nspecies), mob_coef
+     * (3, nspecies, nspecies), dcoeff_kind (nspecies), as in mps_chem */
+    const int *mob_kind, *mob_terms;
+    const double *mob_coef;
+    const int *dcoeff_kind;
 } orc_chem;
 
 /* Outputs of an *_update_transport: vel (3 ghosted components), dcoeff,

@@ -173,7 +173,9 @@ class _Chem(C.Structure):
                 ("ionspecmin", C.c_int), ("ionspecmax", C.c_int),
                 ("reactants", C.c_void_p), ("products", C.c_void_p), ("k_arrh", C.c_void_p),
                 ("elecenergy", C.c_void_p), ("mobility", C.c_void_p), ("dcoeff", C.c_void_p),
-                ("molmass", C.c_void_p), ("charge", C.c_void_p)]
+                ("molmass", C.c_void_p), ("charge", C.c_void_p),
+                ("mob_kind", C.c_void_p), ("mob_terms", C.c_void_p), ("mob_coef", C.c_void_p),
+                ("dcoeff_kind", C.c_void_p)]
 
 
 class _TOut(C.Structure):
@@ -189,8 +191,14 @@ def _chem(ch):
             np.ascontiguousarray(ch.k_arrh, dtype=np.float64), np.ascontiguousarray(ch.elecenergy, dtype=np.float64),
             np.ascontiguousarray(ch.mobility, dtype=np.float64), np.ascontiguousarray(ch.dcoeff, dtype=np.float64),
             np.ascontiguousarray(ch.molmass, dtype=np.float64), np.ascontiguousarray(ch.charge, dtype=np.float64)]
+    models = [None] * 4
+    if getattr(ch, "mob_kind", None) is not None:
+        models = [np.ascontiguousarray(ch.mob_kind, dtype=np.int32), np.asfortranarray(ch.mob_terms, dtype=np.int32),
+                  np.asfortranarray(ch.mob_coef, dtype=np.float64), np.ascontiguousarray(ch.dcoeff_kind, dtype=np.int32)]
+        keep += models
     s = _Chem(len(ch.mobility), int(np.asarray(ch.reactants).shape[0]), int(ch.especnum), int(ch.bgspecnum),
-              int(ch.ionspecmin), int(ch.ionspecmax), *[a.ctypes.data for a in keep])
+              int(ch.ionspecmin), int(ch.ionspecmax), *[a.ctypes.data for a in keep[:8]],
+              *[None if a is None else a.ctypes.data for a in models])
     return s, keep
 
 

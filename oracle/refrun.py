@@ -106,6 +106,54 @@ def run_vtk(case, timeout=120):
     return pvtr, vtr
 
 
+def run_chem(name, Tg, Pg, states, timeout=60):
+    """The reference chem_module ``name`` ("H2_O2", "ion_elec"; built by
+    ``make -C oracle chem``) on sample states [(Te, efield_mag,
+    specden[nspecies]), ...]: its tables and, per state, every reaction's
+    rate and every species' production, mobility, diffusion coefficient,
+    the inelastic term and the electron collision frequency."""
+    exe = os.path.join(HERE, "_ref", "ref_chem_" + name)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as fh:
+            fh.write(struct.pack("<i", len(states)))
+            fh.write(struct.pack("<2d", Tg, Pg))
+            for Te, ef, sd in states:
+                fh.write(struct.pack("<2d", Te, ef))
+                fh.write(np.asarray(sd, dtype="<f8").tobytes())
+        subprocess.run([exe, fin, fout], check=True, timeout=timeout, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.PIPE)
+        with open(fout, "rb") as fh:
+            raw = fh.read()
+    ns, nr, e, bg, imin, imax = struct.unpack("<6i", raw[:24])
+    pos = 24
+
+    def take(dtype, count, shape=None):
+        nonlocal pos
+        a = np.frombuffer(raw, dtype=dtype, count=count, offset=pos).copy()
+        pos += a.nbytes
+        return a.reshape(shape, order="F") if shape else a
+
+    out = {"nspecies": ns, "nreac": nr, "especnum": e, "bgspecnum": bg, "ionspecmin": imin, "ionspecmax": imax}
+    out["reactants"] = take("<i4", nr * ns, (nr, ns))
+    out["products"] = take("<i4", nr * ns, (nr, ns))
+    out["isratearrh"] = take("<i4", nr)
+    out["k_arrh"] = take("<f8", 3 * nr, (3, nr)).T.copy()
+    out["elecenergy"] = take("<f8", nr)
+    out["molmass"] = take("<f8", ns)
+    out["charge"] = take("<f8", ns)
+    per = {k: [] for k in ("rate", "prod", "mob", "dco", "inel", "collfreq")}
+    for _ in states:
+        per["rate"].append(take("<f8", nr))
+        per["prod"].append(take("<f8", ns))
+        per["mob"].append(take("<f8", ns))
+        per["dco"].append(take("<f8", ns))
+        per["inel"].append(take("<f8", 1)[0])
+        per["collfreq"].append(take("<f8", 1)[0])
+    out.update({k: np.array(v) for k, v in per.items()})
+    return out
+
+
 REF_PLASMA = os.path.join(HERE, "_ref", "mps3d_ion_elec")
 
 from mps3d_amd.plasma import ION_ELEC_DECK  # noqa: E402,F401  (the deck layout)

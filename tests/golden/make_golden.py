@@ -6,9 +6,9 @@ oracle/ref_driver/*.f90) on seeded inputs and stores inputs and outputs as
 compressed .npz files. Run it in the build container (where the reference
 tree and MPICH are present):
 
-    make -C oracle ref plasma && python tests/golden/make_golden.py [vtk] [plasma]
+    make -C oracle ref plasma chem && python tests/golden/make_golden.py [vtk] [plasma] [chem]
 
-(``vtk`` / ``plasma`` write only those fixtures.)
+(``vtk`` / ``plasma`` / ``chem`` write only those fixtures.)
 
 The fixtures are data only: inputs, expected outputs and residual histories.
 """
@@ -80,6 +80,32 @@ def vtk_fixture():
     save("vtk_output", **out)
 
 
+CHEM_STATES = 64
+
+
+def chem_fixture():
+    """The reference's chem_modules (H2_O2_chem.f90, ion_elec_chem.f90,
+    driven by oracle/ref_driver/ref_chem.f90) on 64 seeded states each: their
+    tables, and per state every reaction rate, every species' production,
+    mobility and diffusion coefficient, the inelastic term and the electron
+    collision frequency."""
+    rng = np.random.default_rng(20261204)
+    out = {}
+    for name, ns, Pg in (("H2_O2", 10, 13332.2), ("ion_elec", 3, 133.0)):
+        Tg = 300.0
+        lo = {"H2_O2": [21, 21, 14, 12, 12, 12, 12, 12, 17, 17], "ion_elec": [21, 14, 14]}[name]
+        hi = {"H2_O2": [23, 23, 17, 16, 16, 16, 16, 16, 21, 21], "ion_elec": [23, 17, 17]}[name]
+        Te = 10.0 ** rng.uniform(np.log10(3000.0), np.log10(1.0e5), CHEM_STATES)
+        ef = 10.0 ** rng.uniform(2, 6, CHEM_STATES)
+        sd = 10.0 ** rng.uniform(lo, hi, (CHEM_STATES, ns))
+        d = refrun.run_chem(name, Tg, Pg, list(zip(Te, ef, sd)))
+        for k, v in d.items():
+            out["%s_%s" % (name, k)] = np.asarray(v)
+        out[name + "_Te"], out[name + "_ef"], out[name + "_sd"] = Te, ef, sd
+        out[name + "_TgPg"] = np.array([Tg, Pg])
+    save("chem_reference", **out)
+
+
 PLASMA_STEPS = (1, 5)
 
 
@@ -103,9 +129,11 @@ def plasma_fixture():
 
 def main(argv=()):
     assert refrun.available(), "build the reference first: make -C oracle ref"
-    if "vtk" in argv or "plasma" in argv:
+    if "vtk" in argv or "plasma" in argv or "chem" in argv:
         if "vtk" in argv:
             vtk_fixture()
+        if "chem" in argv:
+            chem_fixture()
         if "plasma" in argv:
             plasma_fixture()
         return
@@ -198,6 +226,7 @@ def main(argv=()):
 
     vtk_fixture()
     plasma_fixture()
+    chem_fixture()
 
     with open(os.path.join(OUT, "ref_timings.json"), "w") as fh:
         json.dump({"host": os.uname().nodename, "nproc": os.cpu_count(), "runs": timings}, fh, indent=1)
