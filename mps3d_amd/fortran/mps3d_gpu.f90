@@ -285,15 +285,21 @@ contains
 
     ! initializechemistry: the chem_module's tables (reactants/products
     ! (nreac, nspecies), k_arrh(3, nreac), per-species mobility, dcoeff,
-    ! molmass, charge) copied to the device
+    ! molmass, charge) copied to the device; with the four optional
+    ! arguments, field-dependent mobility and diffusion models (mps_chem in
+    ! include/mps3d.h)
     subroutine gpu_set_chemistry(ctx, nspecies, nreac, especnum, bgspecnum, ionspecmin, ionspecmax, &
-                                 reactants, products, k_arrh, elecenergy, mobility, dcoeff, molmass, charge, status)
+                                 reactants, products, k_arrh, elecenergy, mobility, dcoeff, molmass, charge, status, &
+                                 mob_kind, mob_terms, mob_coef, dcoeff_kind)
         type(c_ptr), intent(in) :: ctx
         integer, intent(in) :: nspecies, nreac, especnum, bgspecnum, ionspecmin, ionspecmax
         integer(c_int), intent(in), target :: reactants(nreac, nspecies), products(nreac, nspecies)
         real*8, intent(in), target :: k_arrh(3, nreac), elecenergy(nreac)
         real*8, intent(in), target :: mobility(nspecies), dcoeff(nspecies), molmass(nspecies), charge(nspecies)
         integer, intent(out) :: status
+        integer(c_int), intent(in), target, optional :: mob_kind(nspecies), mob_terms(nspecies, nspecies)
+        integer(c_int), intent(in), target, optional :: dcoeff_kind(nspecies)
+        real*8, intent(in), target, optional :: mob_coef(3, nspecies, nspecies)
         type(mps_chem) :: ch
         ch%nspecies = nspecies
         ch%nreac = nreac
@@ -309,6 +315,12 @@ contains
         ch%dcoeff = c_loc(dcoeff)
         ch%molmass = c_loc(molmass)
         ch%charge = c_loc(charge)
+        if (present(mob_kind) .and. present(mob_terms) .and. present(mob_coef) .and. present(dcoeff_kind)) then
+            ch%mob_kind = c_loc(mob_kind)
+            ch%mob_terms = c_loc(mob_terms)
+            ch%mob_coef = c_loc(mob_coef)
+            ch%dcoeff_kind = c_loc(dcoeff_kind)
+        endif
         status = mps_set_chemistry(ctx, ch)
     end subroutine gpu_set_chemistry
 
