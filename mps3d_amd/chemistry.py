@@ -49,6 +49,8 @@ class Chemistry:
     bgspecnum: int
     ionspecmin: int
     ionspecmax: int
+    neutralspecmin: int = 0     # solved neutral species (neutralspecmax < min: none)
+    neutralspecmax: int = -1
     # field-dependent transport (None: the constants above); see mps_chem
     mob_kind: np.ndarray | None = None     # (nspecies,) 0 constant, 1 electron fit, 2 reduced ion mobility
     mob_terms: np.ndarray | None = None    # (nspecies partner, nspecies)
@@ -195,7 +197,7 @@ def h2_o2():
         molmass=np.array([2.0 * mp, 32.0 * mp, M_ELEC, 1.0 * mp, 2.0 * mp, 16.0 * mp, 32.0 * mp, 16.0 * mp, 1.0 * mp,
                           16.0 * mp]),
         charge=np.array([0.0, 0.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 0.0, 0.0]),
-        especnum=3, bgspecnum=1, ionspecmin=4, ionspecmax=8,
+        especnum=3, bgspecnum=1, ionspecmin=4, ionspecmax=8, neutralspecmin=9, neutralspecmax=10,
         mob_kind=mob_kind, mob_terms=terms, mob_coef=coef, dcoeff_kind=dk)
 
 

@@ -1,10 +1,11 @@
 """The plasma solver's time-step loop on the device.
 
 :class:`PlasmaRun` is ``timestepping`` of plasma_pde/solve_manager.f90:363-502
-for a table chemistry (ion_elec: background gas, electrons, one ion; no
-neutral solve): per step the Poisson solve and the field, the electron
-energy solve and the electron temperature, the electron solve and the ion
-solve, each as the reference's ``*_solve`` does it -- transport assembly,
+for a chemistry of :mod:`mps3d_amd.chemistry` (ion_elec: one ion; H2_O2:
+five ions and two solved neutral species): per step the Poisson solve and
+the field, the electron energy solve and the electron temperature, the
+electron solve, the ion solves and the neutral solves, each as the
+reference's ``*_solve`` does it -- transport assembly,
 boundary update (``*_update_bcs``), the loop of at most 20 reference V-cycles
 to an RMS residual of 1e-11, floors and ghost values -- with every field
 resident in HBM between steps. Each equation has its own context (the
@@ -12,12 +13,14 @@ reference hierarchy on the deck's process grid, exact arithmetic), so the
 solves follow perform_vcycle's trajectory.
 
 The deck is the dictionary form of the reference's GEOM_INPUTS and
-PLASMA_INPUTS (``ION_ELEC_DECK`` below has the layout). Fields are
+PLASMA_INPUTS (``ION_ELEC_DECK`` and ``H2_O2_DECK`` below). Fields are
 float64 device tensors of shape (nz+2, ny+2, nx+2), the ghosted Fortran
 layout; :meth:`PlasmaRun.fields` returns the physical fields under the
 names of the reference's output files.
 """
 from __future__ import annotations
+
+import math
 
 from .chemistry import K_BMAN, M_ELEC, etempmin_default, ion_elec, real4
 from .solver import FieldSolver
@@ -45,6 +48,23 @@ ION_ELEC_DECK = {
 }
 
 
+# tests/H2_O2/{GEOM_INPUTS,PLASMA_INPUTS} of the reference (the H2_O2_chem deck)
+H2_O2_DECK = {
+    "origin": (0.0, 0.0, 0.0), "length": (0.004, 0.004, 0.004), "n": (32, 32, 32), "p": (2, 4, 1),
+    "timestep": 1.0e-11, "pressure": 13300.0, "temperature": 300.0,
+    "scales": {"LENGTH": 0.004, "TIME": 1.0e-11, "ELECDEN": 1.0e14, "IONDEN": 1.0e14, "SPECDEN": 1.0e15,
+               "POT": 1000.0, "ETEMP": 5800.0},
+    "init": {"H2": 2.0e24, "O2": 1.0e24, "E": 1.0e14, "H+": 0.25e14, "H2+": 0.25e14, "O+": 0.25e14,
+             "O2+": 0.5e14, "O-": 0.25e14, "H": 1.0e12, "O": 1.0e12, "POT": 0.0, "ELECTEMP": 5802.0},
+    "bcs": [("left", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0),
+            ("right", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0),
+            ("bottom", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0),
+            ("top", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0),
+            ("back", "FLUX", "FLUX", "DIRC", "FLUX", 0.0, 0.0, 1750.0, 0.0),
+            ("front", "FLUX", "FLUX", "DIRC", "FLUX", 0.0, 0.0, 0.0, 0.0)],
+}
+
+
 class PlasmaRun:
     def __init__(self, deck, chem=None, device="cuda"):
         import torch
@@ -52,13 +72,14 @@ class PlasmaRun:
         self.dev = torch.device(device)
         self.chem = chem or ion_elec()
         ch = self.chem
-        if ch.ionspecmax != ch.ionspecmin:
-            raise ValueError("one ion species (the ion_elec layout)")
+        self.ions = list(range(ch.ionspecmin - 1, ch.ionspecmax))
+        self.neutrals = list(range(ch.neutralspecmin - 1, ch.neutralspecmax))
         self.n = tuple(int(v) for v in deck["n"])
         self.p = tuple(int(v) for v in deck["p"])
         sc = deck["scales"]
         self.L0, self.t0 = float(sc["LENGTH"]), float(sc["TIME"])
         self.escale_n, self.iscale = float(sc["ELECDEN"]), float(sc["IONDEN"])
+        self.nscale = float(sc["SPECDEN"])
         self.potscale, self.etempscale = float(sc["POT"]), float(sc["ETEMP"])
         # solversetup (solve_manager.f90:316)
         self.eescale = self.escale_n * K_BMAN * self.etempscale
@@ -74,6 +95,7 @@ class PlasmaRun:
             return s
 
         self.pois, self.ener, self.elec, self.ion = ctx(), ctx(), ctx(), ctx()
+        self.neut = ctx() if self.neutrals else None
         # boundary codes and parameters per equation (readsolverinputfile)
         bcs = {b[0]: b[1:] for b in deck["bcs"]}
         self.codes = {eq: tuple(bcs[f][q] for f in FACES) for q, eq in enumerate(("eden", "spec", "pot", "etemp"))}
@@ -84,6 +106,7 @@ class PlasmaRun:
         self.scal = {"pot": (tl, tl2, self.t0, self.t0 / self.potscale),
                      "eden": (tl, tl2, self.t0, self.t0 / self.escale_n),
                      "ion": (tl, tl2, self.t0, self.t0 / self.iscale),
+                     "neut": (tl, tl2, self.t0, self.t0 / self.nscale),
                      "ener": (tl, tl2, self.t0, self.t0 / self.eescale)}
         self.vel_scaling = tl
         self.nemin = 1.0e11 / self.escale_n                 # eden_initialize
@@ -99,16 +122,21 @@ class PlasmaRun:
         e = ch.especnum - 1
         self.eps = 1.5 * self.numden[e] * K_BMAN * self.Te     # real*4 1.5 is exact
         self.eprod = torch.zeros(g, dtype=torch.float64, device=self.dev)
+        self.nprod = [torch.zeros(g, dtype=torch.float64, device=self.dev) for _ in self.neutrals]
         # module solutions (scaled, ghosted): poisson_init, eden_init, ...
         self.phi_s = self.pot / self.potscale
         self.ne_s = self.numden[e] / self.escale_n
-        self.ni_s = self.numden[ch.ionspecmin - 1] / self.iscale
+        self.ni_s = [self.numden[i] / self.iscale for i in self.ions]
+        self.nn_s = [self.numden[i] / self.nscale for i in self.neutrals]
         self.eps_s = self.eps / self.eescale
         # *_set_bcs: face values bc_params / scale on every face
         self.faces = {
             "pot": [self._face_full(f, self.params["pot"][q] / self.potscale) for q, f in enumerate(FACES)],
             "eden": [self._face_full(f, self.params["eden"][q] / self.escale_n) for q, f in enumerate(FACES)],
-            "ion": [self._face_full(f, self.params["spec"][q] / self.iscale) for q, f in enumerate(FACES)],
+            "ion": [[self._face_full(f, self.params["spec"][q] / self.iscale) for q, f in enumerate(FACES)]
+                    for _ in self.ions],
+            "neut": [[self._face_full(f, self.params["spec"][q] / self.nscale) for q, f in enumerate(FACES)]
+                     for _ in self.neutrals],
             "ener": [self._face_full(f, 0.0) for f in FACES],
         }
         self.t = 0.0
@@ -151,13 +179,13 @@ class PlasmaRun:
     def step(self):
         torch = self.torch
         ch = self.chem
-        e, ion = ch.especnum - 1, ch.ionspecmin - 1
+        e = ch.especnum - 1
         dt_s = self.dt / self.t0
         cyc = []
         # poisson_solve (poissoneq.f90:387-470); the potential's faces stay
         # (poisson_update_bcs changes only 'DZGD' electrodes)
-        co = self.pois.poisson_transport(self.numden[e].reshape(-1), self.numden[ion].reshape(-1),
-                                         self.scal["pot"], timederiv=False, dt=dt_s)
+        ions = self.numden[self.ions[0]:self.ions[-1] + 1].reshape(-1)
+        co = self.pois.poisson_transport(self.numden[e].reshape(-1), ions, self.scal["pot"], timederiv=False, dt=dt_s)
         cyc.append(self._solve(self.pois, self.phi_s, self.codes["pot"], self.faces["pot"], co, False, dt_s,
                                faces_changed=self.steps == 0))
         self.pot = self.phi_s * self.potscale
@@ -181,13 +209,29 @@ class PlasmaRun:
         inner.copy_(torch.where(inner < self.nemin, torch.full_like(inner, self.nemin), inner))
         self.elec.fill_ghosts(self.ne_s)
         self.numden[e].copy_(self.ne_s * self.escale_n)
-        # ions_solve (ionseq.f90:560-629)
-        co = self.ion.species_transport(ch.ionspecmin, self.E.reshape(-1), self.Te.reshape(-1),
-                                        self.numden.reshape(-1), self.scal["ion"], soln=self.ni_s.reshape(-1),
-                                        dt=dt_s, Tg=self.Tg, Pg=self.Pg)
-        self._ion_bcs()
-        cyc.append(self._solve(self.ion, self.ni_s, self.codes["spec"], self.faces["ion"], co, True, dt_s))
-        self.numden[ion].copy_(self.ni_s * self.iscale)
+        # ions_solve (ionseq.f90:560-629): every ion from the same densities,
+        # which take the new ion densities after the last one
+        for q, sp in enumerate(self.ions):
+            co = self.ion.species_transport(sp + 1, self.E.reshape(-1), self.Te.reshape(-1),
+                                            self.numden.reshape(-1), self.scal["ion"],
+                                            soln=self.ni_s[q].reshape(-1), dt=dt_s, Tg=self.Tg, Pg=self.Pg)
+            self._ion_bcs(q, sp)
+            cyc.append(self._solve(self.ion, self.ni_s[q], self.codes["spec"], self.faces["ion"][q], co, True, dt_s))
+        for q, sp in enumerate(self.ions):
+            self.numden[sp].copy_(self.ni_s[q] * self.iscale)
+        # neutrals_solve (neutralseq.f90:438-508), after it the same update
+        for q, sp in enumerate(self.neutrals):
+            co = self.neut.species_transport(sp + 1, self.E.reshape(-1), self.Te.reshape(-1),
+                                             self.numden.reshape(-1), self.scal["neut"],
+                                             soln=self.nn_s[q].reshape(-1), dt=dt_s, Tg=self.Tg, Pg=self.Pg,
+                                             out={"prod": self.nprod[q].view(-1)})
+            # neutrals_update_transport leaves vel at zero
+            co["vel"].zero_()
+            self._neutral_bcs(q, sp)
+            cyc.append(self._solve(self.neut, self.nn_s[q], self.codes["spec"], self.faces["neut"][q], co, True,
+                                   dt_s))
+        for q, sp in enumerate(self.neutrals):
+            self.numden[sp].copy_(self.nn_s[q] * self.nscale)
         self.t = self.t + self.dt
         self.steps += 1
         self.cycles.append(tuple(cyc))
@@ -228,24 +272,53 @@ class PlasmaRun:
                 flux = (4.0 / 3.0) * eb * cbar
                 self.faces["ener"][q] = flux * self.vel_scaling
 
-    def _ion_bcs(self):
-        """ions_update_bcs (ionseq.f90:206-476): FLUX faces carry the drift
-        outflow mu n E_normal where it leaves the domain, else zero."""
+    def _bc_mobility(self, sp, f):
+        """getspecmobility of species sp on the boundary cells of face f: the
+        constant, or the reduced-mobility sum N_atm / sum_j n_j / K_j."""
+        ch = self.chem
+        kind = 0 if ch.mob_kind is None else int(ch.mob_kind[sp])
+        if kind == 0:
+            return float(ch.mobility[sp])
+        if kind != 2:
+            raise ValueError("boundary mobility of kind %d" % kind)
+        total = None
+        for j in range(ch.nspecies):
+            if ch.mob_terms[j, sp]:
+                inv = self._bc(self.numden[j], f) / float(ch.mob_coef[0, j, sp])
+                total = inv if total is None else total + inv
+        natm = 101325.0 / K_BMAN / 300.0
+        return (float(ch.mobility[sp]) * natm) / total
+
+    def _ion_bcs(self, q, sp):
+        """ions_update_bcs (ionseq.f90:206-476) for ion sp: FLUX faces carry
+        the drift outflow mu n E_normal where it leaves the domain, else
+        zero."""
         torch = self.torch
-        mu = float(self.chem.mobility[self.chem.ionspecmin - 1])
-        for q, f in enumerate(FACES):
-            code = self.codes["spec"][q]
+        faces = self.faces["ion"][q]
+        for k, f in enumerate(FACES):
+            code = self.codes["spec"][k]
             if code == "DIRC":
                 # the reference divides by ionscale on every call (ionseq.f90:243)
-                self.faces["ion"][q] = self.faces["ion"][q] / self.iscale
+                faces[k] = faces[k] / self.iscale
             elif code == "FLUX":
+                mu = self._bc_mobility(sp, f)
                 En = self._bc(self.E[self._axis(f)], f)
-                nb = self._bc(self.ni_s, f)
+                nb = self._bc(self.ni_s[q], f)
                 low = f in ("left", "bottom", "back")
                 drift = mu * nb * En * self.vel_scaling
                 out = (mu * En < 0) if low else (mu * En > 0)
                 val = -drift if low else drift
-                self.faces["ion"][q] = torch.where(out, val, torch.zeros_like(val))
+                faces[k] = torch.where(out, val, torch.zeros_like(val))
+
+    def _neutral_bcs(self, q, sp):
+        """neutrals_update_bcs (neutralseq.f90:206-358): FLUX faces get the
+        thermal outflow 0.25 n cbar, cbar = sqrt(8 k_B T_g / (pi m))."""
+        const_1 = 8.0 * K_BMAN / PI_REF / float(self.chem.molmass[sp])
+        cbar = math.sqrt(const_1 * self.Tg)
+        faces = self.faces["neut"][q]
+        for k, f in enumerate(FACES):
+            if self.codes["spec"][k] == "FLUX":
+                faces[k] = 0.25 * self._bc(self.nn_s[q], f) * cbar * self.vel_scaling
 
     # ------------------------------------------------------------ output
     def fields(self):
@@ -262,8 +335,11 @@ class PlasmaRun:
         out["Electron_Temp (K)"] = host(self.Te)
         out["Electron_energy (J/m3)"] = host(self.eps)
         out["E_prod (#/m3/s)"] = host(self.eprod)
+        for q, sp in enumerate(self.neutrals):
+            out["%s_prod (#/m3/s)" % ch.names[sp]] = host(self.nprod[q])
         return out
 
     def close(self):
-        for s in (self.pois, self.ener, self.elec, self.ion):
-            s.close()
+        for s in (self.pois, self.ener, self.elec, self.ion, self.neut):
+            if s is not None:
+                s.close()

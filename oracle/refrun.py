@@ -204,14 +204,14 @@ def _gather(td, name, n, p):
     return out
 
 
-def run_plasma(deck, nsteps, timeout=600):
+def run_plasma(deck, nsteps, timeout=600, chem="ion_elec"):
     """The reference plasma solver (oracle/_ref/mps3d_ion_elec) on ``deck``
     for nsteps time steps: (initial fields, fields after nsteps steps), each
     {scalar name: global (nx, ny, nz) array} as its VTK output holds them."""
     p = deck["p"]
     with tempfile.TemporaryDirectory() as td:
         write_deck(td, deck, nsteps)
-        cmd = [mpiexec_path(), "-n", str(int(np.prod(p))), REF_PLASMA]
+        cmd = [mpiexec_path(), "-n", str(int(np.prod(p))), os.path.join(HERE, "_ref", "mps3d_" + chem)]
         subprocess.run(["bash", "-c", "ulimit -s unlimited && exec \"$@\"", "_"] + cmd, check=True,
                        timeout=timeout, cwd=td, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
         return _gather(td, "output_0000000", deck["n"], p), _gather(td, "output_final", deck["n"], p)

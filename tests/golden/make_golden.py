@@ -109,22 +109,24 @@ def chem_fixture():
 PLASMA_STEPS = (1, 5)
 
 
-def plasma_fixture():
-    """The reference plasma solver (plasma_pde/main.f90, ion_elec chemistry)
-    on the tests/ion_elec deck at 16^3 cells over a 2x2x1 process grid: the
-    fields of its VTK output after 1 and after 5 time steps."""
-    deck = dict(refrun.ION_ELEC_DECK)
+def plasma_fixture(chem="ion_elec"):
+    """The reference plasma solver (plasma_pde/main.f90 with the ion_elec or
+    the H2_O2 chemistry) on its tests/ion_elec or tests/H2_O2 deck at 16^3
+    cells over a 2x2x1 process grid: the fields of its VTK output after 1 and
+    after 5 time steps."""
+    from mps3d_amd.plasma import H2_O2_DECK, ION_ELEC_DECK
+    deck = dict(ION_ELEC_DECK if chem == "ion_elec" else H2_O2_DECK)
     deck.update(n=(16, 16, 16), p=(2, 2, 1))
     out = {"n": np.array(deck["n"]), "p": np.array(deck["p"])}
     names = None
     for k in PLASMA_STEPS:
-        ini, fin = refrun.run_plasma(deck, k)
+        ini, fin = refrun.run_plasma(deck, k, chem=chem)
         names = sorted(fin)
         for i, nm in enumerate(names):
             out["s%d_f%d" % (k, i)] = fin[nm]
     out["names"] = np.array(names)
     out["steps"] = np.array(PLASMA_STEPS)
-    save("plasma_ion_elec16", **out)
+    save("plasma_ion_elec16" if chem == "ion_elec" else "plasma_h2_o2_16", **out)
 
 
 def main(argv=()):
@@ -135,7 +137,8 @@ def main(argv=()):
         if "chem" in argv:
             chem_fixture()
         if "plasma" in argv:
-            plasma_fixture()
+            plasma_fixture("ion_elec")
+            plasma_fixture("H2_O2")
         return
     timings = {}
 
@@ -225,7 +228,8 @@ def main(argv=()):
     save("sg1d", rows=rows, out=refrun.run_sg1d(rows))
 
     vtk_fixture()
-    plasma_fixture()
+    plasma_fixture("ion_elec")
+    plasma_fixture("H2_O2")
     chem_fixture()
 
     with open(os.path.join(OUT, "ref_timings.json"), "w") as fh:

@@ -1,8 +1,10 @@
 """The device time-step loop (mps3d_amd/plasma.py, timestepping of
 plasma_pde/solve_manager.f90:363-502) against the reference plasma solver
-itself: tests/golden/plasma_ion_elec16.npz holds the fields of the
-reference's VTK output after 1 and after 5 steps of its tests/ion_elec deck
-at 16^3 cells on a 2x2x1 process grid (plasma_pde/main.f90 with the ion_elec
+itself: tests/golden/plasma_ion_elec16.npz and plasma_h2_o2_16.npz hold the
+fields of the reference's VTK output after 1 and after 5 steps of its
+tests/ion_elec deck (one ion) and its tests/H2_O2 deck (the default
+chemistry: five ions, two solved neutral species, field-dependent mobilities)
+at 16^3 cells on a 2x2x1 process grid (plasma_pde/main.f90 with each
 chemistry, built from the reference sources and run under MPI by
 tests/golden/make_golden.py).
 
@@ -11,8 +13,8 @@ fields follow the reference's trajectory; what separates them is rounding
 (the cell width from the scaled lengths, exp and pow of the device library,
 the MPI sum order of the RMS norm), bounded here by 1e-12 relative to each
 field's largest magnitude (the electric field: to the largest component);
-measured: bit-equal after one step except the production (3e-16), 2e-15 at
-most after five.
+measured: ion_elec bit-equal after one step except the production (3e-16),
+2e-15 at most after five; H2_O2 6e-16 after one step, 5e-15 after five.
 """
 import numpy as np
 import pytest
@@ -24,14 +26,16 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
+@pytest.mark.parametrize("chem", ["ion_elec", "H2_O2"])
 @pytest.mark.parametrize("k", [1, 5])
-def test_time_steps_follow_the_reference_run(gpu, k):
-    from mps3d_amd.plasma import ION_ELEC_DECK, PlasmaRun
-    d = golden("plasma_ion_elec16")
+def test_time_steps_follow_the_reference_run(gpu, k, chem):
+    from mps3d_amd.chemistry import h2_o2, ion_elec
+    from mps3d_amd.plasma import H2_O2_DECK, ION_ELEC_DECK, PlasmaRun
+    d = golden("plasma_ion_elec16" if chem == "ion_elec" else "plasma_h2_o2_16")
     names = [str(s) for s in d["names"]]
-    deck = dict(ION_ELEC_DECK)
+    deck = dict(ION_ELEC_DECK if chem == "ion_elec" else H2_O2_DECK)
     deck.update(n=tuple(int(v) for v in d["n"]), p=tuple(int(v) for v in d["p"]))
-    run = PlasmaRun(deck)
+    run = PlasmaRun(deck, chem=ion_elec() if chem == "ion_elec" else h2_o2())
     try:
         run.run(k)
         got = run.fields()
@@ -44,6 +48,6 @@ def test_time_steps_follow_the_reference_run(gpu, k):
         scale = emax if nm.startswith("Electric") else np.abs(ref).max()
         err = np.abs(got[nm] - ref).max() / scale
         worst[nm] = err
-    print(k, worst, run.cycles)
+    print(chem, k, worst, run.cycles)
     for nm, err in worst.items():
         assert err <= TOL, (nm, err)
