@@ -233,12 +233,14 @@ def step_kernels(n, reps=20):
     # the whole time step (solve_manager's loop: four transport assemblies,
     # four V-cycle solves to 1e-11, the field and the temperature) on the
     # reference's ion_elec deck and on a 128^3 version of it
-    from mps3d_amd.plasma import ION_ELEC_DECK, PlasmaRun
-    for tag, n, reps in (("32", 32, 5), ("128", 128, 3)):
-        deck = dict(ION_ELEC_DECK)
+    from mps3d_amd.chemistry import h2_o2, ion_elec
+    from mps3d_amd.plasma import H2_O2_DECK, ION_ELEC_DECK, PlasmaRun
+    for tag, base, chem, n, reps in (("32", ION_ELEC_DECK, ion_elec, 32, 10), ("128", ION_ELEC_DECK, ion_elec, 128, 3),
+                                     ("h2_o2_32", H2_O2_DECK, h2_o2, 32, 10)):
+        deck = dict(base)
         deck["n"] = (n, n, n)
-        run = PlasmaRun(deck)
-        run.step()
+        run = PlasmaRun(deck, chem=chem())
+        run.run(2)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         run.run(reps)
