@@ -65,6 +65,29 @@ H2_O2_DECK = {
 }
 
 
+def output_names(chem):
+    """The reference's output scalars in its order (all_scalarnames,
+    solve_manager.f90:270-289): species densities, potential, field, electron
+    temperature and energy, electron and solved-neutral production."""
+    names = ["%s_density (#/m3)" % nm for nm in chem.names] + ["Potential (V)"]
+    names += ["Electric_field_%s (V/m)" % a for a in "XYZ"]
+    names += ["Electron_Temp (K)", "Electron_energy (J/m3)", "E_prod (#/m3/s)"]
+    names += ["%s_prod (#/m3/s)" % chem.names[sp] for sp in range(chem.neutralspecmin - 1, chem.neutralspecmax)]
+    return names
+
+
+def write_deck_output(dirname, fields, chem, deck):
+    """writeoutputfile (solve_manager.f90:353-361) for physical fields
+    {output name: (nx, ny, nz) array}: the reference's VTK files, one piece
+    per rank of the deck's process grid, coordinates in units of the deck's
+    length scale (scalelengths, solve_manager.f90:152-172)."""
+    from . import vtk
+    names = output_names(chem)
+    return vtk.write_output(dirname, [fields[nm] for nm in names], names, tuple(deck["n"]), tuple(deck["length"]),
+                            tuple(deck["p"]), origin=tuple(deck.get("origin", (0.0, 0.0, 0.0))),
+                            length_scale=float(deck["scales"]["LENGTH"]))
+
+
 class PlasmaRun:
     def __init__(self, deck, chem=None, device="cuda"):
         import torch
@@ -85,6 +108,7 @@ class PlasmaRun:
         self.eescale = self.escale_n * K_BMAN * self.etempscale
         self.dt = float(deck["timestep"])
         self.Tg, self.Pg = float(deck["temperature"]), float(deck["pressure"])
+        self.deck = dict(deck)
         length = tuple(float(L) / self.L0 for L in deck["length"])   # scalelengths
         self.length = length
 
@@ -338,6 +362,14 @@ class PlasmaRun:
         for q, sp in enumerate(self.neutrals):
             out["%s_prod (#/m3/s)" % ch.names[sp]] = host(self.nprod[q])
         return out
+
+    def output_names(self):
+        return output_names(self.chem)
+
+    def write_output(self, dirname):
+        """writeoutputfile: the current fields in the reference's VTK files
+        (write_deck_output)."""
+        return write_deck_output(dirname, self.fields(), self.chem, self.deck)
 
     def close(self):
         for s in (self.pois, self.ener, self.elec, self.ion, self.neut):

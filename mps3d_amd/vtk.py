@@ -93,11 +93,14 @@ def _vtr_bytes(blocks, names, lo, hi, offs, dxs):
     return b"".join(out)
 
 
-def write_output(dirname, fields, names, n, length, p=(1, 1, 1), origin=(0.0, 0.0, 0.0)):
+def write_output(dirname, fields, names, n, length, p=(1, 1, 1), origin=(0.0, 0.0, 0.0), length_scale=1.0):
     """writeoutput: ``<dirname>.pvtr`` and ``<dirname>/file_<rank>.vtr`` for
     the px*py*pz blocks (rank r at block (r mod px, (r / px) mod py,
-    r / (px py)), find3dindex, mgridlib/globalvars.f90:73-87). Returns the
-    paths written."""
+    r / (px py)), find3dindex, mgridlib/globalvars.f90:73-87). Coordinates
+    are divided by length_scale the way the plasma solver scales its
+    geometry (scalelengths, plasma_pde/solve_manager.f90:152-172: block
+    offsets and lengths in metres, then divided). Returns the paths
+    written."""
     n = tuple(int(v) for v in n)
     px, py, pz = (int(v) for v in p)
     if any(nv % pv for nv, pv in zip(n, (px, py, pz))):
@@ -118,8 +121,8 @@ def write_output(dirname, fields, names, n, length, p=(1, 1, 1), origin=(0.0, 0.
         pi = (r % px, (r // px) % py, r // (px * py))
         lo = tuple(q * m for q, m in zip(pi, l))
         hi = tuple((q + 1) * m for q, m in zip(pi, l))
-        offs = tuple(float(o) + float(q * m) * dd for o, q, m, dd in zip(origin, pi, l, d))
-        dxs = tuple((m * dd) / m for m, dd in zip(l, d))
+        offs = tuple((float(o) + float(q * m) * dd) / length_scale for o, q, m, dd in zip(origin, pi, l, d))
+        dxs = tuple(((m * dd) / length_scale) / m for m, dd in zip(l, d))
         blocks = [a[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] for a in data]
         src = "%s/file_%08d.vtr" % (dirname, r)
         with open(src, "wb") as fh:

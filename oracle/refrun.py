@@ -204,7 +204,7 @@ def _gather(td, name, n, p):
     return out
 
 
-def run_plasma(deck, nsteps, timeout=600, chem="ion_elec"):
+def run_plasma(deck, nsteps, timeout=600, chem="ion_elec", raw=False):
     """The reference plasma solver (oracle/_ref/mps3d_ion_elec) on ``deck``
     for nsteps time steps: (initial fields, fields after nsteps steps), each
     {scalar name: global (nx, ny, nz) array} as its VTK output holds them."""
@@ -214,7 +214,14 @@ def run_plasma(deck, nsteps, timeout=600, chem="ion_elec"):
         cmd = [mpiexec_path(), "-n", str(int(np.prod(p))), os.path.join(HERE, "_ref", "mps3d_" + chem)]
         subprocess.run(["bash", "-c", "ulimit -s unlimited && exec \"$@\"", "_"] + cmd, check=True,
                        timeout=timeout, cwd=td, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
-        return _gather(td, "output_0000000", deck["n"], p), _gather(td, "output_final", deck["n"], p)
+        res = (_gather(td, "output_0000000", deck["n"], p), _gather(td, "output_final", deck["n"], p))
+        if raw:
+            files = {"output_final.pvtr": open(os.path.join(td, "output_final.pvtr"), "rb").read()}
+            for r in range(int(np.prod(p))):
+                name = "output_final/file_%08d.vtr" % r
+                files[name] = open(os.path.join(td, name), "rb").read()
+            res = res + (files,)
+        return res
 
 
 def run(case, timeout=600):

@@ -129,6 +129,22 @@ def plasma_fixture(chem="ion_elec"):
     save("plasma_ion_elec16" if chem == "ion_elec" else "plasma_h2_o2_16", **out)
 
 
+def plasma_output_fixture():
+    """The bytes of the reference plasma solver's VTK output (output_final:
+    the .pvtr and the four ranks' .vtr) after one step of each deck at 16^3
+    on 2x2x1 ranks."""
+    from mps3d_amd.plasma import H2_O2_DECK, ION_ELEC_DECK
+    out = {}
+    for chem, base in (("ion_elec", ION_ELEC_DECK), ("H2_O2", H2_O2_DECK)):
+        deck = dict(base)
+        deck.update(n=(16, 16, 16), p=(2, 2, 1))
+        _, _, files = refrun.run_plasma(deck, 1, chem=chem, raw=True)
+        for name, data in files.items():
+            key = "%s__%s" % (chem, name.replace("/", "__").replace(".", "_"))
+            out[key] = np.frombuffer(data, dtype=np.uint8)
+    save("plasma_output_files", **out)
+
+
 def main(argv=()):
     assert refrun.available(), "build the reference first: make -C oracle ref"
     if "vtk" in argv or "plasma" in argv or "chem" in argv:
@@ -139,6 +155,7 @@ def main(argv=()):
         if "plasma" in argv:
             plasma_fixture("ion_elec")
             plasma_fixture("H2_O2")
+            plasma_output_fixture()
         return
     timings = {}
 
@@ -230,6 +247,7 @@ def main(argv=()):
     vtk_fixture()
     plasma_fixture("ion_elec")
     plasma_fixture("H2_O2")
+    plasma_output_fixture()
     chem_fixture()
 
     with open(os.path.join(OUT, "ref_timings.json"), "w") as fh:

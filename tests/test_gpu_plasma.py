@@ -51,3 +51,42 @@ def test_time_steps_follow_the_reference_run(gpu, k, chem):
     print(chem, k, worst, run.cycles)
     for nm, err in worst.items():
         assert err <= TOL, (nm, err)
+
+
+@pytest.mark.parametrize("chem", ["ion_elec", "H2_O2"])
+def test_time_step_output_files(gpu, chem, tmp_path, monkeypatch):
+    """One device step of each deck written as the reference writes
+    output_final (tests/golden/plasma_output_files.npz): the .pvtr byte for
+    byte, and every .vtr with the same header and coordinates byte for byte
+    and every scalar to TOL of its largest magnitude."""
+    import os
+
+    from mps3d_amd import vtk
+    from mps3d_amd.chemistry import h2_o2, ion_elec
+    from mps3d_amd.plasma import H2_O2_DECK, ION_ELEC_DECK, PlasmaRun
+    d = golden("plasma_output_files")
+    deck = dict(ION_ELEC_DECK if chem == "ion_elec" else H2_O2_DECK)
+    deck.update(n=(16, 16, 16), p=(2, 2, 1))
+    run = PlasmaRun(deck, chem=ion_elec() if chem == "ion_elec" else h2_o2())
+    try:
+        run.step()
+        monkeypatch.chdir(tmp_path)
+        run.write_output("output_final")
+    finally:
+        run.close()
+    with open("output_final.pvtr", "rb") as fh:
+        assert fh.read() == d["%s__output_final_pvtr" % chem].tobytes()
+    for r in range(4):
+        ref = d["%s__output_final__file_%08d_vtr" % (chem, r)].tobytes()
+        with open(os.path.join("output_final", "file_%08d.vtr" % r), "rb") as fh:
+            got = fh.read()
+        cut = ref.index(b"<AppendedData")
+        assert len(got) == len(ref) and got[:cut] == ref[:cut]
+        (tmp_path / "ref.vtr").write_bytes(ref)
+        a, b = vtk.read_vtr(os.path.join("output_final", "file_%08d.vtr" % r)), vtk.read_vtr(str(tmp_path / "ref.vtr"))
+        for nm in b:
+            if nm in ("X", "Y", "Z"):
+                assert np.array_equal(a[nm], b[nm])
+            else:
+                scale = max(np.abs(b[nm]).max(), 1e-300)
+                assert np.abs(a[nm] - b[nm]).max() <= TOL * scale, (r, nm)

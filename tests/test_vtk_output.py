@@ -55,3 +55,38 @@ def test_ghosted_fields_and_reader(tmp_path):
 def test_uneven_decomposition_is_rejected(tmp_path):
     with pytest.raises(ValueError):
         vtk.write_output(str(tmp_path / "x"), [np.zeros((5, 4, 4))], ("A",), (5, 4, 4), (1, 1, 1), (2, 1, 1))
+
+
+@pytest.mark.parametrize("chem", ["ion_elec", "H2_O2"])
+def test_deck_output_matches_the_reference_plasma_output(chem, tmp_path, monkeypatch):
+    """output_final of the reference plasma solver after one step of its
+    deck (16^3, 2x2x1 ranks; tests/golden/plasma_output_files.npz): read back
+    and written again by write_deck_output with the deck's scalar names and
+    length scale, every file comes out byte for byte."""
+    from mps3d_amd.chemistry import h2_o2, ion_elec
+    from mps3d_amd.plasma import H2_O2_DECK, ION_ELEC_DECK, write_deck_output
+    d = golden("plasma_output_files")
+    deck = dict(ION_ELEC_DECK if chem == "ion_elec" else H2_O2_DECK)
+    deck.update(n=(16, 16, 16), p=(2, 2, 1))
+    ch = ion_elec() if chem == "ion_elec" else h2_o2()
+    ref = {}
+    fields = {}
+    for r in range(4):
+        key = "%s__output_final__file_%08d_vtr" % (chem, r)
+        ref[r] = d[key].tobytes()
+        path = tmp_path / ("ref_%d.vtr" % r)
+        path.write_bytes(ref[r])
+        blk = vtk.read_vtr(str(path))
+        pi = (r % 2, (r // 2) % 2, 0)
+        for nm, a in blk.items():
+            if nm in ("X", "Y", "Z"):
+                continue
+            g = fields.setdefault(nm, np.zeros((16, 16, 16), order="F"))
+            g[pi[0] * 8:(pi[0] + 1) * 8, pi[1] * 8:(pi[1] + 1) * 8, :] = a
+    monkeypatch.chdir(tmp_path)
+    write_deck_output("output_final", fields, ch, deck)
+    with open("output_final.pvtr", "rb") as fh:
+        assert fh.read() == d["%s__output_final_pvtr" % chem].tobytes()
+    for r in range(4):
+        with open(os.path.join("output_final", "file_%08d.vtr" % r), "rb") as fh:
+            assert fh.read() == ref[r], r
