@@ -104,7 +104,7 @@ def glow_gradients(model, nsteps, ckpt, V0=1000.0, s0=1.0, seed=20261202):
     q = q0.clone().requires_grad_(True)
     J = model.run(V, s, q, nsteps, checkpoint_every=ckpt)
     J.backward()
-    return float(J), float(V.grad), float(s.grad), q.grad.detach(), q0
+    return float(J.detach()), float(V.grad), float(s.grad), q.grad.detach(), q0
 
 
 def fd_checks(model, nsteps, grads, V0=1000.0, s0=1.0, rel=1e-6):
