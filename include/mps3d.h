@@ -209,8 +209,9 @@ int mps_efield(mps_ctx *ctx, double *phi, double *efield, double scale, int limi
 /* ---- Transport assembly (the step before and after each solve) ----------
  * Per-cell coefficients and sources of the plasma equations, computed on the
  * device in the ghosted layout mps_set_coeffs takes, for chemistries given
- * as tables: Arrhenius rates in T_e and constant mobility and diffusion per
- * species, the form of plasma_pde/chemistry/ion_elec_chem.f90. Inputs and
+ * as tables: Arrhenius rates in T_e, and per species a constant mobility and
+ * diffusion (plasma_pde/chemistry/ion_elec_chem.f90) or the field-dependent
+ * mobility fits and Einstein diffusion of H2_O2_chem.f90. Inputs and
  * outputs are host or device arrays (host arrays are staged for the call);
  * numden is the reference's numden(0:nx+1,0:ny+1,0:nz+1,nspecies), efield
  * and vel three ghosted components one after the other. scaling[4] =
@@ -246,9 +247,9 @@ typedef struct {
     const int *dcoeff_kind;         /* (nspecies) */
 } mps_chem;
 
-/* initializechemistry (ion_elec_chem.f90:104-109, with the tables of
- * assignreactions and setspecparams, :30-102): copies the tables to the
- * device (at most 16 species, 64 reactions, stoichiometry 0..16). */
+/* initializechemistry (ion_elec_chem.f90:104-109, H2_O2_chem.f90:433-438,
+ * with the tables of assignreactions and setspecparams): copies the tables
+ * to the device (at most 16 species, 64 reactions, stoichiometry 0..16). */
 int mps_set_chemistry(mps_ctx *ctx, const mps_chem *chem);
 
 /* eden_update_transport (edeneq.f90:360-444) for specnum = especnum, with
