@@ -121,6 +121,9 @@ typedef struct {
     int tol_kind;               /* MPS_TOL_*                                   */
     int arith;                  /* MPS_ARITH_* for the solver's own rows       */
     int start;                  /* MPS_START_*                                 */
+    int dc_inner;               /* FP32 V-cycles per FP64 defect-correction
+                                   step under MPS_PREC_FP32_DC, 1..4
+                                   (default 2)                                 */
 } mps_opts;
 
 typedef struct mps_ctx mps_ctx;

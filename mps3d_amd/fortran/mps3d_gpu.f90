@@ -40,7 +40,7 @@ module mps3d_gpu
     type, bind(C) :: mps_opts
         integer(c_int) :: hierarchy, precision, flux, nu1, nu2, max_cycles
         real(c_double) :: tol
-        integer(c_int) :: tol_kind, arith, start
+        integer(c_int) :: tol_kind, arith, start, dc_inner
     end type mps_opts
 
     interface

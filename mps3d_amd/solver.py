@@ -75,7 +75,7 @@ class FieldSolver:
 
     def __init__(self, n, length=(1.0, 1.0, 1.0), p=(1, 1, 1), hierarchy="ref", precision="fp64",
                  flux="upwind", nu1=2, nu2=2, max_cycles=20, tol=1e-11, tol_kind="rms", arith="exact",
-                 start="guess"):
+                 start="guess", dc_inner=None):
         self.lib = M.load()
         self.n = tuple(int(v) for v in (n if not np.isscalar(n) else (n, n, n)))
         self.length = tuple(float(v) for v in (length if not np.isscalar(length) else (length,) * 3))
@@ -91,6 +91,8 @@ class FieldSolver:
         o.tol_kind = _TOL[tol_kind] if isinstance(tol_kind, str) else int(tol_kind)
         o.arith = {"exact": M.ARITH_EXACT, "fast": M.ARITH_FAST}[arith] if isinstance(arith, str) else int(arith)
         o.start = {"guess": 0, "fmg": 1}[start] if isinstance(start, str) else int(start)
+        if dc_inner is not None:
+            o.dc_inner = int(dc_inner)
         self.opts = o
         self.max_cycles = int(max_cycles)
         h = C.c_void_p()
