@@ -1704,6 +1704,63 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
     return MPS_OK;
 }
 
+// restrict_boundary (mgridsteps.f90:55-128) of one face on the device: the
+// 4-child mean of an (fa, fb) face onto (ca, cb), in the host loop's order;
+// c32 (nullable) gets the FP32 copy
+__global__ void k_restrict_face(const double *__restrict__ F, int fa, double *__restrict__ C, int ca, int cb,
+                                float *__restrict__ c32)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ca * cb) return;
+    const int p = t % ca + 1, q = t / ca + 1;
+    const double v = 0.25 * (F[(2 * p - 2) + (size_t)fa * (2 * q - 2)] + F[(2 * p - 1) + (size_t)fa * (2 * q - 2)] +
+                             F[(2 * p - 2) + (size_t)fa * (2 * q - 1)] + F[(2 * p - 1) + (size_t)fa * (2 * q - 1)]);
+    C[t] = v;
+    if (c32) c32[t] = (float)v;
+}
+
+// mps_set_bc for faces that are all device arrays (or NULL): copies and
+// coarse faces made on the context's stream, no host round trip
+void set_bc_device(mps_ctx *c, const double *const faces[6])
+{
+    const int nl = c->nlev();
+    const Geom &g0 = c->geom[0];
+    const long long fsz[6] = {(long long)g0.ny * g0.nz, (long long)g0.ny * g0.nz, (long long)g0.nx * g0.nz,
+                              (long long)g0.nx * g0.nz, (long long)g0.nx * g0.ny, (long long)g0.nx * g0.ny};
+    for (int f = 0; f < 6; ++f) {
+        if (!c->faces0[f]) c->faces0[f] = dalloc<double>(fsz[f]);
+        if (faces && faces[f])
+            HIPCHK(hipMemcpyAsync(c->faces0[f], faces[f], fsz[f] * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c->stream));
+        else
+            fill<double>(c, c->faces0[f], fsz[f], 0.0);
+    }
+    if (c->opts.hierarchy == MPS_HIER_REF) {
+        for (int l = 1; l < nl; ++l) {
+            const Geom &gf = c->geom[l - 1], &gc = c->geom[l];
+            const int fa[6] = {gf.ny, gf.ny, gf.nx, gf.nx, gf.nx, gf.nx};
+            const int ca[6] = {gc.ny, gc.ny, gc.nx, gc.nx, gc.nx, gc.nx};
+            const int cb[6] = {gc.nz, gc.nz, gc.nz, gc.nz, gc.ny, gc.ny};
+            for (int f = 0; f < 6; ++f) {
+                const size_t nc = (size_t)ca[f] * cb[f];
+                auto &B64 = c->L64[l];
+                if (!B64.faces[f]) B64.faces[f] = dalloc<double>(nc);
+                float *c32 = nullptr;
+                if (c->opts.precision == MPS_PREC_FP32_DC) {
+                    auto &B32 = c->L32[l];
+                    if (!B32.faces[f]) B32.faces[f] = dalloc<float>(nc);
+                    c32 = B32.faces[f];
+                }
+                const double *src = l == 1 ? c->faces0[f] : c->L64[l - 1].faces[f];
+                hipLaunchKernelGGL(k_restrict_face, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->stream, src,
+                                   fa[f], B64.faces[f], ca[f], cb[f], c32);
+            }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 int mps_set_bc(mps_ctx *c, const int codes[6], const double *const faces[6])
 {
     if (!c || !codes) return fail(MPS_ERR_ARG, "null argument");
@@ -1712,6 +1769,18 @@ int mps_set_bc(mps_ctx *c, const int codes[6], const double *const faces[6])
         for (int f = 0; f < 6; ++f) {
             if (codes[f] < 0 || codes[f] > 3) return fail(MPS_ERR_ARG, "bad BC code %d", codes[f]);
             c->bc[f] = codes[f];
+        }
+        bool all_dev = true;
+        for (int f = 0; f < 6 && faces; ++f)
+            if (faces[f] && !is_device_ptr(faces[f])) all_dev = false;
+        if (faces && all_dev) {
+            const int st = after_null_stream(c);
+            if (st) return st;
+            set_bc_device(c, faces);
+            c->bc_set = true;
+            ++c->gen;
+            c->transpose_ready = false;
+            return MPS_OK;
         }
         const int nl = c->nlev();
         c->faces_host.assign((size_t)nl * 6, {});
