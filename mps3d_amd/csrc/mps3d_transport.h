@@ -1,9 +1,12 @@
 // Transport assembly of the plasma equations on the device: the per-cell
 // coefficients and sources each implicit solve of a time step takes
 // (plasma_pde/{edeneq,ionseq,eenergyeq,poissoneq}.f90, *_update_transport,
-// update_electrontemp), for chemistries given as tables (Arrhenius rates in
-// T_e, constant mobility and diffusion per species, as
-// plasma_pde/chemistry/ion_elec_chem.f90).
+// update_electrontemp), for chemistries given as tables: Arrhenius rates in
+// T_e, and per species a constant mobility and diffusion
+// (plasma_pde/chemistry/ion_elec_chem.f90) or the field-dependent mobility
+// fits and Einstein diffusion of plasma_pde/chemistry/H2_O2_chem.f90,
+// evaluated per cell (the solved neutral species of neutrals_update_transport
+// take the species form with their zero mobility).
 //
 // Every kernel is a one-pass, HBM-bound stream over the ghosted grid (the
 // Joule term reads the six neighbours of n_e and E as well, which the L2
