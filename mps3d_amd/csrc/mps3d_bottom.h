@@ -24,6 +24,11 @@ constexpr int kBottomMax = 512;
 // arrays in LDS (the reference's coarse solves use m = n, which is 8-64 on
 // the usual process grids); larger m keep them in the work buffer
 constexpr int kBottomLdsM = 64;
+// Coarse grids of up to this many cells with m <= kBottomLdsM also keep an
+// interior-only copy of the Krylov basis in LDS (column stride n + 1), so
+// the Gram-Schmidt products and the orthogonalised vector read no global
+// memory; the ghosted basis stays in the work buffer for the operator.
+constexpr int kBottomLdsN = 64;
 
 template <typename T>
 __host__ __device__ inline long long bottom_work_elems(int m, long long ng)
@@ -61,6 +66,9 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
     __shared__ int cmap[kBottomMax];   // interior cell q (i fastest) -> ghosted index
     __shared__ T pr[kBottomMax];       // products of one inner product
     __shared__ T wsh[kBottomMax];      // w of the current Arnoldi step, by cell
+    __shared__ T sV[(kBottomLdsM + 1) * (kBottomLdsN + 1)];
+    const bool vlds = lds && n <= kBottomLdsN;
+    const int ldv = n + 1;
     const int tid = threadIdx.x, nt = blockDim.x;
 
     auto cidx = [&](int q, int &i, int &j, int &k) {
@@ -83,31 +91,45 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
         }
         __syncthreads();
     };
-    // gseidel_precond, niter = 1 (discretepde.f90:238-337)
+    // gseidel_precond, niter = 1 (discretepde.f90:238-337), from out = 0:
+    // every neighbour of a red cell is a zero black cell, so the red update
+    // (in - of) / dg has of = +0 exactly and is in / dg; the red pass sets
+    // the black cells to zero beside it, with no separate clearing pass. The
+    // ghost entries of out stay zero from the kernel's start (rows do not
+    // read them, and nothing writes them).
     auto precond = [&](T *out, const T *in) {
-        zero(out);
         if (n == 1) {
             if (tid == 0) {
                 int i, j, k;
                 long long c = cidx(0, i, j, k);
                 T dg, of;
-                row<T, OP, VEL, REAC>(L, out, i, j, k, c, dg, of);
+                row<T, OP, VEL, REAC>(L, in, i, j, k, c, dg, of);
                 out[c] = in[c] / dg;
             }
             __syncthreads();
             return;
         }
-        for (int red = 1; red >= 0; --red) {
-            for (int q = tid; q < n; q += nt) {
-                int i, j, k;
-                long long c = cidx(q, i, j, k);
-                if (((i + j + k) & 1) != red) continue;
-                T dg, of;
-                row<T, OP, VEL, REAC>(L, out, i, j, k, c, dg, of);
-                out[c] = (in[c] - of) / dg;
+        for (int q = tid; q < n; q += nt) {   // red cells, black cells cleared
+            int i, j, k;
+            long long c = cidx(q, i, j, k);
+            if (((i + j + k) & 1) != 1) {
+                out[c] = T(0);
+                continue;
             }
-            __syncthreads();
+            T dg, of;
+            row<T, OP, VEL, REAC>(L, in, i, j, k, c, dg, of);   // dg only: it does not depend on u
+            out[c] = in[c] / dg;
         }
+        __syncthreads();
+        for (int q = tid; q < n; q += nt) {   // black cells from the new red ones
+            int i, j, k;
+            long long c = cidx(q, i, j, k);
+            if (((i + j + k) & 1) != 0) continue;
+            T dg, of;
+            row<T, OP, VEL, REAC>(L, out, i, j, k, c, dg, of);
+            out[c] = (in[c] - of) / dg;
+        }
+        __syncthreads();
     };
     for (int q = tid; q < n; q += nt) {
         int i, j, k;
@@ -132,6 +154,10 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
     if (tid == 0) *status = 0;
     zero(x);
     zero(r0);
+    zero(av);   // apply and precond write interiors only: the ghost entries
+    zero(w);    // of these stay zero for the whole solve
+    zero(r);
+    zero(mb);
     for (int q = tid; q < n; q += nt) {
         int i, j, k;
         long long c = cidx(q, i, j, k);
@@ -165,7 +191,6 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
         }
         for (long long q = tid; q < ng; q += nt) x0[q] = x[q];
         __syncthreads();
-        zero(av);
         apply(av, x0);
         zero(r0);
         for (int q = tid; q < n; q += nt) {
@@ -180,13 +205,14 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
         __syncthreads();
         if (fabs((double)beta) < (double)tol) break;
         for (long long q = tid; q < ng; q += nt) V[q] = r[q] / beta;
+        if (vlds)
+            for (int q = tid; q < n; q += nt) sV[q] = r[cmap[q]] / beta;
         __syncthreads();
         int optk = m;
         int lucky = 0;
         T residnorm = T(0);
         for (int kd = 1; kd <= m; ++kd) {
             const T *vj = V + ng * (long long)(kd - 1);
-            zero(av);
             apply(av, vj);
             precond(w, av);
             // classical Gram-Schmidt: h_ii = <w, v_ii> for ii = 1..kd, one
@@ -194,23 +220,57 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
             for (int q = tid; q < n; q += nt) wsh[q] = w[cmap[q]];
             __syncthreads();
             for (int ii = 1 + tid; ii <= kd; ii += nt) {
-                const T *vi = V + ng * (long long)(ii - 1);
                 T h = T(0);
-                for (int q = 0; q < n; ++q) h = h + wsh[q] * vi[cmap[q]];
+                if (vlds) {
+                    const T *vi = sV + ldv * (ii - 1);
+                    for (int q = 0; q < n; ++q) h = h + wsh[q] * vi[q];
+                } else {
+                    const T *vi = V + ng * (long long)(ii - 1);
+                    for (int q = 0; q < n; ++q) h = h + wsh[q] * vi[cmap[q]];
+                }
                 HM(ii, kd) = h;
             }
             __syncthreads();
-            for (long long q = tid; q < ng; q += nt) {
-                T t = w[q];
-                for (int ii = 1; ii <= kd; ++ii) t = t - HM(ii, kd) * V[ng * (long long)(ii - 1) + q];
-                wj[q] = t;
+            T hn;
+            if (vlds) {
+                // wj by cell over the interior (the ghost entries of every
+                // basis vector are zero), kept in wsh, with the products of
+                // <wj, wj> formed beside it and summed in order by one lane
+                for (int q = tid; q < n; q += nt) {
+                    T t = wsh[q];
+                    for (int ii = 1; ii <= kd; ++ii) t = t - HM(ii, kd) * sV[ldv * (ii - 1) + q];
+                    wsh[q] = t;
+                    pr[q] = t * t;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    T s2 = T(0);
+                    for (int q = 0; q < n; ++q) s2 = s2 + pr[q];
+                    sh[0] = s2;
+                }
+                __syncthreads();
+                hn = sqrt(sh[0]);
+            } else {
+                for (long long q = tid; q < ng; q += nt) {
+                    T t = w[q];
+                    for (int ii = 1; ii <= kd; ++ii) t = t - HM(ii, kd) * V[ng * (long long)(ii - 1) + q];
+                    wj[q] = t;
+                }
+                __syncthreads();
+                hn = sqrt(dot(wj, wj));
             }
-            __syncthreads();
-            T hn = sqrt(dot(wj, wj));
             int nanflag = 0;
             if (hn > T(1e-25)) {
                 T *vn = V + ng * (long long)kd;
-                for (long long q = tid; q < ng; q += nt) vn[q] = wj[q] / hn;
+                if (vlds) {
+                    for (int q = tid; q < n; q += nt) {
+                        const T v = wsh[q] / hn;
+                        vn[cmap[q]] = v;
+                        sV[ldv * kd + q] = v;
+                    }
+                } else {
+                    for (long long q = tid; q < ng; q += nt) vn[q] = wj[q] / hn;
+                }
             } else if (!(hn != hn) && !((hn / hn != T(1)) && hn != T(0))) {
                 lucky = 1;
             } else {
