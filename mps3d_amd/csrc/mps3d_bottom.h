@@ -141,6 +141,7 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
         __syncthreads();
         if (tid == 0) {
             T s = T(0);
+#pragma unroll 8
             for (int q = 0; q < n; ++q) s = s + pr[q];
             sh[0] = s;
         }
@@ -215,6 +216,8 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
             const T *vj = V + ng * (long long)(kd - 1);
             apply(av, vj);
             precond(w, av);
+            // The serial loops below are unrolled so that their loads issue
+            // ahead of the dependent chain; the summation order is unchanged.
             // classical Gram-Schmidt: h_ii = <w, v_ii> for ii = 1..kd, one
             // lane each, then wj = w - h_1 v_1 - h_2 v_2 - ... in that order
             for (int q = tid; q < n; q += nt) wsh[q] = w[cmap[q]];
@@ -223,9 +226,11 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
                 T h = T(0);
                 if (vlds) {
                     const T *vi = sV + ldv * (ii - 1);
+#pragma unroll 8
                     for (int q = 0; q < n; ++q) h = h + wsh[q] * vi[q];
                 } else {
                     const T *vi = V + ng * (long long)(ii - 1);
+#pragma unroll 8
                     for (int q = 0; q < n; ++q) h = h + wsh[q] * vi[cmap[q]];
                 }
                 HM(ii, kd) = h;
@@ -238,6 +243,7 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
                 // <wj, wj> formed beside it and summed in order by one lane
                 for (int q = tid; q < n; q += nt) {
                     T t = wsh[q];
+#pragma unroll 8
                     for (int ii = 1; ii <= kd; ++ii) t = t - HM(ii, kd) * sV[ldv * (ii - 1) + q];
                     wsh[q] = t;
                     pr[q] = t * t;
@@ -245,6 +251,7 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
                 __syncthreads();
                 if (tid == 0) {
                     T s2 = T(0);
+#pragma unroll 8
                     for (int q = 0; q < n; ++q) s2 = s2 + pr[q];
                     sh[0] = s2;
                 }
@@ -287,6 +294,7 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
                 // carried in a register, the chain touches memory only to
                 // load the untouched entries and store the finished ones
                 T hc = HM(1, kd);
+#pragma unroll 8
                 for (int ii = 1; ii <= kd - 1; ++ii) {
                     const T hn1 = HM(ii + 1, kd), c = cs[ii - 1], s = sn[ii - 1];
                     const T t = c * hc + s * hn1;
