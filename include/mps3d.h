@@ -283,6 +283,17 @@ int mps_poisson_transport(mps_ctx *ctx, double dt, int timederiv, const double *
                           const double *iondens, const double scaling[4], const double *soln,
                           double *vel, double *dcoeff, double *reac, double *source);
 
+/* The FLUX faces of ions_update_bcs (ionseq.f90:247-476) for ion species
+ * specnum (1-based): on the boundary cells of each face whose pointer is not
+ * null, the drift outflow mu n E_normal velscale where mu E_normal points
+ * out of the domain, else 0. efield: 3 ghosted components; numden: every
+ * species, ghosted, for the mobility (getspecmobility: constant or reduced
+ * ion mobility); ionsoln: the ion's scaled solution, ghosted. faces[6]:
+ * left/right (ny,nz), bottom/top (nx,nz), back/front (nx,ny), first index
+ * fastest, as mps_set_bc takes them. */
+int mps_ion_drift_faces(mps_ctx *ctx, int specnum, const double *efield, const double *numden,
+                        const double *ionsoln, double velscale, double *const faces[6]);
+
 /* update_electrontemp (eenergyeq.f90:842-877): electemp = eenergysoln *
  * eenergyscale / (3/2 n_e k_B) on the interior; below etempmin the
  * temperature is etempmin and eenergysoln (in/out) is reset to match, in the

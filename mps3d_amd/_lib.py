@@ -29,7 +29,7 @@ EXPORTS = (
     "mps_vcycle", "mps_solve", "mps_apply", "mps_apply_transpose", "mps_bvec", "mps_smooth",
     "mps_fill_ghosts", "mps_gmres", "mps_solve_transpose", "mps_last_timing", "mps_stream", "mps_solve_rhs",
     "mps_efield", "mps_set_chemistry", "mps_species_transport", "mps_eenergy_transport",
-    "mps_poisson_transport", "mps_electron_temp",
+    "mps_poisson_transport", "mps_electron_temp", "mps_ion_drift_faces",
     "mps_error_string", "mps_last_error", "mps_time_kernel", "mps_grad_velocity",
 )
 
@@ -99,6 +99,7 @@ def load():
                                         C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]
     L.mps_poisson_transport.argtypes = [_vp, C.c_double, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.mps_electron_temp.argtypes = [_vp, _vp, _vp, C.c_double, C.c_double, _vp]
+    L.mps_ion_drift_faces.argtypes = [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp]
     L.mps_gmres.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), _dp]
     L.mps_solve_transpose.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_int), _dp]
     L.mps_last_timing.argtypes = [_vp, _dp, C.POINTER(C.c_int)]

@@ -2351,6 +2351,46 @@ extern "C" int mps_poisson_transport(mps_ctx *c, double dt, int timederiv, const
     return MPS_OK;
 }
 
+extern "C" int mps_ion_drift_faces(mps_ctx *c, int specnum, const double *efield, const double *numden,
+                                   const double *ionsoln, double velscale, double *const faces[6])
+{
+    int st = guard(c, false);
+    if (st) return st;
+    if (!c->chem) return fail(MPS_ERR_STATE, "set the chemistry first (mps_set_chemistry)");
+    if (!efield || !numden || !ionsoln || !faces) return fail(MPS_ERR_ARG, "null argument");
+    const int sp = specnum - 1;
+    if (sp < 0 || sp >= c->chem_host.ns) return fail(MPS_ERR_ARG, "species %d out of range", specnum);
+    const int kind = c->chem_host.mkind[sp];
+    if (kind != 0 && kind != 2)
+        return fail(MPS_ERR_ARG, "boundary mobility of kind %d (constant or reduced ion mobility only)", kind);
+    const Geom &g = c->geom[0];
+    const size_t ng = (size_t)g.ng();
+    try {
+        Staging S(c);
+        const double *E = S.in(efield, 3 * ng);
+        const double *nd = S.in(numden, (size_t)c->chem_host.ns * ng);
+        const double *ni = S.in(ionsoln, ng);
+        const size_t fsz[6] = {(size_t)g.ny * g.nz, (size_t)g.ny * g.nz, (size_t)g.nx * g.nz,
+                               (size_t)g.nx * g.nz, (size_t)g.nx * g.ny, (size_t)g.nx * g.ny};
+        IonFaces F{};
+        size_t most = 0;
+        for (int f = 0; f < 6; ++f) {
+            F.f[f] = S.out(faces[f], fsz[f], false);
+            if (faces[f]) most = std::max(most, fsz[f]);
+        }
+        if (most > 0) {
+            Lev<double> v = view(c, c->L64, 0);
+            hipLaunchKernelGGL(k_ion_drift_faces, dim3((unsigned)((most + 255) / 256), 6), dim3(256), 0, c->stream,
+                               v, (const ChemDev *)c->chem, sp, E, nd, ni, velscale, F);
+            HIPCHK(hipGetLastError());
+        }
+        S.finish();
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
+}
+
 extern "C" int mps_electron_temp(mps_ctx *c, double *eenergysoln, const double *elecden, double eenergyscale,
                                  double etempmin, double *electemp)
 {

@@ -273,6 +273,56 @@ __global__ void __launch_bounds__(256) k_poisson_transport(Lev<double> L, const 
     }
 }
 
+// The FLUX branch of ions_update_bcs (ionseq.f90:247-476): the face value
+// of ion sp on the boundary cells of a FLUX face is the drift outflow
+// mu n E_normal vscale where mu E_normal leaves the domain, else 0
+// (n the scaled ion solution, mu from the densities); one face per
+// blockIdx.y, the face arrays as the boundary conditions take them
+// ((ny,nz), (nx,nz), (nx,ny), first index fastest). A null face is skipped.
+struct IonFaces {
+    double *f[6];
+};
+
+__global__ void __launch_bounds__(256) k_ion_drift_faces(Lev<double> L, const ChemDev *__restrict__ Cp, int sp,
+                                                         const double *__restrict__ E,
+                                                         const double *__restrict__ nd,
+                                                         const double *__restrict__ ni, double vscale, IonFaces F)
+{
+    const int face = blockIdx.y;
+    double *out = F.f[face];
+    if (!out) return;
+    const int axis = face / 2;
+    const bool low = (face & 1) == 0;
+    const int n0 = axis == 0 ? L.ny : L.nx;
+    const int n1 = axis == 2 ? L.ny : L.nz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n0 * n1) return;
+    const int a = t % n0 + 1, b = t / n0 + 1;
+    int i, j, k;
+    if (axis == 0) {
+        i = low ? 1 : L.nx;
+        j = a;
+        k = b;
+    } else if (axis == 1) {
+        i = a;
+        j = low ? 1 : L.ny;
+        k = b;
+    } else {
+        i = a;
+        j = b;
+        k = low ? 1 : L.nz;
+    }
+    const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
+    const long long ng = L.sz * (L.nz + 2);
+    const ChemDev &C = *Cp;
+    const double mu = spec_mobility(C, sp, nd, ng, c, 0.0);   // kinds 0 and 2: no T_e
+    const double En = E[(long long)axis * ng + c];
+    const double drift = mu * ni[c] * En * vscale;
+    const double muE = mu * En;
+    const bool leaves = low ? (muE < 0.0) : (muE > 0.0);
+    out[t] = leaves ? (low ? -drift : drift) : 0.0;
+}
+
 // update_electrontemp (eenergyeq.f90:842-877): T_e = n_eps scale /
 // (3/2 n_e k_B) on the interior, clamped below at etempmin, where the
 // energy density is reset to match the clamped temperature (in the cell

@@ -296,43 +296,23 @@ class PlasmaRun:
                 flux = (4.0 / 3.0) * eb * cbar
                 self.faces["ener"][q] = flux * self.vel_scaling
 
-    def _bc_mobility(self, sp, f):
-        """getspecmobility of species sp on the boundary cells of face f: the
-        constant, or the reduced-mobility sum N_atm / sum_j n_j / K_j."""
-        ch = self.chem
-        kind = 0 if ch.mob_kind is None else int(ch.mob_kind[sp])
-        if kind == 0:
-            return float(ch.mobility[sp])
-        if kind != 2:
-            raise ValueError("boundary mobility of kind %d" % kind)
-        total = None
-        for j in range(ch.nspecies):
-            if ch.mob_terms[j, sp]:
-                inv = self._bc(self.numden[j], f) / float(ch.mob_coef[0, j, sp])
-                total = inv if total is None else total + inv
-        natm = 101325.0 / K_BMAN / 300.0
-        return (float(ch.mobility[sp]) * natm) / total
-
     def _ion_bcs(self, q, sp):
         """ions_update_bcs (ionseq.f90:206-476) for ion sp: FLUX faces carry
         the drift outflow mu n E_normal where it leaves the domain, else
         zero."""
-        torch = self.torch
         faces = self.faces["ion"][q]
+        flux = [None] * 6
         for k, f in enumerate(FACES):
             code = self.codes["spec"][k]
             if code == "DIRC":
                 # the reference divides by ionscale on every call (ionseq.f90:243)
                 faces[k] = faces[k] / self.iscale
             elif code == "FLUX":
-                mu = self._bc_mobility(sp, f)
-                En = self._bc(self.E[self._axis(f)], f)
-                nb = self._bc(self.ni_s[q], f)
-                low = f in ("left", "bottom", "back")
-                drift = mu * nb * En * self.vel_scaling
-                out = (mu * En < 0) if low else (mu * En > 0)
-                val = -drift if low else drift
-                faces[k] = torch.where(out, val, torch.zeros_like(val))
+                flux[k] = faces[k]    # rewritten in place: mps_set_bc copies the faces
+        if any(t is not None for t in flux):
+            # all FLUX faces of this ion in one launch (mps_ion_drift_faces)
+            self.ion.ion_drift_faces(sp + 1, self.E.reshape(-1), self.numden.reshape(-1),
+                                     self.ni_s[q].reshape(-1), self.vel_scaling, flux)
 
     def _neutral_bcs(self, q, sp):
         """neutrals_update_bcs (neutralseq.f90:206-358): FLUX faces get the

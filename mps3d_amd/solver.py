@@ -384,6 +384,38 @@ class FieldSolver:
             self._release()
         return o
 
+    def ion_drift_faces(self, specnum, efield, numden, ionsoln, velscale, faces):
+        """The FLUX faces of ions_update_bcs for ion specnum (1-based): writes
+        the drift outflow into each face of `faces` (six entries: left/right
+        (ny,nz), bottom/top (nx,nz), back/front (nx,ny) arrays in Fortran
+        order, or device tensors of those sizes; None skips a face)."""
+        ns = self._need_chem().nspecies
+        nx, ny, nz = self.n
+        sizes = (ny * nz, ny * nz, nx * nz, nx * nz, nx * ny, nx * ny)
+        ptrs = (C.c_void_p * 6)()
+        for k, f in enumerate(faces):
+            if f is None:
+                continue
+            if _is_torch(f):
+                if not (f.is_cuda and f.dtype.is_floating_point and f.element_size() == 8 and f.is_contiguous()
+                        and f.numel() == sizes[k]):
+                    raise ValueError("face %d must be a contiguous float64 GPU tensor of %d elements" % (k, sizes[k]))
+                if not self._torch_call:
+                    self.order_torch_in()
+                ptrs[k] = f.data_ptr()
+            else:
+                if not (isinstance(f, np.ndarray) and f.dtype == np.float64 and f.flags.f_contiguous
+                        and f.size == sizes[k]):
+                    raise ValueError("face %d must be a Fortran-ordered float64 array of %d elements" % (k, sizes[k]))
+                ptrs[k] = f.ctypes.data
+        try:
+            M.check(self.lib.mps_ion_drift_faces(self._h, int(specnum), self._fields(efield, 3),
+                                                 self._fields(numden, ns), self._fields(ionsoln, 1),
+                                                 float(velscale), ptrs))
+        finally:
+            self._release()
+        return faces
+
     def electron_temp(self, eenergysoln, elecden, eenergyscale, etempmin, electemp):
         """update_electrontemp: writes electemp's interior and clamps
         eenergysoln (both in/out)."""
