@@ -141,6 +141,15 @@ module mps3d_gpu
             real(c_double), value :: eenergyscale, etempmin
         end function mps_electron_temp
 
+        integer(c_int) function mps_ion_drift_faces(ctx, specnum, efield, numden, ionsoln, velscale, faces) &
+                bind(C, name='mps_ion_drift_faces')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, efield, numden, ionsoln
+            integer(c_int), value :: specnum
+            real(c_double), value :: velscale
+            type(c_ptr), intent(in) :: faces(6)
+        end function mps_ion_drift_faces
+
         integer(c_int) function mps_fill_ghosts(ctx, soln) bind(C, name='mps_fill_ghosts')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, soln
@@ -386,6 +395,30 @@ contains
         integer, intent(out) :: status
         status = mps_electron_temp(ctx, c_loc(eenergysoln), c_loc(elecden), eenergyscale, etempmin, c_loc(electemp))
     end subroutine gpu_update_electrontemp
+
+    ! the FLUX faces of ions_update_bcs for ion specnum: the faces with
+    ! flux(f) /= 0 (left, right, bottom, top, back, front) get the drift outflow
+    subroutine gpu_ions_flux_faces(ctx, specnum, efield, numden, ionsoln, velscale, flux, leftface, rightface, &
+                                   bottomface, topface, backface, frontface, lx, ly, lz, nspec, status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: specnum, lx, ly, lz, nspec, flux(6)
+        real*8, intent(in), target :: efield(0:lx+1, 0:ly+1, 0:lz+1, 3)
+        real*8, intent(in), target :: numden(0:lx+1, 0:ly+1, 0:lz+1, nspec), ionsoln(0:lx+1, 0:ly+1, 0:lz+1)
+        real*8, intent(in) :: velscale
+        real*8, intent(inout), target :: leftface(ly, lz), rightface(ly, lz), bottomface(lx, lz), topface(lx, lz)
+        real*8, intent(inout), target :: backface(lx, ly), frontface(lx, ly)
+        integer, intent(out) :: status
+        type(c_ptr) :: f(6)
+        f = c_null_ptr
+        if (flux(1) /= 0) f(1) = c_loc(leftface)
+        if (flux(2) /= 0) f(2) = c_loc(rightface)
+        if (flux(3) /= 0) f(3) = c_loc(bottomface)
+        if (flux(4) /= 0) f(4) = c_loc(topface)
+        if (flux(5) /= 0) f(5) = c_loc(backface)
+        if (flux(6) /= 0) f(6) = c_loc(frontface)
+        status = mps_ion_drift_faces(ctx, int(specnum, c_int), c_loc(efield), c_loc(numden), c_loc(ionsoln), &
+                                     velscale, f)
+    end subroutine gpu_ions_flux_faces
 
     ! text of the library's last error
     function gpu_last_error() result(msg)
