@@ -206,6 +206,11 @@ struct mps_ctx {
 namespace {
 
 dim3 cells_grid(const Geom &g) { return dim3((unsigned)((g.nx * g.ny + kBlock - 1) / kBlock), (unsigned)g.nz); }
+// one colour of a red-black sweep: (nx + 1) / 2 threads per row
+dim3 colour_grid(const Geom &g)
+{
+    return dim3((unsigned)(((g.nx + 1) / 2 * g.ny + kBlock - 1) / kBlock), (unsigned)g.nz);
+}
 // reductions: blocks loop over k so a grid has few partial sums
 dim3 red_grid(const Geom &g)
 {
@@ -513,9 +518,13 @@ void launch_colour(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int red, int 
 {
     Lev<T> v = view(c, L, l);
     const Geom &g = c->geom[l];
+    // threads for the colour's cells only once that still leaves 256
+    // workgroups (128^3: 11 % faster per launch; at 32^3 it was 5 % slower)
+    const int compact = (long long)((g.nx + 1) / 2) * g.ny * g.nz >= 65536 ? 1 : 0;
     with_op(op, c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
-        hipLaunchKernelGGL((k_colour<T, decltype(O)::value, decltype(V)::value, decltype(R)::value>), cells_grid(g),
-                           block(), 0, c->stream, v, L[l].u, (const T *)L[l].b, red);
+        hipLaunchKernelGGL((k_colour<T, decltype(O)::value, decltype(V)::value, decltype(R)::value>),
+                           compact ? colour_grid(g) : cells_grid(g), block(), 0, c->stream, v, L[l].u,
+                           (const T *)L[l].b, red, compact);
     });
 }
 

@@ -377,13 +377,31 @@ __global__ void __launch_bounds__(kBlock) k_apply(Lev<T> L, const T *__restrict_
 
 // one red (red=1) or black (red=0) pass: u = (b - off)/diag on one colour;
 // red = odd (i+j+k), the reference's colouring (discretepde.f90:49-57)
+// compact != 0: one thread per cell of the colour: row (j, k) holds the
+// cells i = i0, i0 + 2, ... <= nx, i0 the first of the colour, and
+// (nx + 1) / 2 threads per row cover them (colour_grid). compact == 0: one
+// thread per cell (cells_grid), for small levels, where the compact grid
+// would leave too few waves to hide the memory latency.
 template <typename T, int OP, bool VEL, bool REAC>
-__global__ void __launch_bounds__(kBlock) k_colour(Lev<T> L, T *__restrict__ u, const T *__restrict__ b, int red)
+__global__ void __launch_bounds__(kBlock) k_colour(Lev<T> L, T *__restrict__ u, const T *__restrict__ b, int red,
+                                                  int compact)
 {
     int i, j, k;
     long long c;
-    if (!cell_of(L, i, j, k, c)) return;
-    if (((i + j + k) & 1) != red) return;
+    if (compact) {
+        const int hx = (L.nx + 1) / 2;
+        const int t = blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= hx * L.ny) return;
+        j = t / hx + 1;
+        k = blockIdx.y + 1;
+        const int i0 = ((1 + j + k) & 1) == red ? 1 : 2;
+        i = i0 + 2 * (t - (j - 1) * hx);
+        if (i > L.nx) return;
+        c = (long long)i + (long long)L.sy * j + L.sz * k;
+    } else {
+        if (!cell_of(L, i, j, k, c)) return;
+        if (((i + j + k) & 1) != red) return;
+    }
     T dg, of;
     row<T, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
     u[c] = gs_update(L, b[c], of, dg);
