@@ -22,12 +22,18 @@
  * re-entrant. Use one host thread per context.
  *
  * Ordering of device arrays: every call first orders the context's stream
- * after the work already queued on the null (default) stream, and returns
- * with its results complete (the context's stream synchronized), so device
- * arrays written on the null stream before a call, or read anywhere after
- * it, need nothing more. Arrays produced on another non-blocking stream must
- * be ordered before the call by the caller (e.g. hipStreamWaitEvent on
- * mps_stream(ctx)).
+ * after the work already queued on the null (default) stream, so device
+ * arrays written on the null stream before a call need nothing more
+ * (arrays produced on another non-blocking stream must be ordered before
+ * the call by the caller, e.g. hipStreamWaitEvent on mps_stream(ctx)). A
+ * call that reads or writes any host array returns with its results
+ * complete. A call on device arrays only may return with its work still
+ * queued on the context's stream (mps_set_bc, mps_set_coeffs,
+ * mps_set_source, mps_fill_ghosts, mps_floor, mps_efield, the transport and
+ * boundary-face entry points): later calls on the same context are ordered
+ * after it; other readers order after mps_stream(ctx) (or synchronize it).
+ * The solve entry points return with their results complete: their
+ * stopping test reads the residual norm.
  */
 #ifndef MPS3D_H
 #define MPS3D_H
@@ -154,6 +160,13 @@ int mps_set_coeffs(mps_ctx *ctx, const double *dcoeff, const double *velx, const
                    const double *velz, const double *reac, const double *source, int timederiv,
                    double dt);
 
+/* The source of the last mps_set_coeffs replaced, the other coefficients
+ * and the hierarchy built from them kept: the Poisson equation's per-step
+ * update (poisson_update_transport, poissoneq.f90:335-385, changes only its
+ * space-charge source; the coarse levels use no source,
+ * discretepde.f90:611-616). source ghosted, NULL = 0. */
+int mps_set_source(mps_ctx *ctx, const double *source);
+
 /* perform_vcycle (mgridsteps.f90:384-579) on the finest level: one V-cycle
  * on soln (ghosted, in/out) with multigrid source mgridsterm (interior,
  * NULL = 0); resnorm gets the RMS residual after the cycle. Ghost values of
@@ -193,6 +206,20 @@ int mps_bvec(mps_ctx *ctx, const double *mgridsterm, int applybc, double *b);
  * niter red-black sweeps, then res = b - A soln (interior, nullable), in the
  * context's arithmetic mode (bit-exact with the reference in EXACT). */
 int mps_smooth(mps_ctx *ctx, double *soln, const double *mgridsterm, int niter, double *res);
+
+/* restriction (mgridsteps.f90:11-53): coarse (nx/2, ny/2, nz/2) = the
+ * 8-child mean of fine (nx, ny, nz), both interior arrays, in the
+ * reference's summation order, by the kernel the reference-hierarchy
+ * V-cycle runs. The caller loops use it for output as well
+ * (plasma_pde/solve_manager.f90:573). Needs even nx, ny, nz. */
+int mps_restrict(mps_ctx *ctx, const double *fine, double *coarse);
+
+/* prolongation1 (mgridsteps.f90:170-216): fine (nx, ny, nz interior, out) =
+ * the central-slope interpolation of coarse, a ghosted
+ * (nx/2+2, ny/2+2, nz/2+2) array whose ghost layer is read as given, by
+ * the kernel of the reference-hierarchy V-cycle's correction step. Needs
+ * even nx, ny, nz. */
+int mps_prolong1(mps_ctx *ctx, const double *coarse, double *fine);
 
 /* update_boundary_ghostvalues (discretepde.f90:1121-1267): DIRC ghost =
  * 2 g - u, other codes ghost = u, on the six faces of soln. */
@@ -248,6 +275,15 @@ typedef struct {
     const int *mob_terms;           /* (nspecies, nspecies), Fortran order */
     const double *mob_coef;         /* (3, nspecies, nspecies), Fortran order */
     const int *dcoeff_kind;         /* (nspecies) */
+    /* Reaction rates (getspecproduction / getelectroninelasticterm with
+     * isratearrh and getcustomrate); both NULL = every reaction Arrhenius
+     * (getarrhrate from k_arrh). rate_kind(r): 0 Arrhenius; 1 the fit of
+     * O2_chem.f90:413-448, c0 exp(c1/T + c2/T**2 + c3/T**3 + c4/T**4)
+     * with rate_coef(1:5, r) = (c0, c1, c2, c3, c4), signed; 2
+     * alpha |E| exp(-E0/|E|) of H2_chem.f90:423-437 with rate_coef(1:2, r)
+     * = (alpha, E0) and |E| the cell's field magnitude. T is T_e. */
+    const int *rate_kind;           /* (nreac) */
+    const double *rate_coef;        /* (5, nreac), Fortran order */
 } mps_chem;
 
 /* initializechemistry (ion_elec_chem.f90:104-109, H2_O2_chem.f90:433-438,
@@ -294,6 +330,32 @@ int mps_poisson_transport(mps_ctx *ctx, double dt, int timederiv, const double *
 int mps_ion_drift_faces(mps_ctx *ctx, int specnum, const double *efield, const double *numden,
                         const double *ionsoln, double velscale, double *const faces[6]);
 
+/* The FLUX faces of eden_update_bcs (edeneq.f90:203-358): on the faces
+ * whose code (codes[6], the equation's MPS_BC_*) is FLUX, the thermal
+ * outflow 0.25 n_e cbar velscale, cbar = sqrt(8 k_B T_e / (PI m_e)), from
+ * the boundary cells of elecden (scaled) and electemp (ghosted); faces of
+ * other codes, and null face pointers, are left as they are. faces[6] in
+ * the layout mps_set_bc takes. */
+int mps_eden_faces(mps_ctx *ctx, const int codes[6], const double *elecden, const double *electemp,
+                   double velscale, double *const faces[6]);
+
+/* The FLUX faces of neutrals_update_bcs (neutralseq.f90:206-358): 0.25 n
+ * cbar velscale with cbar = sqrt(8 k_B Tg / (PI molmass)). */
+int mps_neutral_faces(mps_ctx *ctx, const int codes[6], const double *neutralden, double molmass, double Tg,
+                      double velscale, double *const faces[6]);
+
+/* eenergy_update_bcs (eenergyeq.f90:208-392): DIRC faces
+ * 1.5 k_B n_e bcparams[f] / eenergyscale, FLUX faces 4/3 n_eps cbar
+ * velscale with cbar from T = n_eps eenergyscale / (1.5 n_e k_B);
+ * elecenergy scaled, elecden physical, both ghosted. */
+int mps_eenergy_faces(mps_ctx *ctx, const int codes[6], const double *elecenergy, const double *elecden,
+                      double eenergyscale, const double bcparams[6], double velscale, double *const faces[6]);
+
+/* The end of eden_solve (edeneq.f90:509-523): interior values of soln below
+ * vmin become vmin, then the ghost values from the context's boundary
+ * conditions (update_boundary_ghostvalues). */
+int mps_floor(mps_ctx *ctx, double *soln, double vmin);
+
 /* update_electrontemp (eenergyeq.f90:842-877): electemp = eenergysoln *
  * eenergyscale / (3/2 n_e k_B) on the interior; below etempmin the
  * temperature is etempmin and eenergysoln (in/out) is reset to match, in the
@@ -331,6 +393,13 @@ int mps_solve_transpose(mps_ctx *ctx, const double *g, double *lambda, int *ncyc
 int mps_grad_velocity(mps_ctx *ctx, const double *lam, const double *x, double *gvx, double *gvy,
                       double *gvz);
 
+/* The grid a context was created for and the number of species of its
+ * chemistry (0 before mps_set_chemistry; nspecies nullable). No reference
+ * counterpart: callers that size arrays themselves (the Fortran wrappers,
+ * whose arrays are dimensioned from lx, ly, lz) check them against the
+ * context before passing them. */
+int mps_get_grid(mps_ctx *ctx, mps_grid *grid, int *nspecies);
+
 /* Timing of the last mps_solve / mps_solve_transpose on the context's stream
  * (HIP events): total milliseconds and per-cycle milliseconds. */
 int mps_last_timing(mps_ctx *ctx, double *total_ms, int *ncycles);
@@ -338,9 +407,11 @@ int mps_last_timing(mps_ctx *ctx, double *total_ms, int *ncycles);
 /* Measurement hook (no reference counterpart): times `reps` back-to-back
  * launches of one hot kernel of the current hierarchy with HIP events on
  * the context's stream, after the context has solved once.
- *   kernel 0  one smoothing sweep on the finest level: the fused red-black
- *             sweep where the operator has no convection term, else one
- *             colour pass
+ *   kernel 0  smoothing on the finest level: the two red-black sweeps of a
+ *             smoothing step in one pass where that kernel applies (FP32
+ *             rows from one coefficient field without convection or
+ *             reaction), else one fused red-black sweep where the operator
+ *             has no convection term, else one colour pass
  *   kernel 1  residual b - A u on the finest level
  *   kernel 2  restriction of the finest residual to level 1
  *   kernel 3  prolongation + correction from level 1 to the finest level
@@ -350,6 +421,16 @@ int mps_last_timing(mps_ctx *ctx, double *total_ms, int *ncycles);
  * bytes) the kernel's symbol for matching profiler output. */
 int mps_time_kernel(mps_ctx *ctx, int kernel, int fp32, int reps, double *avg_ms, double *alg_bytes,
                     char *name, int name_len);
+
+/* Measurement hook (no reference counterpart): the HBM rate a hand-written
+ * streaming kernel reaches on the current device, 16-byte accesses per lane,
+ * arrays of bytes_per_array each (allocated for the call; take them well
+ * above the 256 MB Infinity Cache), timed over `reps` launches with HIP
+ * events. kind 0 read, 1 write, 2 copy, 3 triad o = a + s b, 4 the
+ * red-black sweep's pattern o = a + s (b + c) (three streams read, one
+ * written). avg_ms gets the mean launch time, bytes the bytes one launch
+ * reads and writes. */
+int mps_time_stream(int kind, long long bytes_per_array, int reps, double *avg_ms, double *bytes);
 
 /* The context's HIP stream (hipStream_t), for callers that order work
  * against it. */

@@ -30,7 +30,9 @@ EXPORTS = (
     "mps_fill_ghosts", "mps_gmres", "mps_solve_transpose", "mps_last_timing", "mps_stream", "mps_solve_rhs",
     "mps_efield", "mps_set_chemistry", "mps_species_transport", "mps_eenergy_transport",
     "mps_poisson_transport", "mps_electron_temp", "mps_ion_drift_faces",
-    "mps_error_string", "mps_last_error", "mps_time_kernel", "mps_grad_velocity",
+    "mps_error_string", "mps_last_error", "mps_time_kernel", "mps_grad_velocity", "mps_time_stream",
+    "mps_get_grid", "mps_restrict", "mps_prolong1",
+    "mps_eden_faces", "mps_neutral_faces", "mps_eenergy_faces", "mps_floor", "mps_set_source",
 )
 
 
@@ -50,7 +52,8 @@ class Chem(C.Structure):
                 ("ionspecmin", C.c_int), ("ionspecmax", C.c_int),
                 ("reactants", _vp), ("products", _vp), ("k_arrh", _vp), ("elecenergy", _vp),
                 ("mobility", _vp), ("dcoeff", _vp), ("molmass", _vp), ("charge", _vp),
-                ("mob_kind", _vp), ("mob_terms", _vp), ("mob_coef", _vp), ("dcoeff_kind", _vp)]
+                ("mob_kind", _vp), ("mob_terms", _vp), ("mob_coef", _vp), ("dcoeff_kind", _vp),
+                ("rate_kind", _vp), ("rate_coef", _vp)]
 
 
 class Opts(C.Structure):
@@ -107,6 +110,15 @@ def load():
     L.mps_stream.restype = _vp
     L.mps_grad_velocity.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
     L.mps_time_kernel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_char_p, C.c_int]
+    L.mps_eden_faces.argtypes = [_vp, C.POINTER(C.c_int), _vp, _vp, C.c_double, _vp]
+    L.mps_neutral_faces.argtypes = [_vp, C.POINTER(C.c_int), _vp, C.c_double, C.c_double, C.c_double, _vp]
+    L.mps_eenergy_faces.argtypes = [_vp, C.POINTER(C.c_int), _vp, _vp, C.c_double, _vp, C.c_double, _vp]
+    L.mps_floor.argtypes = [_vp, _vp, C.c_double]
+    L.mps_set_source.argtypes = [_vp, _vp]
+    L.mps_restrict.argtypes = [_vp, _vp, _vp]
+    L.mps_prolong1.argtypes = [_vp, _vp, _vp]
+    L.mps_get_grid.argtypes = [_vp, C.POINTER(Grid), C.POINTER(C.c_int)]
+    L.mps_time_stream.argtypes = [C.c_int, C.c_longlong, C.c_int, _dp, _dp]
     L.mps_error_string.argtypes = [C.c_int]
     L.mps_error_string.restype = C.c_char_p
     L.mps_last_error.argtypes = []

@@ -149,7 +149,11 @@ class LinearSolve(torch.autograd.Function):
         gb = interior(lam).clone()
         gv = [None, None, None]
         if ctx.has_vel and any(ctx.needs_input_grad[2:5]):
-            gv = op.grad_velocity(lam, xg)
+            full = op.grad_velocity(lam, xg)
+            # a gradient only for the components that were passed (and want
+            # one): a component given as None has no gradient slot
+            gv = [full[d] if (v.numel() and ctx.needs_input_grad[2 + d]) else None
+                  for d, v in enumerate((vx, vy, vz))]
         return None, gb, gv[0], gv[1], gv[2], None
 
 

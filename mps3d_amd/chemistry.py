@@ -7,7 +7,8 @@ rows of A, alpha, Ea): Arrhenius rates in the electron temperature, and per
 species a constant mobility and diffusion coefficient (the form of
 ``plasma_pde/chemistry/ion_elec_chem.f90``, :func:`ion_elec`) or the
 field-dependent models of ``plasma_pde/chemistry/H2_O2_chem.f90``
-(:func:`h2_o2`). ``FieldSolver.set_chemistry`` copies it to the device.
+(:func:`h2_o2`); reaction rates Arrhenius or the custom fits of
+``O2_chem.f90`` (:func:`o2`). ``H2_chem.f90`` is :func:`h2`. ``FieldSolver.set_chemistry`` copies it to the device.
 """
 from __future__ import annotations
 
@@ -56,6 +57,9 @@ class Chemistry:
     mob_terms: np.ndarray | None = None    # (nspecies partner, nspecies)
     mob_coef: np.ndarray | None = None     # (3, nspecies partner, nspecies)
     dcoeff_kind: np.ndarray | None = None  # (nspecies,) 0 constant, 1 k_B T_e mu/(Z e), 2 k_B T_g mu/(Z e)
+    # reaction rates (None: every reaction Arrhenius); see mps_chem
+    rate_kind: np.ndarray | None = None    # (nreac,) 0 Arrhenius, 1 exponential fit in 1/T, 2 alpha |E| exp(-E0/|E|)
+    rate_coef: np.ndarray | None = None    # (5, nreac)
 
     @property
     def nspecies(self):
@@ -82,6 +86,9 @@ class Chemistry:
             out["mob_terms"] = np.asfortranarray(self.mob_terms, dtype=np.int32)
             out["mob_coef"] = np.asfortranarray(self.mob_coef, dtype=np.float64)
             out["dcoeff_kind"] = np.ascontiguousarray(self.dcoeff_kind, dtype=np.int32)
+        if self.rate_kind is not None:
+            out["rate_kind"] = np.ascontiguousarray(self.rate_kind, dtype=np.int32)
+            out["rate_coef"] = np.asfortranarray(self.rate_coef, dtype=np.float64)
         return out
 
     def struct(self):
@@ -199,6 +206,84 @@ def h2_o2():
         charge=np.array([0.0, 0.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 0.0, 0.0]),
         especnum=3, bgspecnum=1, ionspecmin=4, ionspecmax=8, neutralspecmin=9, neutralspecmax=10,
         mob_kind=mob_kind, mob_terms=terms, mob_coef=coef, dcoeff_kind=dk)
+
+
+def o2():
+    """plasma_pde/chemistry/O2_chem.f90: O2 (background), E, O+, O2+, O-, O
+    (the solved neutral); 7 reactions, of which 1-3 (ionisation to O2+,
+    dissociative attachment, dissociative ionisation) take getcustomrate's
+    fits in 1/T_e (:413-448, rate_kind 1: A exp(a/T + b/T^2 + c/T^3 + d/T^4)
+    with the double literals of the fit) and 4-7 the Arrhenius form;
+    constant mobilities and diffusion coefficients (:336-381).
+
+    The tables are the doubles assignreactions (:30-202) leaves when
+    compiled, taken from the reference build (oracle/ref_driver/ref_chem4.f90);
+    a test checks them against it. The module's masses come from
+    mgridlib/fundconstants.f90, whose mass_prot equals plasma_pde's."""
+    reactants = np.array([[1, 1, 0, 0, 0, 0], [1, 1, 0, 0, 0, 0], [1, 1, 0, 0, 0, 0], [0, 1, 0, 0, 0, 1],
+                          [0, 0, 0, 1, 1, 0], [0, 0, 1, 0, 1, 0], [0, 1, 0, 0, 1, 0]], dtype=np.int32)
+    products = np.array([[0, 2, 0, 1, 0, 0], [0, 1, 0, 0, 0, 2], [0, 0, 0, 0, 1, 1], [0, 2, 1, 0, 0, 0],
+                         [1, 0, 0, 0, 0, 1], [0, 0, 0, 0, 0, 2], [0, 2, 0, 0, 0, 1]], dtype=np.int32)
+    k_arrh = np.array([
+        [6.683279354557637e-24, 2.0, 146216.703125],
+        [4.229999716356402e-15, 0.0, 64521.01953125],
+        [4.59999969154597e-17, 0.0, 33769.09375],
+        [1.2852985741823762e-17, 0.699999988079071, 157821.203125],
+        [1.9899998665601042e-13, 0.0, 0.0],
+        [2.6599998216331046e-13, 0.0, 0.0],
+        [2.0999998591840297e-16, 0.5, 39400.0],
+    ])
+    elecenergy = np.array([-12.600000381469727, -5.559999942779541, -3.5999999046325684, -13.600000381469727,
+                           0.0, 0.0, -0.9200000166893005])
+    rate_kind = np.array([1, 1, 1, 0, 0, 0, 0], dtype=np.int32)
+    rate_coef = np.zeros((5, 7))
+    rate_coef[:, 0] = (1.043e-13, -2.74e5, -2.0015e8, 5.33e13, -4.001e17)
+    rate_coef[:, 1] = (9.577e-16, 1.246e5, -8.647e9, 1.381e14, -6.953e17)
+    rate_coef[:, 2] = (1.46e-14, 6.21e4, -7.27e9, 1.25e14, -6.57e17)
+    mp = M_PROT
+    return Chemistry(
+        names=("O2", "E", "O+", "O2+", "O-", "O"),
+        reactants=reactants, products=products, k_arrh=k_arrh, elecenergy=elecenergy,
+        mobility=np.array([0.0, -20.0, real4(0.4), real4(0.2), -real4(0.4), 0.0]),
+        dcoeff=np.array([0.0, 100.0, real4(0.02), real4(0.01), real4(0.02), real4(0.02)]),
+        molmass=np.array([32.0 * mp, M_ELEC, 16.0 * mp, 32.0 * mp, 16.0 * mp, 16.0 * mp]),
+        charge=np.array([0.0, -1.0, 1.0, 1.0, -1.0, 0.0]),
+        especnum=2, bgspecnum=1, ionspecmin=3, ionspecmax=5, neutralspecmin=6, neutralspecmax=6,
+        rate_kind=rate_kind, rate_coef=rate_coef)
+
+
+def h2():
+    """plasma_pde/chemistry/H2_chem.f90: H2 (background), E, H+, H2+, H (the
+    solved neutral); 8 Arrhenius reactions (every isratearrh of
+    assignreactions, :30-220, is .true., so its field-dependent getcustomrate
+    at :423-437 is never reached) and constant mobilities and diffusion
+    coefficients (:350-391). Tables as the reference build holds them
+    (oracle/ref_driver/ref_chem4.f90), checked by a test."""
+    reactants = np.array([[0, 1, 0, 0, 1], [1, 1, 0, 0, 0], [0, 1, 0, 1, 0], [0, 1, 0, 1, 0], [0, 0, 0, 1, 1],
+                          [1, 0, 1, 0, 0], [1, 1, 0, 0, 0], [1, 1, 0, 0, 0]], dtype=np.int32)
+    products = np.array([[0, 2, 1, 0, 0], [0, 2, 1, 0, 1], [0, 1, 1, 0, 1], [0, 2, 2, 0, 0], [1, 0, 1, 0, 0],
+                         [0, 0, 0, 1, 1], [0, 2, 0, 1, 0], [0, 1, 0, 0, 2]], dtype=np.int32)
+    k_arrh = np.array([
+        [6.671210258802429e-17, 0.48930999636650085, 149624.359375],
+        [4.673050547002412e-16, 0.4445599913597107, 437818.75],
+        [6.780687508393681e-14, 0.04876000061631203, 112450.8515625],
+        [1.122866241117621e-16, 0.31393998861312866, 270371.5],
+        [8.999999396502984e-16, 0.0, 0.0],
+        [1.1899999202042835e-28, 0.0, 0.0],
+        [6.269267675269048e-15, 0.17156000435352325, 232987.484375],
+        [1.8635879765620025e-08, -1.236680030822754, 146128.84375],
+    ])
+    elecenergy = np.array([-12.89365005493164, -37.72835922241211, -9.690279960632324, -23.29884910583496,
+                           0.0, 0.0, -20.07733917236328, -12.592430114746094])
+    mp = M_PROT
+    return Chemistry(
+        names=("H2", "E", "H+", "H2+", "H"),
+        reactants=reactants, products=products, k_arrh=k_arrh, elecenergy=elecenergy,
+        mobility=np.array([0.0, -20.0, real4(0.4), real4(0.2), 0.0]),
+        dcoeff=np.array([0.0, 100.0, real4(0.02), real4(0.01), real4(0.01)]),
+        molmass=np.array([2.0 * mp, M_ELEC, 1.0 * mp, 2.0 * mp, 1.0 * mp]),
+        charge=np.array([0.0, -1.0, 1.0, 1.0, 0.0]),
+        especnum=2, bgspecnum=1, ionspecmin=3, ionspecmax=4, neutralspecmin=5, neutralspecmax=5)
 
 
 def etempmin_default():

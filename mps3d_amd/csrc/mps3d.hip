@@ -7,8 +7,10 @@
 #include "mps3d_bottom.h"
 #include "mps3d_kernels.h"
 #include "mps3d_sweep.h"
+#include "mps3d_sweep2.h"
 #include "mps3d_lds_cycle.h"
 #include "mps3d_transport.h"
+#include "mps3d_stream.h"
 
 #include <hip/hip_runtime.h>
 
@@ -185,6 +187,8 @@ struct mps_ctx {
     int last_cycles = 0;
     double *dinv = nullptr;              // dense inverse of the coarsest operator
     double *dinvT = nullptr;             // and of its transpose
+    double *bst[ST_N] = {};              // coarsest-level rows for a device-built inverse
+    double *dwork = nullptr;             // its work matrix when it does not fit LDS
     double *gbuf = nullptr;              // ghosted right-hand side of adjoint solves
     double *fmg_r = nullptr, *fmg_e = nullptr;  // FP64 defect and correction of a full-multigrid start
     bool exact_now = false;              // reference-mirroring call in progress
@@ -192,8 +196,15 @@ struct mps_ctx {
     // outer_step): [0] the V-cycle step, [1] the full-multigrid first step
     bool graphs = true;                  // MPS3D_NO_GRAPHS=1 turns replay off
     bool fused_rr = true;                // MPS3D_NO_FUSED_RESTRICT=1: residual and restriction apart
+    bool sweep_ref = false;              // k_sweep_ref for OP_REF rows (MPS3D_SWEEP_REF=1: on)
+    int sweep2 = 1;                      // precisions whose smoothing pairs run in one pass
+                                         // (bit 1 FP32, bit 2 FP64; MPS3D_SWEEP2 overrides)
     StepGraph step[2];
-    unsigned gen = 0;                    // bumped by set_bc / set_coeffs
+    unsigned gen = 0;                    // bumped when set_bc / set_coeffs change what
+                                         // captured step graphs bake in (see touch_gen)
+    int sig_bc[6] = {-1, -1, -1, -1, -1, -1};
+    int sig_flags = -1;                  // has_vel | has_reac << 1 | timederiv << 2
+    double sig_dt = 0.0;
     ChemDev *chem = nullptr;             // chemistry tables (mps_set_chemistry)
     ChemDev chem_host{};
     bool mg_ok = true;                   // a multigrid hierarchy exists
@@ -277,6 +288,22 @@ int level_op(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l)
 int level_op(const mps_ctx *c, int l) { return c->geom[l].op; }
 
 // ---------------------------------------------------------------- hierarchy
+// Captured step graphs bake in the kernels' arguments: buffer pointers
+// (fixed once allocated) and the level views' BC codes, coefficient flags and
+// time step. New boundary values or coefficient values leave them valid;
+// a change of those settings invalidates them.
+void touch_gen(mps_ctx *c)
+{
+    const int flags = (c->has_vel ? 1 : 0) | (c->has_reac ? 2 : 0) | (c->timederiv ? 4 : 0);
+    bool same = flags == c->sig_flags && c->dt == c->sig_dt;
+    for (int f = 0; f < 6; ++f) same = same && c->bc[f] == c->sig_bc[f];
+    if (same) return;
+    for (int f = 0; f < 6; ++f) c->sig_bc[f] = c->bc[f];
+    c->sig_flags = flags;
+    c->sig_dt = c->dt;
+    ++c->gen;
+}
+
 // keeps only the finest level: operator-level entry points still work,
 // multigrid entry points report the reason
 int no_hierarchy(mps_ctx *c, const char *why)
@@ -450,7 +477,10 @@ void alloc_levels(mps_ctx *c, std::vector<LevelBuf<T>> &L, bool with_coeffs)
         B.b = dalloc<T>(ng);
         B.r = dalloc<T>(ng);
         if (l > 0) B.uold = dalloc<T>(ng);
-        if (with_coeffs) B.D = dalloc<T>(ng);
+        if (with_coeffs) {
+            B.D = dalloc<T>(ng);
+            fill<T>(c, B.D, ng, T(0));
+        }
         for (int d = 0; d < 3; ++d)
             if (!g.w[d].empty()) {
                 B.w[d] = dalloc<double>(g.w[d].size());
@@ -750,6 +780,18 @@ void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *wo
 #endif
 constexpr int kSweepTX = MPS_SWEEP_TX, kSweepTY = MPS_SWEEP_TY, kSweepKZ = MPS_SWEEP_KZ;
 
+// per-face multipliers of the staged ghost coefficient (k_sweep_ref,
+// k_sweep2_fast): -1 on a no-flux face, g/h - 1 on a DIRC face, so that the
+// face weight (D_c + D_ghost) h is zero or the wall term D_c g
+template <typename T>
+GhostD<T> ghost_multipliers(const mps_ctx *c, const Lev<T> &v)
+{
+    const double gf[6] = {v.gx, v.gx, v.gy, v.gy, v.gz, v.gz}, hf[6] = {v.hx, v.hx, v.hy, v.hy, v.hz, v.hz};
+    GhostD<T> G{};
+    for (int f = 0; f < 6; ++f) G.m[f] = c->bc[f] == MPS_BC_DIRC ? (T)(gf[f] / hf[f] - 1.0) : T(-1);
+    return G;
+}
+
 // zero_in: sweep from a zero guess, uin unread (fused fast kernels only)
 template <typename T>
 void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, T *uout, int op, bool zero_in = false)
@@ -762,6 +804,17 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
     const int kz = z_chunk(g, kSweepTX, kSweepTY, MPS_SWEEP_TARGET, kSweepKZ);
     dim3 gr((unsigned)((g.nx + kSweepTX - 1) / kSweepTX), (unsigned)((g.ny + kSweepTY - 1) / kSweepTY),
             (unsigned)((g.nz + kz - 1) / kz));
+    if (v.fast && op == OP_REF && !c->has_reac && c->sweep_ref && g.ng() * (long long)sizeof(T) < (1LL << 31)) {
+        // uniform face factors: boundary rows through staged ghost coefficients
+        GhostD<T> G = ghost_multipliers(c, v);
+        if (zero_in)
+            hipLaunchKernelGGL((k_sweep_ref<T, kSweepTX, kSweepTY, true>), gr, dim3(kSweepTX * kSweepTY), 0,
+                               c->stream, v, G, uin, uout, (const T *)L[l].b, kz);
+        else
+            hipLaunchKernelGGL((k_sweep_ref<T, kSweepTX, kSweepTY, false>), gr, dim3(kSweepTX * kSweepTY), 0,
+                               c->stream, v, G, uin, uout, (const T *)L[l].b, kz);
+        return;
+    }
     auto go = [&](auto O, auto R) {
         if (v.fast && zero_in)
             hipLaunchKernelGGL(
@@ -805,6 +858,52 @@ bool fused_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op
     return (op == OP_STENCIL || !c->has_vel) && L[l].u2 && c->geom[l].ng() < (1LL << 31);
 }
 
+// Two sweeps in one pass (k_sweep2_fast): FP32 fast rows from one
+// coefficient field with uniform face factors (OP_REF: the finest level of
+// either hierarchy), no convection or reaction term, fields below 2^31 bytes.
+#ifndef MPS_SWEEP2_TX
+#define MPS_SWEEP2_TX 32
+#endif
+#ifndef MPS_SWEEP2_TY
+#define MPS_SWEEP2_TY 8
+#endif
+#ifndef MPS_SWEEP2_KZ
+#define MPS_SWEEP2_KZ 48
+#endif
+#ifndef MPS_SWEEP2_TARGET
+#define MPS_SWEEP2_TARGET 1536
+#endif
+#ifndef MPS_SWEEP2_PREC
+#define MPS_SWEEP2_PREC 0   // bit 1: FP32, bit 2: FP64
+#endif
+constexpr int kSw2TX = MPS_SWEEP2_TX, kSw2TY = MPS_SWEEP2_TY;
+
+template <typename T>
+bool sweep2_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op)
+{
+    if (!(c->sweep2 & (sizeof(T) == 4 ? 1 : 2)) || op != OP_REF || c->has_reac || c->has_vel || !L[l].u2)
+        return false;
+    if (c->geom[l].ng() * (long long)sizeof(T) >= (1LL << 31)) return false;
+    return view(c, L, l).fast != 0;
+}
+
+template <typename T>
+void launch_sweep2(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, T *uout, bool zero_in)
+{
+    Lev<T> v = view(c, L, l);
+    const Geom &g = c->geom[l];
+    GhostD<T> G = ghost_multipliers(c, v);
+    const int kz = z_chunk(g, kSw2TX, kSw2TY, MPS_SWEEP2_TARGET, MPS_SWEEP2_KZ);
+    dim3 gr((unsigned)((g.nx + kSw2TX - 1) / kSw2TX), (unsigned)((g.ny + kSw2TY - 1) / kSw2TY),
+            (unsigned)((g.nz + kz - 1) / kz));
+    if (zero_in)
+        hipLaunchKernelGGL((k_sweep2_fast<T, kSw2TX, kSw2TY, true>), gr, dim3(kSweep2Threads), 0, c->stream, v, G,
+                           uin, uout, (const T *)L[l].b, kz);
+    else
+        hipLaunchKernelGGL((k_sweep2_fast<T, kSw2TX, kSw2TY, false>), gr, dim3(kSweep2Threads), 0, c->stream, v, G,
+                           uin, uout, (const T *)L[l].b, kz);
+}
+
 // nu red-black sweeps on level l: pairs of fused sweeps u -> u2 -> u where
 // the fused kernels apply, in-place colour passes otherwise. swap_odd (the
 // cycle's pre- and post-smoothing counts add up to an even number): an odd
@@ -815,6 +914,14 @@ void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op
                   bool swap_odd = false)
 {
     int s = 0;
+    // pairs in one pass swap the level's u and u2: used when the cycle's
+    // pre- and post-smoothing pairs add up to an even number of swaps
+    if (sweep2_ok(c, L, l, op) && (c->opts.nu1 / 2 + c->opts.nu2 / 2) % 2 == 0) {
+        for (; s + 2 <= nu; s += 2) {
+            launch_sweep2<T>(c, L, l, L[l].u, L[l].u2, zero_first && s == 0);
+            std::swap(L[l].u, L[l].u2);
+        }
+    }
     if (fused_ok(c, L, l, op)) {
         for (; s + 2 <= nu; s += 2) {
             launch_sweep<T>(c, L, l, L[l].u, L[l].u2, op, zero_first && s == 0);
@@ -996,24 +1103,24 @@ void fmg(mps_ctx *c, std::vector<LevelBuf<T>> &L, T *bwork)
 // ---------------------------------------------------------- coefficients
 // coarse coefficients from the finer level (REF: 8-child mean as the
 // reference does every cycle; ALIGNED: volume-weighted full weighting)
+// (the coarse arrays' ghost layers are zeroed when they are allocated and
+// never written: the restrictions write interiors only)
 void derive_coarse(mps_ctx *c)
 {
     auto &L = c->L64;
     for (int l = 1; l < c->nlev(); ++l) {
         const Geom &gc = c->geom[l];
         Lev<double> vf = view(c, L, l - 1), vc = view(c, L, l);
-        double *src[5] = {L[l - 1].D, L[l - 1].vx, L[l - 1].vy, L[l - 1].vz, L[l - 1].reac};
-        double *dst[5] = {L[l].D, L[l].vx, L[l].vy, L[l].vz, L[l].reac};
-        for (int q = 0; q < 5; ++q) {
-            if (!src[q]) continue;
-            fill<double>(c, dst[q], gc.ng(), 0.0);
-            if (c->opts.hierarchy == MPS_HIER_REF) {
-                hipLaunchKernelGGL(k_restrict_mean<double>, cells_grid(gc), block(), 0, c->stream, vf, vc,
-                                   (const double *)src[q], dst[q]);
-            } else {
-                AlignedW W{L[l].w[0], L[l].w[1], L[l].w[2]};
+        FieldSet5 F{{L[l - 1].D, L[l - 1].vx, L[l - 1].vy, L[l - 1].vz, L[l - 1].reac},
+                    {L[l].D, L[l].vx, L[l].vy, L[l].vz, L[l].reac}};
+        if (c->opts.hierarchy == MPS_HIER_REF) {
+            hipLaunchKernelGGL(k_restrict_mean5, cells_grid(gc), block(), 0, c->stream, vf, vc, F);
+        } else {
+            AlignedW W{L[l].w[0], L[l].w[1], L[l].w[2]};
+            for (int q = 0; q < 5; ++q) {
+                if (!F.src[q]) continue;
                 hipLaunchKernelGGL(k_restrict_fw<double>, cells_grid(gc), block(), 0, c->stream, vf, vc, W,
-                                   (const double *)src[q], dst[q]);
+                                   (const double *)F.src[q], F.dst[q]);
             }
         }
     }
@@ -1449,10 +1556,46 @@ int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, do
 // coarsest-level dense inverse for the aligned hierarchy: assembles the
 // coarsest operator from its 7-point rows and inverts it (Gauss-Jordan with
 // partial pivoting, FP64) once per set of coefficients
+// Reference hierarchy in fast arithmetic: the coarsest operator's inverse
+// built on the device (k_dense_inverse) once per set of coefficients; the
+// V-cycle's coarse GMRES becomes one matrix-vector product.
+void build_bottom_inverse_device(mps_ctx *c)
+{
+    const int lb = c->nlev() - 1;
+    const Geom &g = c->geom[lb];
+    const int n = (int)g.n();
+    if (n > kBottomMaxDense) return;   // k_bottom stays
+    const long long ng = g.ng();
+    for (int q = 0; q < ST_N; ++q)
+        if (!c->bst[q]) {
+            c->bst[q] = dalloc<double>(ng);
+            fill<double>(c, c->bst[q], ng, 0.0);
+        }
+    if (!c->dinv) c->dinv = dalloc<double>((size_t)kBottomMaxDense * kBottomMaxDense);
+    if (n > kDenseLdsN && !c->dwork) c->dwork = dalloc<double>((size_t)kBottomMaxDense * kBottomMaxDense);
+    Lev<double> v = view(c, c->L64, lb);
+    with_op(level_op(c, lb), c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
+        hipLaunchKernelGGL((k_build_stencil<double, decltype(O)::value, decltype(V)::value, decltype(R)::value>),
+                           cells_grid(g), block(), 0, c->stream, v, c->bst[0], c->bst[1], c->bst[2], c->bst[3],
+                           c->bst[4], c->bst[5], c->bst[6], 0);
+    });
+    hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(kDenseThreads), 0, c->stream, v, (const double *)c->bst[0],
+                       (const double *)c->bst[1], (const double *)c->bst[2], (const double *)c->bst[3],
+                       (const double *)c->bst[4], (const double *)c->bst[5], (const double *)c->bst[6], c->dwork,
+                       c->dinv, (int *)nullptr);
+    HIPCHK(hipGetLastError());
+    c->L64[lb].inv = c->dinv;
+    if (!c->L32.empty()) c->L32[lb].inv = c->dinv;
+}
+
 void build_bottom_inverse(mps_ctx *c)
 {
     for (auto &B : c->L64) B.inv = nullptr;
     for (auto &B : c->L32) B.inv = nullptr;
+    if (c->mg_ok && c->opts.hierarchy == MPS_HIER_REF && c->opts.arith == MPS_ARITH_FAST) {
+        build_bottom_inverse_device(c);
+        return;
+    }
     if (!c->mg_ok || c->opts.hierarchy != MPS_HIER_ALIGNED) return;
     const int lb = c->nlev() - 1;
     const Geom &g = c->geom[lb];
@@ -1538,6 +1681,18 @@ void build_bottom_inverse(mps_ctx *c)
 
 void free_gmres(GmresBufs &G);
 
+// a bare level view of an nx x ny x nz ghosted grid (addressing only)
+Lev<double> grid_view(int nx, int ny, int nz)
+{
+    Lev<double> v{};
+    v.nx = nx;
+    v.ny = ny;
+    v.nz = nz;
+    v.sy = nx + 2;
+    v.sz = (long long)(nx + 2) * (ny + 2);
+    return v;
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1604,6 +1759,8 @@ int mps_destroy(mps_ctx *c)
     dfree(c->user_tmp2);
     dfree(c->dinv);
     dfree(c->dinvT);
+    for (auto p : c->bst) dfree(p);
+    dfree(c->dwork);
     dfree(c->gbuf);
     dfree(c->fmg_r);
     dfree(c->fmg_e);
@@ -1659,6 +1816,10 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
             c->graphs = !(ng && ng[0] == '1');
             const char *nf = std::getenv("MPS3D_NO_FUSED_RESTRICT");
             c->fused_rr = !(nf && nf[0] == '1');
+            const char *sr = std::getenv("MPS3D_SWEEP_REF");
+            c->sweep_ref = sr && sr[0] == '1';
+            const char *n2 = std::getenv("MPS3D_SWEEP2");
+            c->sweep2 = (n2 && n2[0] >= '0' && n2[0] <= '3') ? n2[0] - '0' : MPS_SWEEP2_PREC;
         }
         HIPCHK(hipEventCreate(&c->ev0));
         HIPCHK(hipEventCreate(&c->ev1));
@@ -1767,7 +1928,6 @@ void set_bc_device(mps_ctx *c, const double *const faces[6])
         }
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
 }
 
 int mps_set_bc(mps_ctx *c, const int codes[6], const double *const faces[6])
@@ -1787,7 +1947,7 @@ int mps_set_bc(mps_ctx *c, const int codes[6], const double *const faces[6])
             if (st) return st;
             set_bc_device(c, faces);
             c->bc_set = true;
-            ++c->gen;
+            touch_gen(c);
             c->transpose_ready = false;
             return MPS_OK;
         }
@@ -1841,7 +2001,7 @@ int mps_set_bc(mps_ctx *c, const int codes[6], const double *const faces[6])
             }
         }
         c->bc_set = true;
-        ++c->gen;
+        touch_gen(c);
         c->transpose_ready = false;
     } catch (const HipError &e) {
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
@@ -1884,7 +2044,10 @@ int mps_set_coeffs(mps_ctx *c, const double *dcoeff, const double *velx, const d
             double **vd[4] = {&L[l].vx, &L[l].vy, &L[l].vz, &L[l].reac};
             const bool want[4] = {c->has_vel, c->has_vel, c->has_vel, c->has_reac};
             for (int q = 0; q < 4; ++q) {
-                if (want[q] && !*vd[q]) *vd[q] = dalloc<double>(c->geom[l].ng());
+                if (want[q] && !*vd[q]) {
+                    *vd[q] = dalloc<double>(c->geom[l].ng());
+                    fill<double>(c, *vd[q], c->geom[l].ng(), 0.0);
+                }
                 if (!want[q] && *vd[q]) {
                     dfree(*vd[q]);
                     *vd[q] = nullptr;
@@ -1915,10 +2078,34 @@ int mps_set_coeffs(mps_ctx *c, const double *dcoeff, const double *velx, const d
         }
         if (c->opts.precision == MPS_PREC_FP32_DC) build_fp32(c);
         build_bottom_inverse(c);
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipGetLastError());
+        // host arrays were staged: complete before the call returns; device
+        // arrays: the set-up stays ordered on the context's stream
+        bool host_in = false;
+        for (const double *p : {dcoeff, velx, vely, velz, reac, source})
+            if (p && !is_device_ptr(p)) host_in = true;
+        if (host_in) HIPCHK(hipStreamSynchronize(c->stream));
         c->coeffs_set = true;
-        ++c->gen;
+        touch_gen(c);
         c->transpose_ready = false;
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
+}
+
+int mps_set_source(mps_ctx *c, const double *source)
+{
+    if (!c) return fail(MPS_ERR_ARG, "null context");
+    if (!c->coeffs_set) return fail(MPS_ERR_STATE, "set the coefficients first (mps_set_coeffs)");
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        if (int st = after_null_stream(c)) return st;
+        const long long ng = c->geom[0].ng();
+        if (source) stage_in(c, source, c->source, ng);
+        else fill<double>(c, c->source, ng, 0.0);
+        HIPCHK(hipGetLastError());
+        if (source && !is_device_ptr(source)) HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const HipError &e) {
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
     }
@@ -2006,6 +2193,15 @@ int mps_smooth(mps_ctx *c, double *soln, const double *sterm, int niter, double 
         L[0].b = c->b0;
         const int op = level_op(c, 0);
         smooth_level<double>(c, L, 0, niter, op);
+        if (L[0].u != u) {
+            // an odd number of sweep pairs in one pass left the result in the
+            // level's other buffer: its interior back into soln
+            Lev<double> v = view(c, L, 0);
+            hipLaunchKernelGGL(k_copy_interior<double>, cells_grid(g), block(), 0, c->stream, v,
+                               (const double *)L[0].u, u);
+            L[0].u2 = L[0].u;
+            L[0].u = u;
+        }
         if (res) {
             launch_residual<double, double>(c, L, 0, u, c->b0, L[0].r, nullptr, op);
             Lev<double> v = view(c, L, 0);
@@ -2038,8 +2234,8 @@ int mps_fill_ghosts(mps_ctx *c, double *soln)
         for (int f = 0; f < 6; ++f) F.f[f] = c->faces0[f];
         long long nt = (long long)g.ny * g.nz + (long long)g.nx * g.nz + (long long)g.nx * g.ny;
         hipLaunchKernelGGL(k_ghosts, dim3((unsigned)((nt + kBlock - 1) / kBlock)), block(), 0, c->stream, v, u, F);
+        HIPCHK(hipGetLastError());
         unbind_solution(c, soln, staged);
-        HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const HipError &e) {
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
     }
@@ -2074,13 +2270,13 @@ int mps_efield(mps_ctx *c, double *phi, double *efield, double scale, int limite
         for (int f = 0; f < 6; ++f) z.bc[f] = BC_ZGRD;
         const Faces none{};
         for (int q = 0; q < 3; ++q) hipLaunchKernelGGL(k_ghosts, gg, block(), 0, c->stream, z, E + q * ng, none);
+        HIPCHK(hipGetLastError());
         unbind_solution(c, phi, staged);
         if (!dev_out) {
             HIPCHK(hipMemcpyAsync(efield, E, 3 * ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
             dfree(E);
         }
-        HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const HipError &e) {
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
     }
@@ -2126,12 +2322,14 @@ struct Staging {
         back_n.push_back(n);
         return d;
     }
+    // host outputs copied back and the stream synchronized; with device
+    // arrays only, the results stay ordered on the context's stream
     void finish()
     {
         for (size_t q = 0; q < back.size(); ++q)
             HIPCHK(hipMemcpyAsync(back[q].first, back[q].second, back_n[q] * sizeof(double), hipMemcpyDeviceToHost,
                                   c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        if (!back.empty() || !owned.empty()) HIPCHK(hipStreamSynchronize(c->stream));
     }
 };
 
@@ -2223,6 +2421,14 @@ extern "C" int mps_set_chemistry(mps_ctx *c, const mps_chem *chem)
             for (int q = 0; q < 3; ++q) h.mco[s2][j][q] = chem->mob_coef[q + 3 * (j + (size_t)ns * s2)];
         }
         if (mk != 0 && nterms == 0) return fail(MPS_ERR_ARG, "a mobility fit needs at least one partner species");
+    }
+    if ((chem->rate_kind == nullptr) != (chem->rate_coef == nullptr))
+        return fail(MPS_ERR_ARG, "rate_kind and rate_coef go together");
+    for (int r = 0; r < nr; ++r) {
+        const int rk = chem->rate_kind ? chem->rate_kind[r] : 0;
+        if (rk < 0 || rk > 2) return fail(MPS_ERR_ARG, "rate_kind must be 0..2");
+        h.rkind[r] = rk;
+        for (int q = 0; q < 5; ++q) h.rco[r][q] = chem->rate_coef ? chem->rate_coef[q + 5 * (size_t)r] : 0.0;
     }
     try {
         if (!c->chem) c->chem = dalloc<ChemDev>(1);
@@ -2400,6 +2606,100 @@ extern "C" int mps_ion_drift_faces(mps_ctx *c, int specnum, const double *efield
     return MPS_OK;
 }
 
+// the boundary updaters' faces (k_thermal_faces): kind as the kernel's
+static int thermal_faces(mps_ctx *c, int kind, const int codes[6], const double *n, const double *T, double c1,
+                         double Tg, double escale, const double twall[6], double vscale, double *const faces[6])
+{
+    int st = guard(c, false);
+    if (st) return st;
+    if (!codes || !faces || !n || (kind != 1 && !T)) return fail(MPS_ERR_ARG, "null argument");
+    for (int f = 0; f < 6; ++f)
+        if (codes[f] < 0 || codes[f] > 3) return fail(MPS_ERR_ARG, "bad BC code %d", codes[f]);
+    const Geom &g = c->geom[0];
+    const size_t ng = (size_t)g.ng();
+    try {
+        Staging S(c);
+        const double *nd = S.in(n, ng);
+        const double *Td = T ? S.in(T, ng) : nullptr;
+        const size_t fsz[6] = {(size_t)g.ny * g.nz, (size_t)g.ny * g.nz, (size_t)g.nx * g.nz,
+                               (size_t)g.nx * g.nz, (size_t)g.nx * g.ny, (size_t)g.nx * g.ny};
+        ThermalFaces F{};
+        size_t most = 0;
+        for (int f = 0; f < 6; ++f) {
+            F.code[f] = codes[f];
+            F.twall[f] = twall ? twall[f] : 0.0;
+            // faces of other codes keep their values: in/out
+            F.f[f] = S.out(faces[f], fsz[f], true);
+            if (faces[f]) most = std::max(most, fsz[f]);
+        }
+        if (most > 0) {
+            Lev<double> v = view(c, c->L64, 0);
+            hipLaunchKernelGGL(k_thermal_faces, dim3((unsigned)((most + 255) / 256), 6), dim3(256), 0, c->stream, v,
+                               kind, nd, Td, c1, Tg, escale, vscale, F);
+            HIPCHK(hipGetLastError());
+        }
+        S.finish();
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
+}
+
+// const_1 = 8 k_B / PI / m of the updaters, PI the reference's default-real
+// literal (plasma_pde/fundconstants.f90:8)
+static double thermal_c1(double mass)
+{
+    const double pi_ref = (double)3.14159265f;
+    return 8.0 * kBoltz / pi_ref / mass;
+}
+
+extern "C" int mps_eden_faces(mps_ctx *c, const int codes[6], const double *elecden, const double *electemp,
+                              double velscale, double *const faces[6])
+{
+    return thermal_faces(c, 0, codes, elecden, electemp, thermal_c1(9.109382e-31), 0.0, 0.0, nullptr, velscale,
+                         faces);
+}
+
+extern "C" int mps_neutral_faces(mps_ctx *c, const int codes[6], const double *neutralden, double molmass,
+                                 double Tg, double velscale, double *const faces[6])
+{
+    if (!(molmass > 0.0)) return fail(MPS_ERR_ARG, "molmass must be > 0");
+    return thermal_faces(c, 1, codes, neutralden, nullptr, thermal_c1(molmass), Tg, 0.0, nullptr, velscale, faces);
+}
+
+extern "C" int mps_eenergy_faces(mps_ctx *c, const int codes[6], const double *elecenergy, const double *elecden,
+                                 double eenergyscale, const double bcparams[6], double velscale,
+                                 double *const faces[6])
+{
+    if (!bcparams) return fail(MPS_ERR_ARG, "null bcparams");
+    return thermal_faces(c, 2, codes, elecenergy, elecden, thermal_c1(9.109382e-31), 0.0, eenergyscale, bcparams,
+                         velscale, faces);
+}
+
+extern "C" int mps_floor(mps_ctx *c, double *soln, double vmin)
+{
+    int st = guard(c, false);
+    if (st) return st;
+    if (!c->bc_set) return fail(MPS_ERR_STATE, "set boundary conditions first");
+    if (!soln) return fail(MPS_ERR_ARG, "null soln");
+    try {
+        const Geom &g = c->geom[0];
+        bool staged;
+        double *u = bind_solution(c, soln, staged);
+        Lev<double> v = view(c, c->L64, 0);
+        hipLaunchKernelGGL(k_floor, cells_grid(g), block(), 0, c->stream, v, u, vmin);
+        Faces F{};
+        for (int f = 0; f < 6; ++f) F.f[f] = c->faces0[f];
+        const long long nt = (long long)g.ny * g.nz + (long long)g.nx * g.nz + (long long)g.nx * g.ny;
+        hipLaunchKernelGGL(k_ghosts, dim3((unsigned)((nt + kBlock - 1) / kBlock)), block(), 0, c->stream, v, u, F);
+        HIPCHK(hipGetLastError());
+        unbind_solution(c, soln, staged);
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return MPS_OK;
+}
+
 extern "C" int mps_electron_temp(mps_ctx *c, double *eenergysoln, const double *elecden, double eenergyscale,
                                  double etempmin, double *electemp)
 {
@@ -2531,6 +2831,83 @@ int mps_solve_rhs(mps_ctx *c, const double *bsrc, double *soln, int *ncycles, do
     return status;
 }
 
+int mps_restrict(mps_ctx *c, const double *fine, double *coarse)
+{
+    int st = guard(c, false);
+    if (st) return st;
+    if (!fine || !coarse) return fail(MPS_ERR_ARG, "null argument");
+    const Geom &g = c->geom[0];
+    if ((g.nx & 1) || (g.ny & 1) || (g.nz & 1)) return fail(MPS_ERR_ARG, "restriction needs even nx, ny, nz");
+    std::vector<double *> tmp;
+    try {
+        const Lev<double> vf = grid_view(g.nx, g.ny, g.nz), vc = grid_view(g.nx / 2, g.ny / 2, g.nz / 2);
+        const long long ngc = (long long)(vc.nx + 2) * (vc.ny + 2) * (vc.nz + 2), nc = (long long)vc.nx * vc.ny * vc.nz;
+        for (int q = 0; q < 4; ++q) tmp.push_back(dalloc<double>(q == 0 ? g.ng() : ngc));
+        fill<double>(c, tmp[0], g.ng(), 0.0);
+        hipLaunchKernelGGL(k_interior_in<double>, cells_grid(g), block(), 0, c->stream, vf, stage_interior(c, fine),
+                           tmp[0]);
+        // the V-cycle's kernel: u and r restricted together (here both are fine)
+        Geom gc{};
+        gc.nx = vc.nx, gc.ny = vc.ny, gc.nz = vc.nz;
+        hipLaunchKernelGGL(k_restrict_fas<double>, cells_grid(gc), block(), 0, c->stream, vf, vc,
+                           (const double *)tmp[0], (const double *)tmp[0], tmp[1], tmp[2], tmp[3]);
+        double *out = is_device_ptr(coarse) ? coarse : c->user_tmp2;
+        hipLaunchKernelGGL(k_interior_out<double>, cells_grid(gc), block(), 0, c->stream, vc, (const double *)tmp[2],
+                           out);
+        if (out != coarse)
+            HIPCHK(hipMemcpyAsync(coarse, out, nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError &e) {
+        for (auto p : tmp) dfree(p);
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    for (auto p : tmp) dfree(p);
+    return MPS_OK;
+}
+
+int mps_prolong1(mps_ctx *c, const double *coarse, double *fine)
+{
+    int st = guard(c, false);
+    if (st) return st;
+    if (!fine || !coarse) return fail(MPS_ERR_ARG, "null argument");
+    const Geom &g = c->geom[0];
+    if ((g.nx & 1) || (g.ny & 1) || (g.nz & 1)) return fail(MPS_ERR_ARG, "prolongation needs even nx, ny, nz");
+    std::vector<double *> tmp;
+    try {
+        const Lev<double> vf = grid_view(g.nx, g.ny, g.nz), vc = grid_view(g.nx / 2, g.ny / 2, g.nz / 2);
+        const long long ngc = (long long)(vc.nx + 2) * (vc.ny + 2) * (vc.nz + 2);
+        for (int q = 0; q < 3; ++q) tmp.push_back(dalloc<double>(q == 0 ? g.ng() : ngc));
+        fill<double>(c, tmp[0], g.ng(), 0.0);   // fine, zero: the kernel adds the interpolant
+        fill<double>(c, tmp[2], ngc, 0.0);      // the "old" coarse values: zero
+        stage_in(c, coarse, tmp[1], ngc);
+        Geom gc{};
+        gc.nx = vc.nx, gc.ny = vc.ny, gc.nz = vc.nz;
+        const FacesT<double> none{};
+        hipLaunchKernelGGL((k_prolong1_correct<double, true>), cells_grid(gc), block(), 0, c->stream, vf, vc, tmp[0],
+                           (const double *)tmp[1], (const double *)tmp[2], none);
+        double *out = is_device_ptr(fine) ? fine : c->user_tmp2;
+        hipLaunchKernelGGL(k_interior_out<double>, cells_grid(g), block(), 0, c->stream, vf, (const double *)tmp[0],
+                           out);
+        if (out != fine) HIPCHK(hipMemcpyAsync(fine, out, g.n() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError &e) {
+        for (auto p : tmp) dfree(p);
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    for (auto p : tmp) dfree(p);
+    return MPS_OK;
+}
+
+int mps_get_grid(mps_ctx *c, mps_grid *grid, int *nspecies)
+{
+    if (!c || !grid) return fail(MPS_ERR_ARG, "null argument");
+    *grid = c->grid;
+    if (nspecies) *nspecies = c->chem ? c->chem_host.ns : 0;
+    return MPS_OK;
+}
+
 int mps_last_timing(mps_ctx *c, double *total_ms, int *ncycles)
 {
     if (!c) return fail(MPS_ERR_ARG, "null context");
@@ -2651,6 +3028,35 @@ double multiaxpy(mps_ctx *c, GmresBufs &G, double *w, double *const *vs, int nv,
     return read_scalar(c);
 }
 
+// performgmres with the reference's preconditioner on the finest level (at
+// most kBottomMax cells): b (interior, device) into the level's zero-ghost
+// right-hand side buffer, then k_bottom on x (ghosted, device) with `work`
+// (bottom_work_elems doubles; k_bottom initialises every part it reads).
+// Queues the copy of the kernel's NaN status for gmres_ref_status.
+void gmres_ref_launch(mps_ctx *c, const double *bd, double *u, int m, int max_restarts, double tol, double *work)
+{
+    const Geom &g = c->geom[0];
+    double *bg = c->L64[0].r;
+    fill<double>(c, bg, g.ng(), 0.0);
+    Lev<double> v = view(c, c->L64, 0);
+    hipLaunchKernelGGL(k_interior_in<double>, cells_grid(g), block(), 0, c->stream, v, bd, bg);
+    with_op(level_op(c, 0), c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
+        hipLaunchKernelGGL((k_bottom<double, decltype(O)::value, decltype(V)::value, decltype(R)::value>), dim3(1),
+                           block(), 0, c->stream, v, u, (const double *)bg, work, m, max_restarts, tol, c->d_status);
+    });
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_scalar + 1, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+}
+
+// status of the last gmres_ref_launch (after the stream is synchronized)
+int gmres_ref_status(mps_ctx *c)
+{
+    int bstat;
+    std::memcpy(&bstat, c->h_scalar + 1, sizeof(int));
+    if (bstat != 0) return fail(MPS_ERR_NAN, "NaN or Inf in Arnoldi");
+    return MPS_OK;
+}
+
 void free_gmres(GmresBufs &G)
 {
     for (auto p : G.V) dfree(p);
@@ -2719,26 +3125,18 @@ int mps_gmres(mps_ctx *c, const double *b, double *x, int m, int max_restarts, d
                 return fail(MPS_ERR_ARG, "precond 0 (reference GMRES) supports at most %d cells", kBottomMax);
             if (m > 4 * kBottomMax) return fail(MPS_ERR_ARG, "m too large");
             double *work = dalloc<double>(bottom_work_elems<double>(m, g.ng()));
-            double *bg = c->L64[0].r;
-            fill<double>(c, bg, g.ng(), 0.0);
-            const double *bd = stage_interior(c, b);
-            Lev<double> v = view(c, c->L64, 0);
-            hipLaunchKernelGGL(k_interior_in<double>, cells_grid(g), block(), 0, c->stream, v, bd, bg);
-            with_op(level_op(c, 0), c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
-                hipLaunchKernelGGL((k_bottom<double, decltype(O)::value, decltype(V)::value, decltype(R)::value>),
-                                   dim3(1), block(), 0, c->stream, v, u, (const double *)bg, work, m, max_restarts,
-                                   tol, c->d_status);
-            });
-            HIPCHK(hipMemcpyAsync(c->h_scalar + 1, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-            unbind_solution(c, x, staged);
-            HIPCHK(hipStreamSynchronize(c->stream));
+            try {
+                gmres_ref_launch(c, stage_interior(c, b), u, m, max_restarts, tol, work);
+                unbind_solution(c, x, staged);
+                HIPCHK(hipStreamSynchronize(c->stream));
+            } catch (...) {
+                dfree(work);
+                throw;
+            }
             dfree(work);
-            int bstat;
-            std::memcpy(&bstat, c->h_scalar + 1, sizeof(int));
             if (iters) *iters = -1;
             if (relres) *relres = -1.0;
-            if (bstat != 0) return fail(MPS_ERR_NAN, "NaN or Inf in Arnoldi");
-            return MPS_OK;
+            return gmres_ref_status(c);
         }
         // ---------------- FGMRES(m), right-preconditioned by one V-cycle
         if (m + 1 > kMaxVec) return fail(MPS_ERR_ARG, "m at most %d", kMaxVec - 1);
@@ -2842,6 +3240,10 @@ int mps_gmres(mps_ctx *c, const double *b, double *x, int m, int max_restarts, d
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
     }
 }
+
+}  // extern "C"
+
+extern "C" {
 
 int mps_apply_transpose(mps_ctx *c, const double *x, double *atx)
 {
@@ -2977,11 +3379,17 @@ int time_kernel_t(mps_ctx *c, std::vector<LevelBuf<T>> &L, int kernel, int reps,
         L[0].b = (T *)c->b0;
     }
     std::function<void()> launch;
-    if (kernel == 0 && fused_ok(c, L, 0, op)) {
+    if (kernel == 0 && sweep2_ok(c, L, 0, op)) {
+        // two red-black sweeps in one pass u -> u2: 2 (3 + 2c) w per cell
+        launch = [&]() { launch_sweep2<T>(c, L, 0, L[0].u, L[0].u2, false); };
+        *bytes = 2.0 * (3.0 + 2.0 * cf) * w * N;
+        name = "k_sweep2_fast";
+    } else if (kernel == 0 && fused_ok(c, L, 0, op)) {
         // one fused red+black sweep u -> u2 (both colours: (3 + 2c) w per cell)
         launch = [&]() { launch_sweep<T>(c, L, 0, L[0].u, L[0].u2, op); };
         *bytes = (3.0 + 2.0 * cf) * w * N;
-        name = view(c, L, 0).fast ? "k_sweep_fast" : "k_sweep";
+        const bool ref = view(c, L, 0).fast && op == OP_REF && !c->has_reac && c->sweep_ref;
+        name = ref ? "k_sweep_ref" : view(c, L, 0).fast ? "k_sweep_fast" : "k_sweep";
     } else if (kernel == 0) {
         launch = [&]() { launch_colour<T>(c, L, 0, 1, op); };
         *bytes = (1.5 + cf) * w * N;  // other colour u (1/2), own b (1/2), coefficients; own u write (1/2)
@@ -3062,6 +3470,64 @@ extern "C" int mps_time_kernel(mps_ctx *c, int kernel, int fp32, int reps, doubl
     } catch (const HipError &e) {
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
     }
+}
+
+extern "C" int mps_time_stream(int kind, long long bytes_per_array, int reps, double *avg_ms, double *bytes)
+{
+    if (kind < 0 || kind > 4 || reps < 1 || bytes_per_array < 16 || !avg_ms || !bytes)
+        return fail(MPS_ERR_ARG, "bad arguments");
+    const long long n4 = bytes_per_array / 16;
+    const int narr = kind == 0 ? 1 : kind == 1 ? 1 : kind == 2 ? 2 : kind == 3 ? 3 : 4;
+    float4 *arr[4] = {};
+    double *part = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int status = MPS_OK;
+    try {
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        for (int q = 0; q < narr; ++q) {
+            arr[q] = dalloc<float4>((size_t)n4);
+            HIPCHK(hipMemsetAsync(arr[q], 0, (size_t)n4 * 16, st));
+        }
+        const long long per_block = (long long)kStreamBlock * kStreamUnroll;
+        const unsigned nb = (unsigned)((n4 + per_block - 1) / per_block);
+        part = dalloc<double>(nb);
+        // inputs first, the output last: read (a), write (o), copy (a -> o),
+        // triad (a, b -> o), sweep pattern (a, b, c -> o)
+        const float4 *a = arr[0], *b = narr > 2 ? arr[1] : nullptr, *cc = narr > 3 ? arr[2] : nullptr;
+        float4 *o = kind == 0 ? nullptr : arr[narr - 1];
+        if (kind == 1) a = nullptr;
+        auto launch = [&]() {
+            switch (kind) {
+            case 0: hipLaunchKernelGGL(k_stream<0>, dim3(nb), dim3(kStreamBlock), 0, st, a, b, cc, o, n4, 0.5f, part); break;
+            case 1: hipLaunchKernelGGL(k_stream<1>, dim3(nb), dim3(kStreamBlock), 0, st, a, b, cc, o, n4, 0.5f, part); break;
+            case 2: hipLaunchKernelGGL(k_stream<2>, dim3(nb), dim3(kStreamBlock), 0, st, a, b, cc, o, n4, 0.5f, part); break;
+            case 3: hipLaunchKernelGGL(k_stream<3>, dim3(nb), dim3(kStreamBlock), 0, st, a, b, cc, o, n4, 0.5f, part); break;
+            default: hipLaunchKernelGGL(k_stream<4>, dim3(nb), dim3(kStreamBlock), 0, st, a, b, cc, o, n4, 0.5f, part); break;
+            }
+        };
+        launch();  // warm
+        HIPCHK(hipEventRecord(e0, st));
+        for (int q = 0; q < reps; ++q) launch();
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipGetLastError());
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = ms / reps;
+        *bytes = (double)narr * (double)n4 * 16.0;
+    } catch (const HipError &e) {
+        status = fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    for (auto p : arr) dfree(p);
+    dfree(part);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    return status;
 }
 
 extern "C" int mps_grad_velocity(mps_ctx *c, const double *lam, const double *x, double *gvx, double *gvy,

@@ -92,11 +92,14 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
         __syncthreads();
     };
     // gseidel_precond, niter = 1 (discretepde.f90:238-337), from out = 0:
-    // every neighbour of a red cell is a zero black cell, so the red update
-    // (in - of) / dg has of = +0 exactly and is in / dg; the red pass sets
-    // the black cells to zero beside it, with no separate clearing pass. The
-    // ghost entries of out stay zero from the kernel's start (rows do not
-    // read them, and nothing writes them).
+    // the interior of out is cleared, then the red cells take
+    // (in - of) / dg with of formed from the cleared black neighbours (as in
+    // the reference: a non-finite coefficient makes it NaN there too), then
+    // the black cells from the new red ones. Invariant: every buffer that
+    // precond writes (mb, r, w) has zero ghost entries, set once at the
+    // kernel's start and never written afterwards; explicit (OP_STENCIL)
+    // rows of boundary cells multiply those ghosts by their zero
+    // coefficients.
     auto precond = [&](T *out, const T *in) {
         if (n == 1) {
             if (tid == 0) {
@@ -109,16 +112,15 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
             __syncthreads();
             return;
         }
-        for (int q = tid; q < n; q += nt) {   // red cells, black cells cleared
+        for (int q = tid; q < n; q += nt) out[cmap[q]] = T(0);
+        __syncthreads();
+        for (int q = tid; q < n; q += nt) {   // red cells from the cleared black ones
             int i, j, k;
             long long c = cidx(q, i, j, k);
-            if (((i + j + k) & 1) != 1) {
-                out[c] = T(0);
-                continue;
-            }
+            if (((i + j + k) & 1) != 1) continue;
             T dg, of;
-            row<T, OP, VEL, REAC>(L, in, i, j, k, c, dg, of);   // dg only: it does not depend on u
-            out[c] = in[c] / dg;
+            row<T, OP, VEL, REAC>(L, out, i, j, k, c, dg, of);
+            out[c] = (in[c] - of) / dg;
         }
         __syncthreads();
         for (int q = tid; q < n; q += nt) {   // black cells from the new red ones
@@ -475,6 +477,126 @@ __global__ void __launch_bounds__(kBlock) k_transpose_stencil(Lev<T> L, const T 
     if (j < L.ny) t3[c + L.sy] = s4[c];
     if (k > 1) t6[c - L.sz] = s5[c];
     if (k < L.nz) t5[c + L.sz] = s6[c];
+}
+
+// ------------------------------------------------------------------------
+// Dense inverse of the coarsest operator on the device, for the reference
+// hierarchy in fast arithmetic: the reference's coarse GMRES (m = n, one
+// restart, tol 1e-15, mgridsteps.f90:342-355) is an exact solve of the
+// coarsest system up to rounding, which x = A^-1 b gives directly
+// (k_bottom_dense, one small matrix-vector product per V-cycle).
+//
+// One workgroup assembles A (row-major, n x n) from the level's 7-point rows
+// (k_build_stencil output) and inverts it in place by Gauss-Jordan
+// elimination with partial pivoting (rows swapped during the sweep, the
+// swaps undone as column swaps at the end); the matrix sits in LDS for
+// n <= kDenseLdsN, else in the global work array. Ties in the pivot search
+// go to the lowest row, so the result is deterministic. A zero pivot sets
+// *status = -1 (the inverse is then meaningless).
+// ------------------------------------------------------------------------
+constexpr int kDenseLdsN = 64;
+constexpr int kDenseThreads = 1024;
+
+__global__ void __launch_bounds__(kDenseThreads) k_dense_inverse(Lev<double> L, const double *__restrict__ st0,
+                                                                 const double *__restrict__ st1,
+                                                                 const double *__restrict__ st2,
+                                                                 const double *__restrict__ st3,
+                                                                 const double *__restrict__ st4,
+                                                                 const double *__restrict__ st5,
+                                                                 const double *__restrict__ st6, double *gwork,
+                                                                 double *Ainv, int *status)
+{
+    __shared__ double sM[kDenseLdsN * kDenseLdsN];
+    __shared__ double col[kBottomMaxDense];
+    __shared__ int perm[kBottomMaxDense];
+    __shared__ double rv[kDenseThreads / 64];
+    __shared__ int ri[kDenseThreads / 64];
+    __shared__ int piv;
+    const int nx = L.nx, ny = L.ny, n = L.nx * L.ny * L.nz;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double *M = n <= kDenseLdsN ? sM : gwork;
+    const long long nn = (long long)n * n;
+    for (long long q = tid; q < nn; q += nt) M[q] = 0.0;
+    __syncthreads();
+    for (int r = tid; r < n; r += nt) {
+        const int i = r % nx + 1, j = (r / nx) % ny + 1, k = r / (nx * ny) + 1;
+        const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
+        double *row = M + (long long)r * n;
+        row[r] = st0[c];
+        if (i > 1) row[r - 1] = st1[c];
+        if (i < nx) row[r + 1] = st2[c];
+        if (j > 1) row[r - nx] = st3[c];
+        if (j < ny) row[r + nx] = st4[c];
+        if (k > 1) row[r - nx * ny] = st5[c];
+        if (k < L.nz) row[r + nx * ny] = st6[c];
+    }
+    if (tid == 0 && status) *status = 0;
+    __syncthreads();
+    for (int kk = 0; kk < n; ++kk) {
+        // pivot: the largest |M[r][kk]|, r >= kk, lowest r on ties
+        double best = -1.0;
+        int bi = kk;
+        for (int r = kk + tid; r < n; r += nt) {
+            const double v = fabs(M[(long long)r * n + kk]);
+            if (v > best) best = v, bi = r;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_down(best, off, 64);
+            const int oi = __shfl_down(bi, off, 64);
+            if (ov > best || (ov == best && oi < bi)) best = ov, bi = oi;
+        }
+        if ((tid & 63) == 0) rv[tid >> 6] = best, ri[tid >> 6] = bi;
+        __syncthreads();
+        if (tid == 0) {
+            double b = rv[0];
+            int p = ri[0];
+            for (int w = 1; w < nt / 64; ++w)
+                if (rv[w] > b || (rv[w] == b && ri[w] < p)) b = rv[w], p = ri[w];
+            piv = p;
+            perm[kk] = p;
+            if (!(b > 0.0) && status) *status = -1;
+        }
+        __syncthreads();
+        const int p = piv;
+        double *rk = M + (long long)kk * n;
+        if (p != kk) {
+            double *rp = M + (long long)p * n;
+            for (int q = tid; q < n; q += nt) {
+                const double t = rk[q];
+                rk[q] = rp[q];
+                rp[q] = t;
+            }
+            __syncthreads();
+        }
+        const double inv = 1.0 / rk[kk];
+        __syncthreads();
+        // pivot row: M[kk][kk] = 1, then the row times 1 / pivot
+        for (int q = tid; q < n; q += nt) rk[q] = (q == kk ? 1.0 : rk[q]) * inv;
+        // the pivot column of the other rows, before the update
+        for (int r = tid; r < n; r += nt) col[r] = M[(long long)r * n + kk];
+        __syncthreads();
+        for (long long q = tid; q < nn; q += nt) {
+            const int r = (int)(q / n), cidx = (int)(q - (long long)r * n);
+            if (r == kk) continue;
+            const double f = col[r];
+            M[q] = (cidx == kk ? 0.0 : M[q]) - f * rk[cidx];
+        }
+        __syncthreads();
+    }
+    // undo the row swaps as column swaps, last first
+    for (int kk = n - 1; kk >= 0; --kk) {
+        const int p = perm[kk];
+        if (p != kk) {
+            for (int r = tid; r < n; r += nt) {
+                double *row = M + (long long)r * n;
+                const double t = row[kk];
+                row[kk] = row[p];
+                row[p] = t;
+            }
+        }
+        __syncthreads();
+    }
+    for (long long q = tid; q < nn; q += nt) Ainv[q] = M[q];
 }
 
 }  // namespace mps

@@ -579,6 +579,33 @@ __global__ void __launch_bounds__(kBlock) k_restrict_mean(Lev<T> F, Lev<T> C, co
     cf[c] = T(1.0 / 8.0) * s;
 }
 
+// the coefficient fields of a level (null = absent) and their coarse copies
+struct FieldSet5 {
+    const double *src[5];
+    double *dst[5];
+};
+
+// k_restrict_mean for up to five fields in one launch
+__global__ void __launch_bounds__(kBlock) k_restrict_mean5(Lev<double> F, Lev<double> C, FieldSet5 S)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(C, i, j, k, c)) return;
+    const long long sy = F.sy, sz = F.sz;
+    const long long f0 = (long long)(2 * i - 1) + sy * (2 * j - 1) + sz * (2 * k - 1);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const double *f = S.src[q];
+        if (!f) continue;
+        double s = 0.0;
+        s = s + f[f0] + f[f0 + 1];
+        s = s + f[f0 + sy] + f[f0 + sy + 1];
+        s = s + f[f0 + sz] + f[f0 + sz + 1];
+        s = s + f[f0 + sz + sy] + f[f0 + sz + sy + 1];
+        S.dst[q][c] = (1.0 / 8.0) * s;
+    }
+}
+
 // FAS right-hand side on the coarse level: b_c = A_c(R u) + R r
 // (mgridsteps.f90:544-550); b_c holds R r on entry.
 template <typename T, int OP, bool VEL, bool REAC>
@@ -598,11 +625,17 @@ struct FacesT {
 };
 
 // value of error_2h = unew - uold at a coarse index that may be a ghost,
-// with update_boundary_ghostvalues applied to both (discretepde.f90:1144-1263)
-template <typename T>
+// with update_boundary_ghostvalues applied to both (discretepde.f90:1144-1263);
+// GIVEN: the ghost values as they stand in unew and uold (prolongation1 reads
+// its argument's ghost layer, mgridsteps.f90:170-216)
+template <typename T, bool GIVEN = false>
 __device__ __forceinline__ T err_at(const Lev<T> &C, const T *unew, const T *uold, const FacesT<T> &G, int i, int j,
                                     int k)
 {
+    if (GIVEN) {
+        const long long g = (long long)i + (long long)C.sy * j + C.sz * k;
+        return unew[g] - uold[g];
+    }
     int ii = i, jj = j, kk = k, face = -1;
     long long fidx = 0;
     if (i == 0) { ii = 1; face = LEFT; fidx = (j - 1) + (long long)C.ny * (k - 1); }
@@ -621,7 +654,7 @@ __device__ __forceinline__ T err_at(const Lev<T> &C, const T *unew, const T *uol
 }
 
 // prolongation1 + correction (mgridsteps.f90:170-216, :562-567)
-template <typename T>
+template <typename T, bool GIVEN = false>
 __global__ void __launch_bounds__(kBlock) k_prolong1_correct(Lev<T> F, Lev<T> C, T *uf, const T *__restrict__ unew,
                                                             const T *__restrict__ uold, FacesT<T> G)
 {
@@ -629,9 +662,9 @@ __global__ void __launch_bounds__(kBlock) k_prolong1_correct(Lev<T> F, Lev<T> C,
     long long c;
     if (!cell_of(C, i, j, k, c)) return;
     const T e = unew[c] - uold[c];
-    const T sx = T(0.5) * (err_at(C, unew, uold, G, i + 1, j, k) - err_at(C, unew, uold, G, i - 1, j, k));
-    const T sy_ = T(0.5) * (err_at(C, unew, uold, G, i, j + 1, k) - err_at(C, unew, uold, G, i, j - 1, k));
-    const T sz_ = T(0.5) * (err_at(C, unew, uold, G, i, j, k + 1) - err_at(C, unew, uold, G, i, j, k - 1));
+    const T sx = T(0.5) * (err_at<T, GIVEN>(C, unew, uold, G, i + 1, j, k) - err_at<T, GIVEN>(C, unew, uold, G, i - 1, j, k));
+    const T sy_ = T(0.5) * (err_at<T, GIVEN>(C, unew, uold, G, i, j + 1, k) - err_at<T, GIVEN>(C, unew, uold, G, i, j - 1, k));
+    const T sz_ = T(0.5) * (err_at<T, GIVEN>(C, unew, uold, G, i, j, k + 1) - err_at<T, GIVEN>(C, unew, uold, G, i, j, k - 1));
     const long long sy = F.sy, sz = F.sz;
     const long long f0 = (long long)(2 * i - 1) + sy * (2 * j - 1) + sz * (2 * k - 1);
     const T q = T(0.25);
@@ -767,6 +800,16 @@ __global__ void k_dc_scale(const double *sumsq, double n, double *scale)
 
 // interior copy between a caller's interior array (nx,ny,nz) and the
 // ghosted layout, either direction
+// interior of a ghosted array into another ghosted array of the same level
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_copy_interior(Lev<T> L, const T *__restrict__ src, T *dst)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    dst[c] = src[c];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_interior_in(Lev<T> L, const T *src, T *dst_ghosted)
 {

@@ -46,6 +46,11 @@ struct ChemDev {
     int mterm[kChemMaxSpecies][kChemMaxSpecies];    // [s][j]: partner j enters s's sum
     double mco[kChemMaxSpecies][kChemMaxSpecies][3];
     int dkind[kChemMaxSpecies];                     // 0 constant, 1 Einstein with T_e, 2 with T_g
+    // reaction rates: 0 getarrhrate; 1 the exponential fit in 1/T of
+    // getcustomrate (O2_chem.f90:413-448); 2 alpha |E| exp(-E0 / |E|) of
+    // getcustomrate (H2_chem.f90:423-437), |E| the cell's field magnitude
+    int rkind[kChemMaxReac];
+    double rco[kChemMaxReac][5];
 };
 
 constexpr double kEvToK = 11604.5;                  // plasma_pde/fundconstants.f90:9
@@ -94,10 +99,25 @@ __host__ __device__ __forceinline__ double powi_sq(double x, int m)
     return r;
 }
 
-// getarrhrate (ion_elec_chem.f90:260-274): A T^alpha exp(-Ea/T)
-__device__ __forceinline__ double arrh_rate(const ChemDev &C, int r, double T)
+// the rate of reaction r at electron temperature T and field magnitude
+// efm, as getspecproduction takes it: getarrhrate (ion_elec_chem.f90:260-274)
+// A T^alpha exp(-Ea/T), or getcustomrate: the fit of O2_chem.f90:413-448,
+// C0 exp(c1/T + c2/T**2 + c3/T**3 + c4/T**4) (coefficients signed, the
+// terms summed left to right, T**n by repeated squaring as Fortran's
+// real**integer), or alpha |E| exp(-E0/|E|) of H2_chem.f90:423-437
+__device__ __forceinline__ double reaction_rate(const ChemDev &C, int r, double T, double efm)
 {
-    return C.A[r] * pow(T, C.alpha[r]) * exp(-C.Ea[r] / T);
+    const int kind = C.rkind[r];
+    if (kind == 0) return C.A[r] * pow(T, C.alpha[r]) * exp(-C.Ea[r] / T);
+    const double *q = C.rco[r];
+    if (kind == 1) {
+        // T**2, T**3, T**4 as the reference build forms them: a chain of
+        // products ((T*T)*T)*T, not (T*T)*(T*T) (bit for bit on its rates)
+        const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+        const double e = ((q[1] / T + q[2] / T2) + q[3] / T3) + q[4] / T4;
+        return q[0] * exp(e);
+    }
+    return q[0] * fabs(efm) * exp(-q[1] / fabs(efm));
 }
 
 // product of the reactant densities of reaction r at cell c
@@ -184,8 +204,9 @@ __global__ void __launch_bounds__(256) k_transport(Lev<double> L, const ChemDev 
     double vx = 0.0, vy = 0.0, vz = 0.0, dc = 0.0, src = 0.0;
     if (g.interior) {
         const double Ex = in.ex[c], Ey = in.ey[c], Ez = in.ez[c];
-        // efield_mag is not used by table chemistries (ion_elec_chem's
-        // mobility, diffusion and Arrhenius rates do not depend on it)
+        // efield_mag as the callers form it (edeneq.f90:405-409): only the
+        // field-dependent custom rate reads it
+        const double efm = sqrt((Ex * Ex + Ey * Ey) + Ez * Ez);
         const double Te = in.te[c];
         if (KIND == 0) {
             const double mu = spec_mobility(C, sp, in.nd, ng, c, Te);
@@ -198,7 +219,7 @@ __global__ void __launch_bounds__(256) k_transport(Lev<double> L, const ChemDev 
             for (int r = 0; r < C.nr; ++r) {
                 const int nm = C.nu[r][sp];
                 if (nm == 0) continue;
-                const double k = arrh_rate(C, r, Te);
+                const double k = reaction_rate(C, r, Te, efm);
                 prod = prod + (double)nm * k * reactant_product(C, r, in.nd, ng, c);
             }
             src = 0.0 + prod;
@@ -224,7 +245,7 @@ __global__ void __launch_bounds__(256) k_transport(Lev<double> L, const ChemDev 
             double inel = 0.0;
             for (int r = 0; r < C.nr; ++r) {
                 if (C.eel[r] == 0.0) continue;
-                const double k = arrh_rate(C, r, Te);
+                const double k = reaction_rate(C, r, Te, efm);
                 inel = inel + k * reactant_product(C, r, in.nd, ng, c) * C.eel[r];
             }
             src = jh - el + inel * kEcharge;
@@ -321,6 +342,87 @@ __global__ void __launch_bounds__(256) k_ion_drift_faces(Lev<double> L, const Ch
     const double muE = mu * En;
     const bool leaves = low ? (muE < 0.0) : (muE > 0.0);
     out[t] = leaves ? (low ? -drift : drift) : 0.0;
+}
+
+// The face values of the thermal boundary updaters, one thread per boundary
+// cell of each face with an output array, in the layout mps_set_bc takes:
+//   kind 0  eden_update_bcs (edeneq.f90:203-358): FLUX faces
+//           0.25 n cbar vscale, cbar = sqrt(c1 T) with T the boundary cell's
+//           electron temperature
+//   kind 1  neutrals_update_bcs (neutralseq.f90:206-358): FLUX faces
+//           0.25 n cbar vscale with cbar = sqrt(c1 Tg)
+//   kind 2  eenergy_update_bcs (eenergyeq.f90:208-392): DIRC faces
+//           1.5 k_B n_e T_wall / escale; FLUX faces 4/3 n_eps cbar vscale,
+//           cbar = sqrt(c1 T), T = n_eps escale / (1.5 n_e k_B)
+// in the reference's operation order; faces of other codes keep their
+// values. n is the scaled density (kinds 0, 1) or energy density (kind 2),
+// T the electron temperature (kind 0) or the physical electron density
+// (kind 2), all ghosted.
+struct ThermalFaces {
+    double *f[6];
+    int code[6];
+    double twall[6];
+};
+
+__device__ __forceinline__ long long face_cell(const Lev<double> &L, int face, int t, int n0)
+{
+    const int axis = face / 2;
+    const bool low = (face & 1) == 0;
+    const int a = t % n0 + 1, b = t / n0 + 1;
+    int i, j, k;
+    if (axis == 0) {
+        i = low ? 1 : L.nx, j = a, k = b;
+    } else if (axis == 1) {
+        i = a, j = low ? 1 : L.ny, k = b;
+    } else {
+        i = a, j = b, k = low ? 1 : L.nz;
+    }
+    return (long long)i + (long long)L.sy * j + L.sz * k;
+}
+
+__global__ void __launch_bounds__(256) k_thermal_faces(Lev<double> L, int kind, const double *__restrict__ n,
+                                                       const double *__restrict__ T, double c1, double Tg,
+                                                       double escale, double vscale, ThermalFaces F)
+{
+    const int face = blockIdx.y;
+    double *out = F.f[face];
+    if (!out) return;
+    const int code = F.code[face];
+    if (code != BC_FLUX && !(kind == 2 && code == BC_DIRC)) return;
+    const int axis = face / 2;
+    const int n0 = axis == 0 ? L.ny : L.nx;
+    const int n1 = axis == 2 ? L.ny : L.nz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n0 * n1) return;
+    const long long c = face_cell(L, face, t, n0);
+    const double q = 0.25;   // a default-real literal, exact
+    if (kind == 0) {
+        const double cbar = sqrt(c1 * T[c]);
+        out[t] = q * n[c] * cbar * vscale;
+    } else if (kind == 1) {
+        const double cbar = sqrt(c1 * Tg);
+        out[t] = q * n[c] * cbar * vscale;
+    } else {
+        const double three_by_two = 1.5, four_by_three = 4.0 / 3.0;
+        if (code == BC_DIRC) {
+            out[t] = three_by_two * kBoltz * T[c] * F.twall[face] / escale;
+        } else {
+            const double etemp = n[c] * escale / (three_by_two * T[c] * kBoltz);
+            const double cbar = sqrt(c1 * etemp);
+            const double flux = four_by_three * n[c] * cbar;
+            out[t] = flux * vscale;
+        }
+    }
+}
+
+// the density floor of eden_solve (edeneq.f90:509-518): interior values
+// below vmin become vmin
+__global__ void __launch_bounds__(256) k_floor(Lev<double> L, double *__restrict__ u, double vmin)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    if (u[c] < vmin) u[c] = vmin;
 }
 
 // update_electrontemp (eenergyeq.f90:842-877): T_e = n_eps scale /

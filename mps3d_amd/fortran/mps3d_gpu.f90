@@ -29,6 +29,7 @@ module mps3d_gpu
         type(c_ptr) :: reactants, products, k_arrh, elecenergy, mobility, dcoeff, molmass, charge
         type(c_ptr) :: mob_kind = c_null_ptr, mob_terms = c_null_ptr, mob_coef = c_null_ptr
         type(c_ptr) :: dcoeff_kind = c_null_ptr
+        type(c_ptr) :: rate_kind = c_null_ptr, rate_coef = c_null_ptr
     end type mps_chem
 
     type, bind(C) :: mps_grid
@@ -60,6 +61,13 @@ module mps3d_gpu
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
         end function mps_destroy
+
+        integer(c_int) function mps_get_grid(ctx, grid, nspecies) bind(C, name='mps_get_grid')
+            import :: c_int, c_ptr, mps_grid
+            type(c_ptr), value :: ctx
+            type(mps_grid), intent(out) :: grid
+            integer(c_int), intent(out) :: nspecies
+        end function mps_get_grid
 
         integer(c_int) function mps_set_bc_str(ctx, codes, faces) bind(C, name='mps_set_bc_str')
             import :: c_int, c_ptr, c_char
@@ -162,6 +170,21 @@ module mps3d_gpu
 
 contains
 
+    ! MPS_ERR_ARG unless (lx, ly, lz) is the context's grid and, with
+    ! nsp >= 0, nsp the number of species of its chemistry: the wrappers'
+    ! array arguments are sized from these, and the library reads and
+    ! writes whole arrays of the context's grid
+    integer function gpu_check_grid(ctx, lx, ly, lz, nsp) result(status)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: lx, ly, lz, nsp
+        type(mps_grid) :: g
+        integer(c_int) :: ns
+        status = mps_get_grid(ctx, g, ns)
+        if (status /= MPS_OK) return
+        if (g%nx /= lx .or. g%ny /= ly .or. g%nz /= lz) status = MPS_ERR_ARG
+        if (nsp >= 0 .and. nsp /= ns) status = MPS_ERR_ARG
+    end function gpu_check_grid
+
     ! context for the grid and process layout of the reference's globals
     ! (g_Nx.., g_xlen.., g_px.. of mgridlib/globalvars.f90:33-52)
     integer function gpu_open(ctx, nx, ny, nz, xlen, ylen, zlen, px, py, pz, opts) result(status)
@@ -200,6 +223,8 @@ contains
         type(c_ptr) :: faces(6)
         integer :: f
         integer(c_int) :: td
+        status = gpu_check_grid(ctx, lx, ly, lz, -1)
+        if (status /= MPS_OK) return
         do f = 1, 6
             codes(4*f-3:4*f) = bc_codes(f)
         enddo
@@ -254,6 +279,8 @@ contains
         real*8, intent(out), target :: history(*)
         integer, intent(out) :: status
         integer(c_int) :: n
+        status = gpu_check_grid(ctx, lx, ly, lz, -1)
+        if (status /= MPS_OK) return
         status = mps_solve(ctx, c_loc(solnvar), n, resnorm, c_loc(history))
         ncycles = n
     end subroutine gpu_solve
@@ -271,6 +298,8 @@ contains
         real*8, intent(out), target :: history(*)
         integer, intent(out) :: status
         integer(c_int) :: n
+        status = gpu_check_grid(ctx, lx, ly, lz, -1)
+        if (status /= MPS_OK) return
         status = mps_solve_rhs(ctx, c_loc(b), c_loc(solnvar), n, resnorm, c_loc(history))
         ncycles = n
     end subroutine gpu_solve_rhs
@@ -287,6 +316,8 @@ contains
         logical, intent(in) :: limiter
         integer, intent(out) :: status
         integer(c_int) :: lim
+        status = gpu_check_grid(ctx, lx, ly, lz, -1)
+        if (status /= MPS_OK) return
         lim = 0
         if (limiter) lim = 1
         status = mps_efield(ctx, c_loc(poissonsoln), c_loc(efield), potscale / lscale, lim)
@@ -349,6 +380,8 @@ contains
         real*8, intent(inout), target :: reac(0:lx+1, 0:ly+1, 0:lz+1), source(0:lx+1, 0:ly+1, 0:lz+1)
         real*8, intent(inout), target :: prod(0:lx+1, 0:ly+1, 0:lz+1)
         integer, intent(out) :: status
+        status = gpu_check_grid(ctx, lx, ly, lz, nsp)
+        if (status /= MPS_OK) return
         status = mps_species_transport(ctx, specnum, dt, 1, c_loc(elecfield), c_loc(electemp), c_loc(numden), &
             Tg, Pg, c_loc(scaling), c_loc(soln), c_loc(vel), c_loc(dcoeff), c_loc(reac), c_loc(source), c_loc(prod))
     end subroutine gpu_species_transport
@@ -365,6 +398,8 @@ contains
         real*8, intent(inout), target :: vel(0:lx+1, 0:ly+1, 0:lz+1, 3), dcoeff(0:lx+1, 0:ly+1, 0:lz+1)
         real*8, intent(inout), target :: reac(0:lx+1, 0:ly+1, 0:lz+1), source(0:lx+1, 0:ly+1, 0:lz+1)
         integer, intent(out) :: status
+        status = gpu_check_grid(ctx, lx, ly, lz, nsp)
+        if (status /= MPS_OK) return
         status = mps_eenergy_transport(ctx, dt, 1, lscale, c_loc(elecfield), c_loc(electemp), c_loc(numden), &
             Tg, Pg, c_loc(scaling), c_loc(soln), c_loc(vel), c_loc(dcoeff), c_loc(reac), c_loc(source))
     end subroutine gpu_eenergy_transport
@@ -380,6 +415,8 @@ contains
         real*8, intent(inout), target :: vel(0:lx+1, 0:ly+1, 0:lz+1, 3), dcoeff(0:lx+1, 0:ly+1, 0:lz+1)
         real*8, intent(inout), target :: reac(0:lx+1, 0:ly+1, 0:lz+1), source(0:lx+1, 0:ly+1, 0:lz+1)
         integer, intent(out) :: status
+        status = gpu_check_grid(ctx, lx, ly, lz, -1)
+        if (status /= MPS_OK) return
         status = mps_poisson_transport(ctx, dt, 1, c_loc(elecden), c_loc(iondens), c_loc(scaling), c_loc(soln), &
             c_loc(vel), c_loc(dcoeff), c_loc(reac), c_loc(source))
     end subroutine gpu_poisson_transport
@@ -393,6 +430,8 @@ contains
         real*8, intent(in), target :: elecden(0:lx+1, 0:ly+1, 0:lz+1)
         real*8, intent(in) :: eenergyscale, etempmin
         integer, intent(out) :: status
+        status = gpu_check_grid(ctx, lx, ly, lz, -1)
+        if (status /= MPS_OK) return
         status = mps_electron_temp(ctx, c_loc(eenergysoln), c_loc(elecden), eenergyscale, etempmin, c_loc(electemp))
     end subroutine gpu_update_electrontemp
 
@@ -409,6 +448,8 @@ contains
         real*8, intent(inout), target :: backface(lx, ly), frontface(lx, ly)
         integer, intent(out) :: status
         type(c_ptr) :: f(6)
+        status = gpu_check_grid(ctx, lx, ly, lz, nspec)
+        if (status /= MPS_OK) return
         f = c_null_ptr
         if (flux(1) /= 0) f(1) = c_loc(leftface)
         if (flux(2) /= 0) f(2) = c_loc(rightface)

@@ -20,14 +20,11 @@ names of the reference's output files.
 """
 from __future__ import annotations
 
-import math
-
-from .chemistry import K_BMAN, M_ELEC, etempmin_default, ion_elec, real4
+from .chemistry import K_BMAN, etempmin_default, ion_elec
 from .solver import FieldSolver
 
 ERRTOLR = 1.0e-11        # mgridlib/globalvars.f90:29
 NVCYCLES = 20            # the *_solve loops
-PI_REF = real4(3.14159265)   # plasma_pde/fundconstants.f90:8, a default-real literal
 FACES = ("left", "right", "bottom", "top", "back", "front")
 
 
@@ -65,6 +62,24 @@ H2_O2_DECK = {
 }
 
 
+# tests/parallel_plate/{GEOM_INPUTS,PLASMA_INPUTS} of the reference (ion_elec
+# chemistry): a 0.625 mm x 1 cm x 0.625 mm column, 32 x 512 x 32 cells on a
+# 2 x 4 x 2 process grid, electrodes on the y faces at 0 V and -250 V
+PARALLEL_PLATE_DECK = {
+    "origin": (0.0, 0.0, 0.0), "length": (0.000625, 0.01, 0.000625), "n": (32, 512, 32), "p": (2, 4, 2),
+    "timestep": 1.0e-10, "pressure": 133.0, "temperature": 300.0,
+    "scales": {"LENGTH": 0.01, "TIME": 1.0e-11, "ELECDEN": 1.0e14, "IONDEN": 1.0e14, "SPECDEN": 1.0e15,
+               "POT": 100.0, "ETEMP": 5800.0},
+    "init": {"N": 3.21e22, "E": 1.0e14, "I+": 1.0e14, "POT": 0.0, "ELECTEMP": 1160.0},
+    "bcs": [("left", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0),
+            ("right", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0),
+            ("bottom", "FLUX", "FLUX", "DIRC", "FLUX", 0.0, 0.0, 0.0, 0.0),
+            ("top", "FLUX", "FLUX", "DIRC", "FLUX", 0.0, 0.0, -250.0, 0.0),
+            ("back", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0),
+            ("front", "ZGRD", "ZGRD", "ZGRD", "ZGRD", 0.0, 0.0, 0.0, 0.0)],
+}
+
+
 def output_names(chem):
     """The reference's output scalars in its order (all_scalarnames,
     solve_manager.f90:270-289): species densities, potential, field, electron
@@ -89,7 +104,15 @@ def write_deck_output(dirname, fields, chem, deck):
 
 
 class PlasmaRun:
-    def __init__(self, deck, chem=None, device="cuda"):
+    """arith "exact" (default): every solve is the reference's V-cycle loop in
+    the reference's rounding (the coarse GMRES included), so the fields follow
+    the reference run's trajectory. arith "fast": the same loops and
+    operators with the rows of the equations without convection in
+    fast arithmetic and the coarsest level solved by a device-built dense
+    inverse (the reference's coarse GMRES with m = n and tolerance 1e-15 is
+    an exact solve; mgridsteps.f90:342-355)."""
+
+    def __init__(self, deck, chem=None, device="cuda", arith="exact"):
         import torch
         self.torch = torch
         self.dev = torch.device(device)
@@ -114,7 +137,7 @@ class PlasmaRun:
 
         def ctx():
             s = FieldSolver(self.n, length, self.p, hierarchy="ref", precision="fp64", flux="upwind",
-                            max_cycles=NVCYCLES, tol=ERRTOLR, tol_kind="rms", arith="exact")
+                            max_cycles=NVCYCLES, tol=ERRTOLR, tol_kind="rms", arith=arith)
             s.set_chemistry(ch)
             return s
 
@@ -148,11 +171,13 @@ class PlasmaRun:
         self.eprod = torch.zeros(g, dtype=torch.float64, device=self.dev)
         self.nprod = [torch.zeros(g, dtype=torch.float64, device=self.dev) for _ in self.neutrals]
         # module solutions (scaled, ghosted): poisson_init, eden_init, ...
-        self.phi_s = self.pot / self.potscale
-        self.ne_s = self.numden[e] / self.escale_n
-        self.ni_s = [self.numden[i] / self.iscale for i in self.ions]
-        self.nn_s = [self.numden[i] / self.nscale for i in self.neutrals]
-        self.eps_s = self.eps / self.eescale
+        # (IEEE divisions, as the reference divides: a tensor divided by a
+        # Python number goes through its reciprocal)
+        self.phi_s = self._div(self.pot, self.potscale)
+        self.ne_s = self._div(self.numden[e], self.escale_n)
+        self.ni_s = [self._div(self.numden[i], self.iscale) for i in self.ions]
+        self.nn_s = [self._div(self.numden[i], self.nscale) for i in self.neutrals]
+        self.eps_s = self._div(self.eps, self.eescale)
         # *_set_bcs: face values bc_params / scale on every face
         self.faces = {
             "pot": [self._face_full(f, self.params["pot"][q] / self.potscale) for q, f in enumerate(FACES)],
@@ -166,6 +191,9 @@ class PlasmaRun:
         self.t = 0.0
         self.steps = 0
         self.cycles = []
+
+    def _div(self, t, v):
+        return t / self.torch.full_like(t, float(v))
 
     # ------------------------------------------------------------ faces
     def _face_shape(self, f):
@@ -191,17 +219,19 @@ class PlasmaRun:
         return {"left": 0, "right": 0, "bottom": 1, "top": 1, "back": 2, "front": 2}[f]
 
     # ------------------------------------------------------------ one step
-    def _solve(self, s, u, codes, faces, co, timederiv, dt_s, faces_changed=True):
+    def _solve(self, s, u, codes, faces, co, timederiv, dt_s, faces_changed=True, coeffs_changed=True):
         if faces_changed:
             s.set_bc(codes, [x.contiguous() for x in faces])
-        V = co["vel"].view(3, -1)
-        s.set_coeffs(co["dcoeff"], V[0], V[1], V[2], co["reac"], co["source"], timederiv, dt_s)
+        if coeffs_changed:
+            V = co["vel"].view(3, -1)
+            s.set_coeffs(co["dcoeff"], V[0], V[1], V[2], co["reac"], co["source"], timederiv, dt_s)
+        else:
+            s.set_source(co["source"])
         k, _, _, _ = s.solve(u)
         s.fill_ghosts(u)
         return k
 
     def step(self):
-        torch = self.torch
         ch = self.chem
         e = ch.especnum - 1
         dt_s = self.dt / self.t0
@@ -210,8 +240,10 @@ class PlasmaRun:
         # (poisson_update_bcs changes only 'DZGD' electrodes)
         ions = self.numden[self.ions[0]:self.ions[-1] + 1].reshape(-1)
         co = self.pois.poisson_transport(self.numden[e].reshape(-1), ions, self.scal["pot"], timederiv=False, dt=dt_s)
+        # (its coefficients never change: the permittivity-free operator
+        # dcoeff = tscale/lscale^2 of poisson_update_transport; only the source)
         cyc.append(self._solve(self.pois, self.phi_s, self.codes["pot"], self.faces["pot"], co, False, dt_s,
-                               faces_changed=self.steps == 0))
+                               faces_changed=self.steps == 0, coeffs_changed=self.steps == 0))
         self.pot = self.phi_s * self.potscale
         # update_efield (poissoneq.f90:565-680)
         self.pois.efield(self.phi_s, self.potscale / self.L0, False, out=self.E)
@@ -229,9 +261,7 @@ class PlasmaRun:
                                          dt=dt_s, Tg=self.Tg, Pg=self.Pg, out={"prod": self.eprod.view(-1)})
         self._eden_bcs()
         cyc.append(self._solve(self.elec, self.ne_s, self.codes["eden"], self.faces["eden"], co, True, dt_s))
-        inner = self.ne_s[1:-1, 1:-1, 1:-1]
-        inner.copy_(torch.where(inner < self.nemin, torch.full_like(inner, self.nemin), inner))
-        self.elec.fill_ghosts(self.ne_s)
+        self.elec.floor(self.ne_s, self.nemin)
         self.numden[e].copy_(self.ne_s * self.escale_n)
         # ions_solve (ionseq.f90:560-629): every ion from the same densities,
         # which take the new ion densities after the last one
@@ -266,35 +296,19 @@ class PlasmaRun:
         return self
 
     # ------------------------------------------------------------ BC updaters
-    def _const_1(self):
-        return 8.0 * K_BMAN / PI_REF / M_ELEC
-
     def _eden_bcs(self):
         """eden_update_bcs (edeneq.f90:203-358): FLUX faces get the thermal
-        outflow 0.25 n cbar, cbar = sqrt(8 k_B T_e / (pi m_e))."""
-        torch = self.torch
-        c1 = self._const_1()
-        for q, f in enumerate(FACES):
-            if self.codes["eden"][q] == "FLUX":
-                cbar = torch.sqrt(c1 * self._bc(self.Te, f))
-                self.faces["eden"][q] = 0.25 * self._bc(self.ne_s, f) * cbar * self.vel_scaling
+        outflow 0.25 n cbar, cbar = sqrt(8 k_B T_e / (pi m_e)), in one launch
+        (mps_eden_faces)."""
+        self.elec.eden_faces(self.codes["eden"], self.ne_s.reshape(-1), self.Te.reshape(-1), self.vel_scaling,
+                             self.faces["eden"])
 
     def _eenergy_bcs(self):
         """eenergy_update_bcs (eenergyeq.f90:208-392): DIRC faces 3/2 k_B n_e
-        T_wall, FLUX faces 4/3 n_eps cbar."""
-        torch = self.torch
-        c1 = self._const_1()
+        T_wall, FLUX faces 4/3 n_eps cbar, in one launch (mps_eenergy_faces)."""
         ne = self.numden[self.chem.especnum - 1]
-        for q, f in enumerate(FACES):
-            code = self.codes["etemp"][q]
-            if code == "DIRC":
-                self.faces["ener"][q] = 1.5 * K_BMAN * self._bc(ne, f) * self.params["etemp"][q] / self.eescale
-            elif code == "FLUX":
-                eb = self._bc(self.eps_s, f)
-                etemp = eb * self.eescale / (1.5 * self._bc(ne, f) * K_BMAN)
-                cbar = torch.sqrt(c1 * etemp)
-                flux = (4.0 / 3.0) * eb * cbar
-                self.faces["ener"][q] = flux * self.vel_scaling
+        self.ener.eenergy_faces(self.codes["etemp"], self.eps_s.reshape(-1), ne.reshape(-1), self.eescale,
+                                self.params["etemp"], self.vel_scaling, self.faces["ener"])
 
     def _ion_bcs(self, q, sp):
         """ions_update_bcs (ionseq.f90:206-476) for ion sp: FLUX faces carry
@@ -306,7 +320,7 @@ class PlasmaRun:
             code = self.codes["spec"][k]
             if code == "DIRC":
                 # the reference divides by ionscale on every call (ionseq.f90:243)
-                faces[k] = faces[k] / self.iscale
+                faces[k] = self._div(faces[k], self.iscale)
             elif code == "FLUX":
                 flux[k] = faces[k]    # rewritten in place: mps_set_bc copies the faces
         if any(t is not None for t in flux):
@@ -316,13 +330,10 @@ class PlasmaRun:
 
     def _neutral_bcs(self, q, sp):
         """neutrals_update_bcs (neutralseq.f90:206-358): FLUX faces get the
-        thermal outflow 0.25 n cbar, cbar = sqrt(8 k_B T_g / (pi m))."""
-        const_1 = 8.0 * K_BMAN / PI_REF / float(self.chem.molmass[sp])
-        cbar = math.sqrt(const_1 * self.Tg)
-        faces = self.faces["neut"][q]
-        for k, f in enumerate(FACES):
-            if self.codes["spec"][k] == "FLUX":
-                faces[k] = 0.25 * self._bc(self.nn_s[q], f) * cbar * self.vel_scaling
+        thermal outflow 0.25 n cbar, cbar = sqrt(8 k_B T_g / (pi m)), in one
+        launch (mps_neutral_faces)."""
+        self.neut.neutral_faces(self.codes["spec"], self.nn_s[q].reshape(-1), float(self.chem.molmass[sp]), self.Tg,
+                                self.vel_scaling, self.faces["neut"][q])
 
     # ------------------------------------------------------------ output
     def fields(self):

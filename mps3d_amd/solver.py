@@ -62,6 +62,21 @@ def from_device(t):
     return np.asfortranarray(t.detach().cpu().numpy().transpose(2, 1, 0))
 
 
+STREAM_KINDS = {"read": 0, "write": 1, "copy": 2, "triad": 3, "sweep": 4}
+
+
+def time_stream(kind, bytes_per_array=1 << 30, reps=10):
+    """HBM rate (GB/s) of a hand-written streaming kernel with 16-byte accesses
+    on the current device: kind "read", "write", "copy", "triad" (o = a + s b)
+    or "sweep" (o = a + s (b + c): three streams read and one written, the
+    red-black sweep's pattern). Returns (GB/s, mean ms per launch, bytes per
+    launch)."""
+    L = M.load()
+    ms, by = C.c_double(), C.c_double()
+    M.check(L.mps_time_stream(STREAM_KINDS[kind], int(bytes_per_array), int(reps), C.byref(ms), C.byref(by)))
+    return by.value / (ms.value * 1e-3) / 1e9, ms.value, by.value
+
+
 def _scaling(sc):
     """{vel, dcoeff, reac, source} scalings as a double[4]."""
     v = [float(x) for x in sc]
@@ -146,9 +161,9 @@ class FieldSolver:
         self._torch_call = True
 
     def order_torch_out(self):
-        """Orders torch's current stream after the context's stream (the
-        library returns with its stream synchronized; this keeps torch work
-        queued later correct on any stream)."""
+        """Orders torch's current stream after the context's stream (device-
+        side, no host wait): a call on device arrays only may return with its
+        work still queued on the context's stream."""
         import torch
         if self._torch_call and self._ext is not None:
             torch.cuda.current_stream().wait_stream(self._ext)
@@ -198,6 +213,13 @@ class FieldSolver:
         shapes = [(ny, nz), (ny, nz), (nx, nz), (nx, nz), (nx, ny), (nx, ny)]
         ks = (C.c_int * 6)(*[M.BC_BY_NAME.get(str(c).strip().upper(), M.BC_NONE) if isinstance(c, str) else int(c)
                              for c in codes])
+        # the library copies exactly ny*nz, nx*nz or nx*ny values from each
+        # device face: a face of another size is refused here, before any call
+        for f in range(6):
+            v = faces[f]
+            if _is_torch(v) and v.numel() != shapes[f][0] * shapes[f][1]:
+                raise ValueError("face %d has %d elements, expected %d (%d x %d)"
+                                 % (f, v.numel(), shapes[f][0] * shapes[f][1], shapes[f][0], shapes[f][1]))
         ptrs = (C.c_void_p * 6)()
         for f in range(6):
             v = faces[f]
@@ -218,6 +240,14 @@ class FieldSolver:
         try:
             args = [self._ptr(a, "g") for a in (dcoeff, velx, vely, velz, reac, source)]
             M.check(self.lib.mps_set_coeffs(self._h, *args, int(bool(timederiv)), float(dt)))
+        finally:
+            self._release()
+
+    def set_source(self, source=None):
+        """The source of the last set_coeffs replaced (coefficients and hierarchy
+        kept): mps_set_source."""
+        try:
+            M.check(self.lib.mps_set_source(self._h, self._ptr(source, "g")))
         finally:
             self._release()
 
@@ -416,6 +446,59 @@ class FieldSolver:
             self._release()
         return faces
 
+    def _codes(self, codes):
+        return (C.c_int * 6)(*[M.BC_BY_NAME.get(str(c).strip().upper(), M.BC_NONE) if isinstance(c, str) else int(c)
+                               for c in codes])
+
+    def _face_ptrs(self, faces):
+        nx, ny, nz = self.n
+        want = (ny * nz, ny * nz, nx * nz, nx * nz, nx * ny, nx * ny)
+        ptrs = (C.c_void_p * 6)()
+        for f in range(6):
+            v = faces[f]
+            if v is None:
+                continue
+            if not _is_torch(v) or v.numel() != want[f] or not v.is_contiguous():
+                raise ValueError("face %d must be a contiguous device tensor of %d values" % (f, want[f]))
+            ptrs[f] = self._ptr(v, "f", True)
+        return ptrs
+
+    def eden_faces(self, codes, elecden, electemp, velscale, faces):
+        """eden_update_bcs (plasma_pde/edeneq.f90:203-358): the FLUX faces of
+        `faces` (six device tensors or None, rewritten in place) get the thermal
+        outflow of the scaled electron density: mps_eden_faces."""
+        try:
+            M.check(self.lib.mps_eden_faces(self._h, self._codes(codes), self._fields(elecden, 1),
+                                            self._fields(electemp, 1), float(velscale), self._face_ptrs(faces)))
+        finally:
+            self._release()
+
+    def neutral_faces(self, codes, neutralden, molmass, Tg, velscale, faces):
+        """neutrals_update_bcs (plasma_pde/neutralseq.f90:206-358): mps_neutral_faces."""
+        try:
+            M.check(self.lib.mps_neutral_faces(self._h, self._codes(codes), self._fields(neutralden, 1),
+                                               float(molmass), float(Tg), float(velscale), self._face_ptrs(faces)))
+        finally:
+            self._release()
+
+    def eenergy_faces(self, codes, elecenergy, elecden, eenergyscale, bcparams, velscale, faces):
+        """eenergy_update_bcs (plasma_pde/eenergyeq.f90:208-392): mps_eenergy_faces."""
+        par = (C.c_double * 6)(*[float(v) for v in bcparams])
+        try:
+            M.check(self.lib.mps_eenergy_faces(self._h, self._codes(codes), self._fields(elecenergy, 1),
+                                               self._fields(elecden, 1), float(eenergyscale), par, float(velscale),
+                                               self._face_ptrs(faces)))
+        finally:
+            self._release()
+
+    def floor(self, soln, vmin):
+        """The end of eden_solve (plasma_pde/edeneq.f90:509-523): interior values
+        below vmin raised to vmin, then the ghost values: mps_floor."""
+        try:
+            M.check(self.lib.mps_floor(self._h, self._ptr(soln, "g", True), float(vmin)))
+        finally:
+            self._release()
+
     def electron_temp(self, eenergysoln, elecden, eenergyscale, etempmin, electemp):
         """update_electrontemp: writes electemp's interior and clamps
         eenergysoln (both in/out)."""
@@ -482,6 +565,37 @@ class FieldSolver:
         finally:
             self._release()
         return b
+
+    def restrict(self, fine, out=None):
+        """restriction (mgridlib/mgridsteps.f90:11-53): the 8-child mean of an
+        interior (nx, ny, nz) array onto (nx/2, ny/2, nz/2). A numpy input
+        returns a numpy array; with device tensors, pass `out` (nz/2, ny/2,
+        nx/2)."""
+        nc = tuple(v // 2 for v in self.n)
+        if _is_torch(fine):
+            if out is None or out.numel() != int(np.prod(nc)):
+                raise ValueError("device input needs an output tensor of %d elements" % int(np.prod(nc)))
+            dst, ret = out, out
+        else:
+            dst = ret = np.zeros(nc, order="F")
+        try:
+            p_out = self._ptr(dst, "c", True) if _is_torch(dst) else C.c_void_p(dst.ctypes.data)
+            M.check(self.lib.mps_restrict(self._h, self._ptr(fine, "i"), p_out))
+        finally:
+            self._release()
+        return ret
+
+    def prolong1(self, coarse):
+        """prolongation1 (mgridlib/mgridsteps.f90:170-216): the interior
+        (nx, ny, nz) interpolant of a ghosted (nx/2+2, ny/2+2, nz/2+2) coarse
+        array, whose ghost layer is read as given."""
+        nc = tuple(v // 2 + 2 for v in self.n)
+        cg = np.asfortranarray(np.asarray(coarse, dtype=np.float64))
+        if cg.shape != nc:
+            raise ValueError("expected coarse ghosted shape %s, got %s" % (nc, cg.shape))
+        fine = np.zeros(self.n, order="F")
+        M.check(self.lib.mps_prolong1(self._h, C.c_void_p(cg.ctypes.data), C.c_void_p(fine.ctypes.data)))
+        return fine
 
     def smooth(self, soln, niter=2, mgridsterm=None):
         res = np.zeros(self.n, order="F")

@@ -848,6 +848,45 @@ static double orc_arrh(const double *k, double T)
     return A * pow(T, alpha) * exp(-Ea / T);
 }
 
+/* getcustomrate of O2_chem.f90:413-448 (kind 1): rateconst = c0; expterm =
+ * c1/T + c2/(T**2) + c3/(T**3) + c4/(T**4) left to right (the reference
+ * writes the signs into the expression: a - b is a + (-b) exactly);
+ * rateconst*exp(expterm). Of H2_chem.f90:423-437 (kind 2):
+ * alpha*abs(efield)*exp(-E0/abs(efield)). */
+/* getcustomrate: kind 1 the fits of O2_chem.f90:413-448,
+ * q0 exp(q1/T + q2/T**2 + q3/T**3 + q4/T**4) (the fit's negative terms as
+ * negative coefficients); kind 2 H2_chem.f90:423-437,
+ * alpha |E| exp(-E0/|E|) with alpha = q0, E0 = q1 */
+static double orc_customrate(const double *q, int kind, double T, double efield)
+{
+    if (kind == 1) {
+        /* the powers as gfortran -O3 forms them from the fit's T**n: a chain
+         * of products, T**4 = ((T*T)*T)*T (the reference build's rates match
+         * this bit for bit and (T*T)*(T*T) in about two cases of three) */
+        double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+        double expterm = q[1] / T + q[2] / T2 + q[3] / T3 + q[4] / T4;
+        return q[0] * exp(expterm);
+    }
+    return q[0] * fabs(efield) * exp(-q[1] / fabs(efield));
+}
+
+/* the rate of reaction r as getspecproduction chooses it (isratearrh) */
+static double orc_rate(const orc_chem *ch, int r, double Te, double efield)
+{
+    int kind = ch->rate_kind ? ch->rate_kind[r] : 0;
+    if (kind == 0) return orc_arrh(ch->k_arrh + 3 * r, Te);
+    return orc_customrate(ch->rate_coef + 5 * r, kind, Te, efield);
+}
+
+/* efield_mag of the callers (edeneq.f90:405-409): sqrt(Ex**2 + Ey**2 + Ez**2) */
+static double orc_efield_mag(const double *efield, size_t ng, size_t c)
+{
+    double m = efield[c] * efield[c];
+    m = m + efield[ng + c] * efield[ng + c];
+    m = m + efield[2 * ng + c] * efield[2 * ng + c];
+    return sqrt(m);
+}
+
 /* specmult of reaction r: prod_j specden(j)**reactants(r,j) */
 static double orc_specmult(const orc_chem *ch, int r, const double *specden)
 {
@@ -856,15 +895,16 @@ static double orc_specmult(const orc_chem *ch, int r, const double *specden)
     return m;
 }
 
-/* getspecproduction, ion_elec_chem.f90:111-156 (specnum 1-based) */
-static double orc_specproduction(const orc_chem *ch, int specnum, double Te, const double *specden)
+/* getspecproduction, ion_elec_chem.f90:111-156, O2_chem.f90:246-291
+ * (specnum 1-based) */
+static double orc_specproduction(const orc_chem *ch, int specnum, double Te, const double *specden, double efield)
 {
     double specprod = 0.0;
     for (int r = 0; r < ch->nreac; ++r) {
         int s = specnum - 1;
         int nmoles = ch->products[r + ch->nreac * s] - ch->reactants[r + ch->nreac * s];
         if (nmoles != 0) {
-            double k = orc_arrh(ch->k_arrh + 3 * r, Te);
+            double k = orc_rate(ch, r, Te, efield);
             double specmult = orc_specmult(ch, r, specden);
             specprod = specprod + nmoles * k * specmult;
         }
@@ -872,13 +912,13 @@ static double orc_specproduction(const orc_chem *ch, int specnum, double Te, con
     return specprod;
 }
 
-/* getelectroninelasticterm, ion_elec_chem.f90:158-199 */
-static double orc_inelterm(const orc_chem *ch, double Te, const double *specden)
+/* getelectroninelasticterm, ion_elec_chem.f90:158-199, O2_chem.f90:293-334 */
+static double orc_inelterm(const orc_chem *ch, double Te, const double *specden, double efield)
 {
     double inelterm = 0.0;
     for (int r = 0; r < ch->nreac; ++r) {
         if (ch->elecenergy[r] != 0) {
-            double k = orc_arrh(ch->k_arrh + 3 * r, Te);
+            double k = orc_rate(ch, r, Te, efield);
             double specmult = orc_specmult(ch, r, specden);
             inelterm = inelterm + k * specmult * ch->elecenergy[r];
         }
@@ -958,7 +998,7 @@ void orc_species_transport(const orc_problem *p, const orc_chem *ch, int specnum
                 double mu = orc_spec_mobility(ch, specnum, sa, electemp[c]);
                 for (int q = 0; q < 3; ++q) out.vel[q * ng + c] = mu * efield[q * ng + c];
                 out.dcoeff[c] = orc_spec_dcoeff(ch, specnum, sa, electemp[c], Tg);
-                double specprod = orc_specproduction(ch, specnum, electemp[c], sa);
+                double specprod = orc_specproduction(ch, specnum, electemp[c], sa, orc_efield_mag(efield, ng, c));
                 out.source[c] = out.source[c] + specprod;
                 if (out.prod) out.prod[c] = out.source[c];
             }
@@ -1018,7 +1058,7 @@ void orc_eenergy_transport(const orc_problem *p, const orc_chem *ch, double dt, 
             for (int i = 1; i <= L.nx; ++i, ++t) {
                 size_t c = IDX(&L, i, j, k);
                 orc_specarray(ch, numden, ng, c, sa);
-                inelastic[t] = orc_inelterm(ch, electemp[c], sa) * ORC_ECHARGE;
+                inelastic[t] = orc_inelterm(ch, electemp[c], sa, orc_efield_mag(efield, ng, c)) * ORC_ECHARGE;
             }
     /* elastic_colterm (:557-612), getelectroncollisionfrequency
      * (ion_elec_chem.f90:244-258) */

@@ -72,6 +72,11 @@ typedef struct {
     const int *mob_kind, *mob_terms;
     const double *mob_coef;
     const int *dcoeff_kind;
+    /* reaction rates (NULL: every reaction getarrhrate): rate_kind (nreac)
+     * 0 Arrhenius, 1 getcustomrate of O2_chem.f90:413-448, 2 getcustomrate
+     * of H2_chem.f90:423-437; rate_coef (5, nreac), as in mps_chem */
+    const int *rate_kind;
+    const double *rate_coef;
 } orc_chem;
 
 /* Outputs of an *_update_transport: vel (3 ghosted components), dcoeff,

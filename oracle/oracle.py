@@ -175,7 +175,7 @@ class _Chem(C.Structure):
                 ("elecenergy", C.c_void_p), ("mobility", C.c_void_p), ("dcoeff", C.c_void_p),
                 ("molmass", C.c_void_p), ("charge", C.c_void_p),
                 ("mob_kind", C.c_void_p), ("mob_terms", C.c_void_p), ("mob_coef", C.c_void_p),
-                ("dcoeff_kind", C.c_void_p)]
+                ("dcoeff_kind", C.c_void_p), ("rate_kind", C.c_void_p), ("rate_coef", C.c_void_p)]
 
 
 class _TOut(C.Structure):
@@ -196,9 +196,13 @@ def _chem(ch):
         models = [np.ascontiguousarray(ch.mob_kind, dtype=np.int32), np.asfortranarray(ch.mob_terms, dtype=np.int32),
                   np.asfortranarray(ch.mob_coef, dtype=np.float64), np.ascontiguousarray(ch.dcoeff_kind, dtype=np.int32)]
         keep += models
+    rates = [None] * 2
+    if getattr(ch, "rate_kind", None) is not None:
+        rates = [np.ascontiguousarray(ch.rate_kind, dtype=np.int32), np.asfortranarray(ch.rate_coef, dtype=np.float64)]
+        keep += rates
     s = _Chem(len(ch.mobility), int(np.asarray(ch.reactants).shape[0]), int(ch.especnum), int(ch.bgspecnum),
               int(ch.ionspecmin), int(ch.ionspecmax), *[a.ctypes.data for a in keep[:8]],
-              *[None if a is None else a.ctypes.data for a in models])
+              *[None if a is None else a.ctypes.data for a in models + rates])
     return s, keep
 
 

@@ -6,9 +6,9 @@ oracle/ref_driver/*.f90) on seeded inputs and stores inputs and outputs as
 compressed .npz files. Run it in the build container (where the reference
 tree and MPICH are present):
 
-    make -C oracle ref plasma chem && python tests/golden/make_golden.py [vtk] [plasma] [chem]
+    make -C oracle ref plasma chem && python tests/golden/make_golden.py [vtk] [plasma] [plasma32] [parallel_plate] [chem]
 
-(``vtk`` / ``plasma`` / ``chem`` write only those fixtures.)
+(``vtk`` / ``plasma`` / ``plasma32`` / ``parallel_plate`` / ``chem`` write only those fixtures.)
 
 The fixtures are data only: inputs, expected outputs and residual histories.
 """
@@ -85,16 +85,19 @@ CHEM_STATES = 64
 
 def chem_fixture():
     """The reference's chem_modules (H2_O2_chem.f90, ion_elec_chem.f90,
-    driven by oracle/ref_driver/ref_chem.f90) on 64 seeded states each: their
-    tables, and per state every reaction rate, every species' production,
-    mobility and diffusion coefficient, the inelastic term and the electron
-    collision frequency."""
+    driven by oracle/ref_driver/ref_chem.f90; O2_chem.f90 and H2_chem.f90,
+    driven by oracle/ref_driver/ref_chem4.f90) on 64 seeded states each:
+    their tables, and per state every reaction rate, every species'
+    production, mobility and diffusion coefficient, the inelastic term and
+    the electron collision frequency."""
     rng = np.random.default_rng(20261204)
     out = {}
-    for name, ns, Pg in (("H2_O2", 10, 13332.2), ("ion_elec", 3, 133.0)):
+    for name, ns, Pg in (("H2_O2", 10, 13332.2), ("ion_elec", 3, 133.0), ("O2", 6, 133.0), ("H2", 5, 133.0)):
         Tg = 300.0
-        lo = {"H2_O2": [21, 21, 14, 12, 12, 12, 12, 12, 17, 17], "ion_elec": [21, 14, 14]}[name]
-        hi = {"H2_O2": [23, 23, 17, 16, 16, 16, 16, 16, 21, 21], "ion_elec": [23, 17, 17]}[name]
+        lo = {"H2_O2": [21, 21, 14, 12, 12, 12, 12, 12, 17, 17], "ion_elec": [21, 14, 14],
+              "O2": [21, 14, 12, 14, 12, 17], "H2": [21, 14, 12, 14, 17]}[name]
+        hi = {"H2_O2": [23, 23, 17, 16, 16, 16, 16, 16, 21, 21], "ion_elec": [23, 17, 17],
+              "O2": [23, 17, 16, 17, 16, 21], "H2": [23, 17, 16, 17, 21]}[name]
         Te = 10.0 ** rng.uniform(np.log10(3000.0), np.log10(1.0e5), CHEM_STATES)
         ef = 10.0 ** rng.uniform(2, 6, CHEM_STATES)
         sd = 10.0 ** rng.uniform(lo, hi, (CHEM_STATES, ns))
@@ -129,6 +132,58 @@ def plasma_fixture(chem="ion_elec"):
     save("plasma_ion_elec16" if chem == "ion_elec" else "plasma_h2_o2_16", **out)
 
 
+def plasma_deck_fixture(chem="ion_elec"):
+    """The reference plasma solver on its own deck (tests/ion_elec or
+    tests/H2_O2: 32^3 cells on the 2x4x1 process grid) after 1 and 5 time
+    steps. Per field of its VTK output: the three lines through the centre
+    (x at j = k = 16, y at i = k = 16, z at i = j = 16, 1-based), the xz
+    plane at j = 16 and the field's sum, sum of squares, min and max."""
+    from mps3d_amd.plasma import H2_O2_DECK, ION_ELEC_DECK
+    deck = dict(ION_ELEC_DECK if chem == "ion_elec" else H2_O2_DECK)
+    out = {"n": np.array(deck["n"]), "p": np.array(deck["p"])}
+    names = None
+    for k in PLASMA_STEPS:
+        _, fin = refrun.run_plasma(deck, k, chem=chem)
+        names = sorted(fin)
+        for i, nm in enumerate(names):
+            F = fin[nm]
+            h = F.shape[0] // 2 - 1
+            out["s%d_f%d_x" % (k, i)] = F[:, h, h]
+            out["s%d_f%d_y" % (k, i)] = F[h, :, h]
+            out["s%d_f%d_z" % (k, i)] = F[h, h, :]
+            out["s%d_f%d_xz" % (k, i)] = F[:, h, :]
+            out["s%d_f%d_stats" % (k, i)] = np.array([F.sum(), (F * F).sum(), F.min(), F.max()])
+    out["names"] = np.array(names)
+    out["steps"] = np.array(PLASMA_STEPS)
+    save("plasma_ion_elec32_p241" if chem == "ion_elec" else "plasma_h2_o2_32_p241", **out)
+
+
+def parallel_plate_fixture():
+    """The reference plasma solver on its tests/parallel_plate deck (ion_elec
+    chemistry, 32 x 512 x 32 cells on the 2x4x2 process grid) after 1 and 5
+    time steps. Per field of its VTK output: the lines through the centre in
+    x, y and z (i = k = 16, j = 256, 1-based), four more y lines (i, k in
+    {4, 28}), and the field's sum, sum of squares, min and max."""
+    from mps3d_amd.plasma import PARALLEL_PLATE_DECK
+    deck = dict(PARALLEL_PLATE_DECK)
+    out = {"n": np.array(deck["n"]), "p": np.array(deck["p"])}
+    names = None
+    for k in PLASMA_STEPS:
+        _, fin = refrun.run_plasma(deck, k, chem="ion_elec", timeout=3600)
+        names = sorted(fin)
+        for i, nm in enumerate(names):
+            F = fin[nm]
+            key = "s%d_f%d" % (k, i)
+            out[key + "_x"] = F[:, 255, 15]
+            out[key + "_y"] = F[15, :, 15]
+            out[key + "_z"] = F[15, 255, :]
+            out[key + "_ylines"] = np.stack([F[a, :, b] for a in (3, 27) for b in (3, 27)])
+            out[key + "_stats"] = np.array([F.sum(), (F * F).sum(), F.min(), F.max()])
+    out["names"] = np.array(names)
+    out["steps"] = np.array(PLASMA_STEPS)
+    save("plasma_parallel_plate", **out)
+
+
 def plasma_output_fixture():
     """The bytes of the reference plasma solver's VTK output (output_final:
     the .pvtr and the four ranks' .vtr) after one step of each deck at 16^3
@@ -147,7 +202,7 @@ def plasma_output_fixture():
 
 def main(argv=()):
     assert refrun.available(), "build the reference first: make -C oracle ref"
-    if "vtk" in argv or "plasma" in argv or "chem" in argv:
+    if set(argv) & {"vtk", "plasma", "chem", "plasma32", "parallel_plate"}:
         if "vtk" in argv:
             vtk_fixture()
         if "chem" in argv:
@@ -156,6 +211,11 @@ def main(argv=()):
             plasma_fixture("ion_elec")
             plasma_fixture("H2_O2")
             plasma_output_fixture()
+        if "plasma32" in argv:
+            plasma_deck_fixture("ion_elec")
+            plasma_deck_fixture("H2_O2")
+        if "parallel_plate" in argv:
+            parallel_plate_fixture()
         return
     timings = {}
 
@@ -248,6 +308,9 @@ def main(argv=()):
     plasma_fixture("ion_elec")
     plasma_fixture("H2_O2")
     plasma_output_fixture()
+    plasma_deck_fixture("ion_elec")
+    plasma_deck_fixture("H2_O2")
+    parallel_plate_fixture()
     chem_fixture()
 
     with open(os.path.join(OUT, "ref_timings.json"), "w") as fh:

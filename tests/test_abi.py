@@ -66,3 +66,27 @@ def test_create_without_device_reports_no_gpu():
     h = C.c_void_p()
     g = M.Grid(4, 4, 4, 1.0, 1.0, 1.0, 1, 1, 1)
     assert lib.mps_create(C.byref(h), C.byref(g), None) == M.ERR_NOLIB
+
+
+def test_wrong_sized_device_face_is_refused_before_any_library_call():
+    """set_bc copies ny*nz / nx*nz / nx*ny values from a device face; a face of
+    another size (here a left face shaped (nz, nx) on a non-cubic grid) is
+    refused on the host, before the library (or the GPU) is touched."""
+    torch = pytest.importorskip("torch")
+    from mps3d_amd import FieldSolver
+
+    class NoLib:
+        def __getattr__(self, name):
+            raise AssertionError("library called: %s" % name)
+
+    s = FieldSolver.__new__(FieldSolver)
+    s.n = (6, 4, 5)
+    s.lib = NoLib()
+    s._keep, s._ext, s._torch_call, s._h = [], None, False, None
+    bad = torch.zeros(5, 6, dtype=torch.float64)        # (nz, nx): 30 values, left face needs ny*nz = 20
+    with pytest.raises(ValueError, match="face 0 has 30 elements, expected 20"):
+        s.set_bc(("DIRC",) * 6, (bad, 0.0, 0.0, 0.0, 0.0, 0.0))
+    ok_left = torch.zeros(4, 5, dtype=torch.float64)
+    bad_front = torch.zeros(6 * 4 + 1, dtype=torch.float64)
+    with pytest.raises(ValueError, match="face 5 has 25 elements, expected 24"):
+        s.set_bc(("DIRC",) * 6, (ok_left, 0.0, 0.0, 0.0, 0.0, bad_front))

@@ -96,6 +96,38 @@ def test_linear_solve_gradients_match_dense_fp64_autograd(gpu):
         assert rel < 1e-8, (name, rel)
 
 
+def test_linear_solve_with_one_velocity_component(gpu):
+    """A velocity given as (None, None, vz): backward returns a gradient for
+    vz only (the missing components have no gradient slot), equal to torch
+    autograd through the dense FP64 solve with vx = vy = 0."""
+    n = (8, 8, 8)
+    L = 0.01
+    h = L / 8
+    dt = 1e-9
+    codes = ("ZGRD", "ZGRD", "ZGRD", "ZGRD", "DIRC", "DIRC")
+    g = torch.Generator().manual_seed(5)
+    shp = (8, 8, 8)
+    D = (50.0 + 50.0 * torch.rand(shp, generator=g, dtype=torch.float64)).cuda()
+    vz = (2e5 * torch.randn(shp, generator=g, dtype=torch.float64)).cuda()
+    b = (1e20 * torch.rand(shp, generator=g, dtype=torch.float64)).cuda()
+    w = torch.randn(shp, generator=g, dtype=torch.float64).cuda()
+    op = Operator(n, (L, L, L), D, codes, flux="sg", timederiv=True, dt=dt, tol=1e-13)
+    bb = b.clone().requires_grad_(True)
+    zz = vz.clone().requires_grad_(True)
+    x = linear_solve(op, bb, (None, None, zz))
+    (w * x).sum().backward()
+    op.close()
+    bd = b.clone().requires_grad_(True)
+    zd = vz.clone().requires_grad_(True)
+    zero = torch.zeros_like(vz)
+    A = dense_sg(n, h, D, [zero, zero, zd], codes, 1.0 / dt)
+    xd = torch.linalg.solve(A, bd.reshape(-1)).reshape(shp)
+    (w * xd).sum().backward()
+    for o, r, name in ((bb.grad, bd.grad, "b"), (zz.grad, zd.grad, "vz")):
+        rel = float(torch.linalg.norm(o - r) / torch.linalg.norm(r))
+        assert rel < 1e-8, (name, rel)
+
+
 def glow_gradients(model, nsteps, ckpt, V0=1000.0, s0=1.0, seed=20261202):
     nz, ny, nx = model.n[2], model.n[1], model.n[0]
     q0 = seeded_source((nz, ny, nx), seed=seed, device="cuda")

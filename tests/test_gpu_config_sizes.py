@@ -1,0 +1,166 @@
+"""Parity at the benchmark configurations' own sizes, the grid transfers
+against the reference directly, and the status codes of the reference's
+failure paths.
+
+* The variable-coefficient Poisson problem at 129^3 (its configuration) and
+  257^3 (the headline size) on the aligned hierarchy, plain FP64 and FP32
+  smoothing under FP64 defect correction, each to a relative residual of
+  1e-10: the residual is recomputed through the exact find_AX / find_bvec
+  entry points (reference rounding), and the two forms agree.
+* The reference hierarchy's 128^3 twin of that problem on 2x2x2 virtual
+  ranks in FP32 + defect correction, against the reference Fortran/MPI
+  run's converged solution (lines through the centre and whole-field sums).
+* restriction and prolongation1 (mgridsteps.f90:11-53, 170-216) through
+  mps_restrict / mps_prolong1, bit for bit against the reference's outputs.
+* Status codes: NaN coefficients (the reference's isit_nan abort,
+  gmres.f90:113-120, 396-400) give MPS_ERR_NAN; a tolerance not met in
+  max_cycles gives MPS_NOT_CONVERGED; a zero right-hand side gives x = 0
+  (gmres.f90:338-342).
+"""
+import numpy as np
+import pytest
+
+from conftest import case_from_golden, golden
+from mps3d_amd import FieldSolver, _lib as M, from_device, ghosted, to_device
+from mps3d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("n", [129, 257])
+def test_varcoef_poisson_at_config_size_fp64_and_fp32dc(gpu, n):
+    c = W.varcoef_poisson(n)
+    print("config size: var-coef Poisson %d^3" % n, flush=True)
+    sol = {}
+    for prec in ("fp64", "fp32dc"):
+        with FieldSolver(c.n, c.length, hierarchy="aligned", precision=prec, max_cycles=40, tol=1e-10,
+                         tol_kind="rel", arith="fast", start="fmg") as s:
+            s.set_case(c)
+            u = to_device(np.zeros(s.shape))
+            k, r, h, st = s.solve(u, raise_on_failure=True)
+            assert r <= 1e-10
+            ug = from_device(u)
+            del u
+            # the FP64 residual recomputed with the reference's rounding
+            ax, _, _ = s.find_AX(ug)
+            b = s.find_bvec(None, True)
+            rel = np.linalg.norm(b - ax) / np.linalg.norm(b)
+            assert rel <= 1e-10, (prec, rel)
+            sol[prec] = ug[1:-1, 1:-1, 1:-1].copy()
+    assert rel_l2(sol["fp32dc"], sol["fp64"]) <= 1e-9
+
+
+def test_reference_twin_128_fp32dc_matches_reference_run(gpu):
+    """The 128^3 twin on the reference hierarchy (2x2x2 virtual ranks), FP32
+    smoothing under FP64 defect correction to a tight tolerance: the
+    converged solution equals the reference run's (20 V-cycles) to 1e-9."""
+    d = golden("solve_varcoef128_p8")
+    c = W.varcoef_poisson(128, blobs=d["blobs"])
+    with FieldSolver(c.n, c.length, p=(2, 2, 2), hierarchy="ref", precision="fp32dc", max_cycles=40,
+                     tol=1e-13, tol_kind="rel") as s:
+        s.set_case(c)
+        u = to_device(np.zeros(s.shape))
+        k, r, h, st = s.solve(u, raise_on_failure=True)
+        U = from_device(u)[1:-1, 1:-1, 1:-1]
+    h2 = 64
+    for key, got in (("line_x", U[:, h2, h2]), ("line_y", U[h2, :, h2]), ("line_z", U[h2, h2, :])):
+        assert rel_l2(got, d[key]) <= 1e-9, key
+    st_ = d["stats"]
+    assert abs(U.sum() - st_[0]) <= 1e-9 * np.abs(U).sum()
+    assert abs((U * U).sum() - st_[1]) <= 1e-9 * st_[1]
+    assert abs(U.min() - st_[2]) <= 1e-9 * np.abs(U).max()
+    assert abs(U.max() - st_[3]) <= 1e-9 * np.abs(U).max()
+
+
+def test_restriction_and_prolongation_match_reference_bitwise(gpu):
+    d = golden("transfer")
+    c = case_from_golden(d)
+    with FieldSolver(c.n, c.length) as s:
+        s.set_case(c)
+        coarse = s.restrict(d["fine"])
+        fine = s.prolong1(d["coarse_g"])
+    assert coarse.shape == d["RESTR"].shape
+    assert np.array_equal(coarse, d["RESTR"])
+    assert np.array_equal(fine, d["PROLONG"])
+
+
+def test_restriction_of_device_arrays_and_constants(gpu):
+    """Device arrays in place; R(const) = const and the 8-child sum is
+    preserved (mgridsteps.f90:11-53) on a 32 x 16 x 8 grid."""
+    import torch
+    n = (32, 16, 8)
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal(n)
+    with FieldSolver(n, (1.0, 1.0, 1.0)) as s:
+        s.set_bc(("ZGRD",) * 6)
+        s.set_coeffs(np.ones(n))
+        fd = torch.from_numpy(np.ascontiguousarray(f.transpose(2, 1, 0))).cuda()
+        cd = torch.empty((4, 8, 16), dtype=torch.float64, device="cuda")
+        s.restrict(fd, out=cd)
+        got = cd.cpu().numpy().transpose(2, 1, 0)
+        assert np.allclose(got.sum() * 8.0, f.sum(), rtol=1e-13, atol=1e-12)
+        one = s.restrict(np.full(n, 3.25))
+        assert np.all(one == 3.25)
+
+
+def test_nan_coefficients_give_the_nan_status(gpu):
+    c = W.varcoef_poisson(16)
+    D = np.array(c.dcoeff, copy=True)
+    D[5, 6, 7] = np.nan
+    with FieldSolver(c.n, c.length, p=(2, 2, 2), max_cycles=3, tol=1e-11) as s:
+        s.set_bc(c.bc_codes, c.bc_vals)
+        s.set_coeffs(D, None, None, None, None, c.source)
+        u = np.zeros(s.shape, order="F")
+        st = s.lib.mps_solve(s._h, s._ptr(u, "g", True), None, None, None)
+        s._release()
+        assert st == M.ERR_NAN
+    # and the reference GMRES alone, on a coarse-sized grid
+    cg = W.random_cdr((4, 4, 4), seed=41)
+    Dg = np.array(cg.dcoeff, copy=True) + 2.0
+    Dg[2, 2, 2] = np.nan
+    with FieldSolver(cg.n, cg.length) as s:
+        s.set_bc(cg.bc_codes, cg.bc_vals)
+        s.set_coeffs(Dg, cg.velx, cg.vely, cg.velz, cg.reac, cg.source, cg.timederiv, cg.dt)
+        x = ghosted(np.zeros(cg.n))
+        with pytest.raises(M.MpsError) as err:
+            s.gmres(np.ones(cg.n), x, m=64, restarts=1, tol=1e-15, precond=0)
+    assert err.value.status == M.ERR_NAN
+
+
+def test_cycle_exhaustion_gives_not_converged(gpu):
+    c = W.varcoef_poisson(33)
+    for hier, p in (("aligned", (1, 1, 1)), ("ref", (1, 1, 1))):
+        cc = c if hier == "aligned" else W.varcoef_poisson(32)
+        with FieldSolver(cc.n, cc.length, p=p, hierarchy=hier, max_cycles=2, tol=1e-30, tol_kind="rel") as s:
+            s.set_case(cc)
+            u = np.zeros(s.shape, order="F")
+            k, r, h, st = s.solve(u)
+            assert st == M.NOT_CONVERGED and k == 2 and r > 1e-30
+            assert b"not met" in s.lib.mps_last_error()
+
+
+def test_zero_right_hand_side_gives_zero_solution(gpu):
+    # performgmres: a zero preconditioned right-hand side returns x = 0
+    # whatever the initial guess (gmres.f90:338-342)
+    cg = W.random_cdr((4, 4, 4), seed=41)
+    with FieldSolver(cg.n, cg.length) as s:
+        s.set_bc(cg.bc_codes, cg.bc_vals)
+        s.set_coeffs(cg.dcoeff + 2.0, cg.velx, cg.vely, cg.velz, cg.reac, cg.source, cg.timederiv, cg.dt)
+        x = ghosted(np.random.default_rng(1).standard_normal(cg.n))
+        it, rr, st = s.gmres(np.zeros(cg.n), x, m=64, restarts=1, tol=1e-15, precond=0)
+        assert st == M.OK
+        assert np.all(x[1:-1, 1:-1, 1:-1] == 0.0)
+    # the V-cycle-preconditioned FGMRES and the multigrid solve of b = 0
+    c = W.varcoef_poisson(33)
+    with FieldSolver(c.n, c.length, hierarchy="aligned", tol=1e-10, tol_kind="rel", max_cycles=5) as s:
+        s.set_case(c)
+        x = ghosted(np.random.default_rng(2).standard_normal(c.n))
+        it, rr, st = s.gmres(np.zeros(c.n), x, m=10, restarts=2, tol=1e-10, precond=1)
+        assert st == M.OK and np.all(x[1:-1, 1:-1, 1:-1] == 0.0)
+        u = np.zeros(s.shape, order="F")
+        k, r, h, st = s.solve_rhs(np.zeros(c.n), u)
+        assert np.all(u[1:-1, 1:-1, 1:-1] == 0.0)

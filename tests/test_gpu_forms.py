@@ -334,7 +334,12 @@ def test_fast_arithmetic_solves_the_same_system(gpu, hier, n, prec):
 
 def test_fast_arithmetic_meets_reference_parity(gpu):
     """The reference's 16^3 var-coef trajectory (8 ranks) solved with fast
-    arithmetic still matches the reference solution to 1e-10."""
+    arithmetic still matches the reference solution to 1e-10, and its
+    residual history the reference's to 1e-6 relative plus the rounding of
+    the residual evaluation: each cell's residual sums seven terms of about
+    the initial residual's per-cell size, so the norm carries an absolute
+    noise of a few eps times the initial norm (measured: up to 7e-16 of it);
+    the allowance is 1e-14 of the initial norm."""
     from conftest import case_from_golden, golden
     d = golden("solve_varcoef16_p8")
     c = case_from_golden(d)
@@ -346,7 +351,8 @@ def test_fast_arithmetic_meets_reference_parity(gpu):
     got = u[1:-1, 1:-1, 1:-1]
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-10
     live = d["hist"] > 1e-12 * d["hist"][0]
-    assert np.max(np.abs(hist[live] - d["hist"][live]) / d["hist"][live]) < 1e-6
+    ref_h = d["hist"][live]
+    assert np.all(np.abs(hist[live] - ref_h) <= 1e-6 * ref_h + 1e-14 * d["hist"][0])
 
 
 @pytest.mark.parametrize("hier,n,p,prec", [("ref", (16, 16, 8), (2, 2, 1), "fp64"),

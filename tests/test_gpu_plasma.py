@@ -90,3 +90,77 @@ def test_time_step_output_files(gpu, chem, tmp_path, monkeypatch):
             else:
                 scale = max(np.abs(b[nm]).max(), 1e-300)
                 assert np.abs(a[nm] - b[nm]).max() <= TOL * scale, (r, nm)
+
+
+@pytest.mark.parametrize("chem", ["ion_elec", "H2_O2"])
+@pytest.mark.parametrize("k", [1, 5])
+def test_deck_size_time_steps_follow_the_reference_run(gpu, k, chem):
+    """The decks at their own size, 32^3 cells on the 2x4x1 process grid
+    (tests/golden/plasma_ion_elec32_p241.npz, plasma_h2_o2_32_p241.npz: per
+    field the three centre lines, the xz plane at j = 16 and the field's sums
+    from the reference plasma solver's output after 1 and 5 steps)."""
+    from mps3d_amd.chemistry import h2_o2, ion_elec
+    from mps3d_amd.plasma import H2_O2_DECK, ION_ELEC_DECK, PlasmaRun
+    d = golden("plasma_ion_elec32_p241" if chem == "ion_elec" else "plasma_h2_o2_32_p241")
+    names = [str(s) for s in d["names"]]
+    deck = dict(ION_ELEC_DECK if chem == "ion_elec" else H2_O2_DECK)
+    assert tuple(deck["n"]) == tuple(int(v) for v in d["n"]) and tuple(deck["p"]) == tuple(int(v) for v in d["p"])
+    run = PlasmaRun(deck, chem=ion_elec() if chem == "ion_elec" else h2_o2())
+    try:
+        run.run(k)
+        got = run.fields()
+    finally:
+        run.close()
+    emax = max(np.abs(d["s%d_f%d_xz" % (k, i)]).max() for i, nm in enumerate(names) if nm.startswith("Electric"))
+    worst = {}
+    for i, nm in enumerate(names):
+        F = got[nm]
+        h = F.shape[0] // 2 - 1
+        key = "s%d_f%d" % (k, i)
+        ref_xz = d[key + "_xz"]
+        scale = emax if nm.startswith("Electric") else max(np.abs(ref_xz).max(), 1e-300)
+        err = max(np.abs(F[:, h, h] - d[key + "_x"]).max(), np.abs(F[h, :, h] - d[key + "_y"]).max(),
+                  np.abs(F[h, h, :] - d[key + "_z"]).max(), np.abs(F[:, h, :] - ref_xz).max()) / scale
+        st = d[key + "_stats"]
+        ssum = abs(F.sum() - st[0]) / max(np.abs(F).sum(), 1e-300)
+        worst[nm] = max(err, ssum)
+    print(chem, k, worst, run.cycles)
+    for nm, err in worst.items():
+        assert err <= TOL, (nm, err)
+
+
+@pytest.mark.parametrize("k", [1, 5])
+def test_parallel_plate_deck_follows_the_reference_run(gpu, k):
+    """The reference's tests/parallel_plate deck: ion_elec chemistry,
+    32 x 512 x 32 cells on the 2x4x2 process grid, electrodes on the y faces
+    (tests/golden/plasma_parallel_plate.npz: per field the centre lines in
+    x, y and z, four more y lines and the field's sums from the reference
+    plasma solver's output after 1 and 5 steps)."""
+    from mps3d_amd.chemistry import ion_elec
+    from mps3d_amd.plasma import PARALLEL_PLATE_DECK, PlasmaRun
+    d = golden("plasma_parallel_plate")
+    names = [str(s) for s in d["names"]]
+    deck = dict(PARALLEL_PLATE_DECK)
+    assert tuple(deck["n"]) == tuple(int(v) for v in d["n"]) and tuple(deck["p"]) == tuple(int(v) for v in d["p"])
+    print("parallel_plate: %d steps" % k, flush=True)
+    run = PlasmaRun(deck, chem=ion_elec())
+    try:
+        run.run(k)
+        got = run.fields()
+    finally:
+        run.close()
+    emax = max(np.abs(d["s%d_f%d_y" % (k, i)]).max() for i, nm in enumerate(names) if nm.startswith("Electric"))
+    worst = {}
+    for i, nm in enumerate(names):
+        F = got[nm]
+        key = "s%d_f%d" % (k, i)
+        ylines = np.stack([F[a, :, b] for a in (3, 27) for b in (3, 27)])
+        scale = emax if nm.startswith("Electric") else max(np.abs(d[key + "_y"]).max(), 1e-300)
+        err = max(np.abs(F[:, 255, 15] - d[key + "_x"]).max(), np.abs(F[15, :, 15] - d[key + "_y"]).max(),
+                  np.abs(F[15, 255, :] - d[key + "_z"]).max(), np.abs(ylines - d[key + "_ylines"]).max()) / scale
+        st = d[key + "_stats"]
+        ssum = abs(F.sum() - st[0]) / max(np.abs(F).sum(), 1e-300)
+        worst[nm] = max(err, ssum)
+    print("parallel_plate", k, worst, run.cycles)
+    for nm, err in worst.items():
+        assert err <= TOL, (nm, err)
