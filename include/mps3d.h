@@ -381,6 +381,21 @@ int mps_electron_temp(mps_ctx *ctx, double *eenergysoln, const double *elecden, 
 int mps_gmres(mps_ctx *ctx, const double *b, double *x, int m, int max_restarts, double tol,
               int precond, int *iters, double *relres);
 
+/* mps_gmres with precond = 0 (performgmres, gmres.f90:197-434, as above)
+ * that allocates nothing: no reference counterpart. For callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). b (interior) and x (ghosted, in/out)
+ * must be device arrays of the context's grid. The GMRES work space is a
+ * static device array of 2^19 doubles (4 MiB) per GPU, held from the first
+ * launch until the process ends: GMRES(m) on the grid must fit in it
+ * ((m + 9) ghosted vectors plus (m + 1)(m + 4) scalars; e.g. m = 64 on 8^3
+ * cells), else MPS_ERR_ARG. Every call on one GPU shares that array, so
+ * calls that overlap on one GPU (contexts on different host threads or
+ * streams) race; calls in sequence are independent (the kernel initialises
+ * every part of the work space it reads). Results equal mps_gmres's with
+ * precond = 0 bit for bit. */
+int mps_gmres_noalloc(mps_ctx *ctx, const double *b, double *x, int m, int max_restarts, double tol);
+
 /* Adjoint solve: A^T lambda = g (g interior, lambda ghosted out) with the
  * transposed operator and the same hierarchy; stops as mps_solve. */
 int mps_solve_transpose(mps_ctx *ctx, const double *g, double *lambda, int *ncycles,

@@ -31,7 +31,7 @@ EXPORTS = (
     "mps_efield", "mps_set_chemistry", "mps_species_transport", "mps_eenergy_transport",
     "mps_poisson_transport", "mps_electron_temp", "mps_ion_drift_faces",
     "mps_error_string", "mps_last_error", "mps_time_kernel", "mps_grad_velocity", "mps_time_stream",
-    "mps_get_grid", "mps_restrict", "mps_prolong1",
+    "mps_get_grid", "mps_gmres_noalloc", "mps_restrict", "mps_prolong1",
     "mps_eden_faces", "mps_neutral_faces", "mps_eenergy_faces", "mps_floor", "mps_set_source",
 )
 
@@ -110,6 +110,7 @@ def load():
     L.mps_stream.restype = _vp
     L.mps_grad_velocity.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
     L.mps_time_kernel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_char_p, C.c_int]
+    L.mps_gmres_noalloc.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double]
     L.mps_eden_faces.argtypes = [_vp, C.POINTER(C.c_int), _vp, _vp, C.c_double, _vp]
     L.mps_neutral_faces.argtypes = [_vp, C.POINTER(C.c_int), _vp, C.c_double, C.c_double, C.c_double, _vp]
     L.mps_eenergy_faces.argtypes = [_vp, C.POINTER(C.c_int), _vp, _vp, C.c_double, _vp, C.c_double, _vp]

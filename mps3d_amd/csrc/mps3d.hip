@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -3243,7 +3244,68 @@ int mps_gmres(mps_ctx *c, const double *b, double *x, int m, int max_restarts, d
 
 }  // extern "C"
 
+// ------------------------------------------------ no-allocation reference GMRES
+// Work space of mps_gmres_noalloc: one array in the library's code object,
+// so each GPU the library runs on holds its own copy from the first launch
+// until the process ends. Its address differs per device; host code uses
+// the address hipGetSymbolAddress returns on that device, never the
+// array's name (a host-side stand-in for it).
+constexpr long long kNoAllocWork = 1LL << 19;  // doubles: 4 MiB
+static __device__ __align__(256) double scratch[kNoAllocWork];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// the device address of `scratch` on device dev (the current device),
+// looked up on the first call per device and kept
+int scratch_on(int dev, double **out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(MPS_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MPS_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch[dev];
+    return MPS_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int mps_gmres_noalloc(mps_ctx *c, const double *b, double *x, int m, int max_restarts, double tol)
+{
+    int st = guard(c);
+    if (st) return st;
+    if (!b || !x || m < 1 || max_restarts < 1 || !(tol >= 0)) return fail(MPS_ERR_ARG, "bad arguments");
+    if (!is_device_ptr(b) || !is_device_ptr(x)) return fail(MPS_ERR_ARG, "mps_gmres_noalloc takes device arrays");
+    const Geom &g = c->geom[0];
+    if (g.n() > kBottomMax)
+        return fail(MPS_ERR_ARG, "mps_gmres_noalloc supports at most %d cells (grid has %lld)", kBottomMax, g.n());
+    const long long need = bottom_work_elems<double>(m, g.ng());
+    if (need > kNoAllocWork)
+        return fail(MPS_ERR_ARG,
+                    "GMRES(%d) on a %dx%dx%d grid needs %lld doubles of work space; the static work space holds "
+                    "%lld",
+                    m, g.nx, g.ny, g.nz, need, kNoAllocWork);
+    double *work = nullptr;
+    st = scratch_on(c->device, &work);
+    if (st) return st;
+    try {
+        gmres_ref_launch(c, b, x, m, max_restarts, tol, work);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+    return gmres_ref_status(c);
+}
 
 int mps_apply_transpose(mps_ctx *c, const double *x, double *atx)
 {
