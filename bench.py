@@ -37,6 +37,11 @@ PEAK_HBM_GBPS = 8000.0
 BYTES_PER_CELL_CYCLE = {"fp64": 302.9, "fp32dc": 199.4}
 
 
+def log(msg):
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def plain_start_ms(case, hier, prec, steps, tol):
     """ms per solve with V-cycles from the zero guess throughout (no
     full-multigrid first pass), for comparison."""
@@ -102,13 +107,22 @@ def gpu_solves(n, steps, warmup, length=0.02, tol=1e-10, start="fmg"):
         ent["vcycles"] = 1 + (k - 1) * inner if st_ == "fmg" else k * inner
         ent["unknowns_per_s"] = float(np.prod(case.n)) * ent["vcycles"] / (wall * 1e-3)
         ent["solve_GBps"] = BYTES_PER_CELL_CYCLE[prec] * float(np.prod(case.n)) * ent["vcycles"] / (wall * 1e-3) / 1e9
-        # hot kernels, timed live with HIP events on the library's stream
+        # hot kernels on the finest level: every launch of one solve like the
+        # timed ones bracketed by HIP events on the library's stream
+        # (avg_us, GBps), and the isolated hook with the Infinity Cache
+        # evicted before each launch (hook_us)
         kern = {}
         for kid, label in ((0, "colour"), (1, "residual"), (2, "restrict"), (3, "prolong")):
             fp32 = prec == "fp32dc" and kid != 1
-            ms, by, name = s.time_kernel(kid, fp32=fp32, reps=50)
-            kern[label] = {"name": name, "fp32": fp32, "avg_us": ms * 1e3, "alg_bytes": by,
-                           "GBps": by / (ms * 1e-3) / 1e9}
+            ms_h, by, name = s.time_kernel(kid, fp32=fp32, reps=20)
+            u.zero_()
+            ms, nl, _ = s.time_kernel_in_solve(u, kid, fp32=fp32)
+            if kid == 2 and hier == "aligned" and prec == "fp32dc":
+                # the finest FP32 restriction is fused with its residual:
+                # (c + 2) w + w/8 bytes per fine cell
+                name, by = "k_resrestrict_planes", (1.0 + 2.0 + 0.125) * 4.0 * float(np.prod(case.n))
+            kern[label] = {"name": name, "fp32": fp32, "avg_us": ms * 1e3, "launches": nl, "alg_bytes": by,
+                           "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None, "hook_us": ms_h * 1e3}
         ent["kernels"] = kern
         if prec == "fp64":
             # adjoint: A^T lambda = g with the transposed operator, same tolerance
@@ -166,12 +180,13 @@ def step_kernels(n, reps=20):
     """The per-step work either side of the solves, on device arrays of the
     same grid: update_efield (mps_efield) and the transport assembly of the
     ion_elec chemistry (electron and energy *_update_transport, Poisson's,
-    update_electrontemp). Mean wall time per call over `reps` calls (each
-    call ends with a stream synchronisation, so this is an upper bound on
-    the kernel time) and the algorithmic GB/s it implies: every array the
-    kernel reads or writes once, ghosted. The fastest single call is given
-    beside the mean: a call's wall time also holds the host's wake-up from
-    the stream synchronisation, which now and then adds milliseconds."""
+    update_electrontemp). Mean wall time per call over `reps` calls, each
+    followed by a device synchronisation (calls on device arrays return with
+    their kernels queued), so this is an upper bound on the kernel time, and
+    the algorithmic GB/s it implies: every array the kernel reads or writes
+    once, ghosted. The fastest single call is given beside the mean: a
+    call's wall time also holds the host's wake-up from the synchronisation,
+    which now and then adds milliseconds."""
     import torch
 
     from mps3d_amd import FieldSolver
@@ -229,9 +244,10 @@ def step_kernels(n, reps=20):
         fn()
         torch.cuda.synchronize()
         per = []
-        for _ in range(reps):   # each call returns with the library's stream synchronized
+        for _ in range(reps):
             t0 = time.perf_counter()
             fn()
+            torch.cuda.synchronize()
             per.append(time.perf_counter() - t0)
         us = sum(per) * 1e6 / reps
         us_min = min(per) * 1e6
@@ -246,11 +262,15 @@ def step_kernels(n, reps=20):
     # reference's ion_elec deck and on a 128^3 version of it
     from mps3d_amd.chemistry import h2_o2, ion_elec
     from mps3d_amd.plasma import H2_O2_DECK, ION_ELEC_DECK, PlasmaRun
-    for tag, base, chem, n, reps in (("32", ION_ELEC_DECK, ion_elec, 32, 10), ("128", ION_ELEC_DECK, ion_elec, 128, 3),
-                                     ("h2_o2_32", H2_O2_DECK, h2_o2, 32, 10)):
+    for tag, base, chem, n, reps, arith in (("32", ION_ELEC_DECK, ion_elec, 32, 10, "exact"),
+                                            ("128", ION_ELEC_DECK, ion_elec, 128, 3, "exact"),
+                                            ("h2_o2_32", H2_O2_DECK, h2_o2, 32, 10, "exact"),
+                                            ("32_fast", ION_ELEC_DECK, ion_elec, 32, 10, "fast"),
+                                            ("128_fast", ION_ELEC_DECK, ion_elec, 128, 3, "fast"),
+                                            ("h2_o2_32_fast", H2_O2_DECK, h2_o2, 32, 10, "fast")):
         deck = dict(base)
         deck["n"] = (n, n, n)
-        run = PlasmaRun(deck, chem=chem())
+        run = PlasmaRun(deck, chem=chem(), arith=arith)
         run.run(2)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -274,55 +294,167 @@ def measured_traffic(kernel, n):
     return d.get("traffic_bytes") if str(d.get("kernel", "")).split("<")[0] == kernel else None
 
 
-def cpu_baseline(n_gpu, cycles_gpu, tol=1e-10):
-    """Reference CPU path on the host cores, on a bounded sample, scaled to
-    milliseconds per n_gpu^3 solve: the reference's time per cell-cycle
-    times n_gpu^3 times the number of its own V-cycles to the same relative
-    residual (counted from its residual history on the sample; the GPU's
-    count when the sample does not reach the tolerance)."""
+# ranks of the reference CPU run: the box's CPU share for one GPU is 16
+# cores, whatever os.cpu_count() reports there
+CPU_SHARE = 16
+TWIN_N = 256
+
+
+def twin_ranks():
+    ncpu = min(os.cpu_count() or 1, CPU_SHARE)
+    for r, p in ((64, (4, 4, 4)), (32, (2, 4, 4)), (16, (2, 2, 4)), (8, (2, 2, 2)), (4, (1, 2, 2)), (2, (1, 1, 2))):
+        if ncpu >= r:
+            return p
+    return (1, 1, 1)
+
+
+def twin_tolerance(case, tol_rel):
+    """The reference's stopping rule (an absolute RMS of the residual) for a
+    relative residual tol_rel: tol_rel times the RMS of b (the residual of
+    the zero initial guess)."""
+    from oracle import oracle as O
+    P = O.Problem(case.n, case.length, (1, 1, 1), case.bc_codes, case.bc_vals, case.dcoeff, source=case.source)
+    return tol_rel * O.rms(O.bvec(P))
+
+
+def cpu_baseline(n=TWIN_N, tol_rel=1e-10, max_cycles=40):
+    """The reference CPU path timed directly on the n^3 twin of the benchmark
+    problem (even n: the reference's own hierarchy on its process grid): the
+    reference Fortran/MPI library built from the reference sources into
+    oracle/_ref/, one rank per core of the GPU's host share, V-cycles from
+    the zero guess until its RMS residual reaches tol_rel times that of the
+    zero guess (its own stopping rule). Falls back to the oracle's C
+    restatement on a 32^3 sample when the reference build or mpiexec is
+    absent."""
     from mps3d_amd import workloads as W
     from oracle import refrun
 
     ncpu = os.cpu_count() or 1
+    case = W.varcoef_poisson(n)
+    tol_abs = twin_tolerance(case, tol_rel)
+    ranks = twin_ranks()
     if refrun.available():
-        ranks = (2, 2, 4) if ncpu >= 16 else (2, 2, 2)
-        n, ncyc = 128, 40
-        case = W.varcoef_poisson(n)
         d = dict(mode=refrun.MODE_SOLVE, n=case.n, len=case.length, p=ranks, bc_codes=list(case.bc_codes),
-                 bc_vals=case.bc_vals, dcoeff=case.dcoeff, source=case.source, nvcycles=ncyc, tol=0.0)
+                 bc_vals=case.bc_vals, dcoeff=case.dcoeff, source=case.source, nvcycles=max_cycles, tol=tol_abs)
         try:
-            o = refrun.run(d, timeout=600)
+            o = refrun.run(d, timeout=900)
             t = float(o["TIME"][0])
-            per_cell_cycle = t / (n ** 3 * ncyc)
-            # its own cycles to the relative tolerance: RMS history over the
-            # RMS of b (the residual of the zero initial guess)
-            from oracle import oracle as O
-            P = O.Problem(case.n, case.length, (1, 1, 1), case.bc_codes, case.bc_vals, case.dcoeff,
-                          source=case.source)
-            rms0 = O.rms(O.bvec(P))
             hist = np.asarray(o["HIST"], dtype=float)
-            hit = np.nonzero((hist > 0) & (hist <= tol * rms0))[0]
-            cyc = int(hit[0]) + 1 if hit.size else cycles_gpu
-            return {"value": per_cell_cycle * n_gpu ** 3 * cyc * 1e3, "unit": "ms per solve (scaled)",
-                    "cores": int(np.prod(ranks)), "kind": "reference",
-                    "sample": "reference Fortran/MPI (amdflang -O3, MPICH) on the 128^3 twin of the same "
-                              "var-coef Poisson, %d ranks, %d V-cycles in %.2f s (%d to relative residual %.0e); "
-                              "scaled per cell-cycle to %d^3 x %d cycles"
-                              % (int(np.prod(ranks)), ncyc, t, cyc, tol, n_gpu, cyc)}
+            cyc = int(np.count_nonzero(hist > 0))
+            return {"value": round(t * 1e3, 1), "unit": "ms per %d^3 solve to rel. residual %.0e (reference algorithm)"
+                    % (n, tol_rel), "cores": int(np.prod(ranks)), "kind": "reference", "os_cpu_count": ncpu,
+                    "cycles": cyc, "converged": bool(hist[cyc - 1] <= tol_abs) if cyc else False,
+                    "sample": "reference Fortran/MPI (amdflang -O3, MPICH), %d ranks on a %dx%dx%d process grid, "
+                              "the %d^3 twin of the same var-coef Poisson, FP64, V(2,2) from the zero guess: "
+                              "%d V-cycles in %.2f s, measured (not scaled)"
+                              % (int(np.prod(ranks)), ranks[0], ranks[1], ranks[2], n, cyc, t)}, tol_abs, ranks
         except Exception as e:  # fall through to the port
             err = str(e)[:200]
     else:
         err = "reference binary or mpiexec missing"
     from oracle import oracle as O
-    n, ncyc = 32, 10
-    case = W.varcoef_poisson(n)
-    P = O.Problem(case.n, case.length, (1, 1, 1), case.bc_codes, case.bc_vals, case.dcoeff, source=case.source)
+    m, ncyc = 32, 10
+    c32 = W.varcoef_poisson(m)
+    P = O.Problem(c32.n, c32.length, (1, 1, 1), c32.bc_codes, c32.bc_vals, c32.dcoeff, source=c32.source)
     t = time.perf_counter()
     O.solve(P, P.zeros(), ncyc, 0.0)
     t = time.perf_counter() - t
-    return {"value": t / (n ** 3 * ncyc) * n_gpu ** 3 * cycles_gpu * 1e3, "unit": "ms per solve (scaled)",
-            "cores": 1, "kind": "port",
-            "sample": "oracle C restatement, 32^3 twin, 10 V-cycles in %.2f s, scaled (%s)" % (t, err)}
+    return {"value": round(t / (m ** 3 * ncyc) * n ** 3 * 14 * 1e3, 1), "unit": "ms per solve (scaled)",
+            "cores": 1, "kind": "port", "os_cpu_count": ncpu,
+            "sample": "oracle C restatement, 32^3 twin, 10 V-cycles in %.2f s, scaled to %d^3 x 14 cycles (%s)"
+                      % (t, n, err)}, tol_abs, ranks
+
+
+def gpu_reference_twin(tol_abs, ranks, n=TWIN_N, steps=3, max_cycles=40):
+    """The GPU running the reference algorithm on the same twin and process
+    grid as the CPU baseline: reference hierarchy, plain FP64, the same RMS
+    stopping rule, exact arithmetic (the reference's rounding) and fast
+    arithmetic. ms per solve from the zero guess."""
+    import torch
+
+    from mps3d_amd import FieldSolver, to_device
+    from mps3d_amd import workloads as W
+
+    case = W.varcoef_poisson(n)
+    out = {"n": n, "p": list(ranks), "tol_rms": tol_abs}
+    for arith in ("exact", "fast"):
+        s = FieldSolver(case.n, case.length, hierarchy="ref", p=ranks, precision="fp64", max_cycles=max_cycles,
+                        tol=tol_abs, tol_kind="rms", arith=arith)
+        s.set_case(case)
+        u = to_device(np.zeros(s.shape))
+        s.solve(u)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            u.zero_()
+            k, r, _, st = s.solve(u)
+        torch.cuda.synchronize()
+        out[arith] = {"ms_per_solve": round((time.perf_counter() - t0) * 1e3 / steps, 2), "cycles": int(k),
+                      "status": int(st)}
+        s.close()
+        del u
+        torch.cuda.empty_cache()
+    return out
+
+
+def stream_ceilings(bytes_per_array=1 << 29, reps=10):
+    """Hand-written streaming kernels (mps_time_stream: 16-byte accesses,
+    256 threads) over 512 MiB per array, far beyond the 256 MB Infinity
+    Cache: read, write, copy, triad and the sweep's pattern (three streams
+    read, one written), GB/s."""
+    from mps3d_amd.solver import STREAM_KINDS, time_stream
+    return {k: round(time_stream(k, bytes_per_array, reps)[0], 1) for k in STREAM_KINDS}
+
+
+def coupled(steps_c3=4):
+    """The coupled configurations around the field solve: the 128^3 glow
+    discharge (FP64, GMRES(30) preconditioned by one V-cycle, dt 1e-11) per
+    implicit step, and the adjoint gradient of the 96^3 model over 50 steps
+    with checkpoints every 10 steps against its forward pass."""
+    import torch
+
+    from mps3d_amd import discharge as DS
+    from mps3d_amd.coupled import GlowModel, seeded_source
+    out = {}
+    model, s0 = DS.glow_discharge(n=128, tol=1e-10)
+    V = torch.tensor([1000.0], dtype=torch.float64, device="cuda")
+    with torch.no_grad():
+        st = model.run(V, 1, s0)
+        torch.cuda.synchronize()
+        model.step_times.clear()
+        model.run(V, steps_c3, st, timed=True)
+    counts = model.solve_counts()
+    out["glow_discharge_128"] = {"ms_per_step": round(1e3 * float(np.median(model.step_times)), 2),
+                                 "steps": steps_c3,
+                                 "gmres_iterations_last": {k: v[-1] for k, v in counts.items() if k != "poisson"},
+                                 "poisson_cycles_last": counts["poisson"][-1]}
+    model.close()
+    torch.cuda.empty_cache()
+    m = GlowModel(n=96, dt=1e-11, p=(3, 3, 3), cycles=(30, 20))
+    q0 = seeded_source((96, 96, 96), device="cuda")
+    Vt = torch.tensor(1000.0, dtype=torch.float64, device="cuda")
+    sm = torch.tensor(1.0, dtype=torch.float64, device="cuda")
+    with torch.no_grad():
+        m.run(Vt, sm, q0, 2)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        m.run(Vt, sm, q0, 50)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t
+    Vg = Vt.clone().requires_grad_(True)
+    sg = sm.clone().requires_grad_(True)
+    qg = q0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    J = m.run(Vg, sg, qg, 50, checkpoint_every=10)
+    J.backward()
+    torch.cuda.synchronize()
+    t_both = time.perf_counter() - t
+    out["adjoint_96_50steps_ckpt10"] = {"forward_s": round(t_fwd, 3), "forward_and_gradient_s": round(t_both, 3),
+                                        "adjoint_over_forward": round((t_both - t_fwd) / t_fwd, 3)}
+    m.close()
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -348,6 +480,7 @@ def main():
         print(json.dumps({"n": args.n, "step_kernels": step_kernels(args.n)}), flush=True)
         return
     case, hier, res, err = gpu_solves(args.n, args.steps, args.warmup)
+    log("solves done")
     copy_gbps = calibration_copy()
     dc, f64 = res["fp32dc"], res["fp64"]
     kc = dc["kernels"]["colour"]
@@ -378,13 +511,27 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kc["name"] + " (FP32 red-black sweep, finest level)",
                      "achieved": round(kc["GBps"], 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": round(kc["GBps"] / PEAK_HBM_GBPS, 4), "traffic": traffic,
-                     "avg_us": round(kc["avg_us"], 2), "alg_bytes": kc["alg_bytes"],
+                     "avg_us": round(kc["avg_us"], 2), "launches_timed": kc["launches"],
+                     "timing": "HIP events around every finest-level launch of one solve like the timed ones",
+                     "avg_us_isolated_cold_cache": round(kc["hook_us"], 2), "alg_bytes": kc["alg_bytes"],
                      "copy_GBps": round(copy_gbps, 1), "frac_of_copy": round(kc["GBps"] / copy_gbps, 4)},
         "kernels": {"fp32dc": dc["kernels"], "fp64": f64["kernels"]},
     }
+    line["stream_GBps"] = stream_ceilings()
+    ceiling = line["stream_GBps"]["sweep"]
+    line["roofline"]["stream_ceiling_GBps"] = ceiling
+    line["roofline"]["frac_of_stream"] = round(kc["GBps"] / ceiling, 4)
+    log("stream ceilings done")
     line["step_kernels"] = step_kernels(args.n)
+    log("step kernels done")
+    line["coupled"] = coupled()
+    line["adjoint_fwd_ratio_c4"] = line["coupled"]["adjoint_96_50steps_ckpt10"]["adjoint_over_forward"]
+    log("coupled configurations done")
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.n, f64["cycles"])
+        cb, tol_abs, ranks = cpu_baseline()
+        line["cpu_baseline"] = cb
+        log("cpu baseline done")
+        line["gpu_reference_twin"] = gpu_reference_twin(tol_abs, ranks)
     print(json.dumps(line), flush=True)
 
 

@@ -419,9 +419,12 @@ int mps_get_grid(mps_ctx *ctx, mps_grid *grid, int *nspecies);
  * (HIP events): total milliseconds and per-cycle milliseconds. */
 int mps_last_timing(mps_ctx *ctx, double *total_ms, int *ncycles);
 
-/* Measurement hook (no reference counterpart): times `reps` back-to-back
- * launches of one hot kernel of the current hierarchy with HIP events on
- * the context's stream, after the context has solved once.
+/* Measurement hook (no reference counterpart): times `reps` launches of one
+ * hot kernel of the current hierarchy with HIP events on the context's
+ * stream, after the context has solved once. Each launch is timed alone,
+ * after 1 GiB of a scratch buffer (allocated for the call) has been read,
+ * so it starts with the Infinity Cache holding none of its data, as inside
+ * a solve.
  *   kernel 0  smoothing on the finest level: the two red-black sweeps of a
  *             smoothing step in one pass where that kernel applies (FP32
  *             rows from one coefficient field without convection or
@@ -436,6 +439,20 @@ int mps_last_timing(mps_ctx *ctx, double *total_ms, int *ncycles);
  * bytes) the kernel's symbol for matching profiler output. */
 int mps_time_kernel(mps_ctx *ctx, int kernel, int fp32, int reps, double *avg_ms, double *alg_bytes,
                     char *name, int name_len);
+
+/* Measurement hook (no reference counterpart): runs mps_solve on soln (as
+ * mps_solve; ncycles nullable) with every launch of one hot kernel on the
+ * finest level timed by its own pair of HIP events on the context's stream,
+ * and returns the mean launch time and the number of launches timed (at
+ * most 256). kernel: 0 the red-black sweep (mps_time_kernel's kernel 0
+ * where it applies; sweeps from a zero guess, which read no iterate, are
+ * not timed), 1 the residual (for MPS_PREC_FP32_DC with fp32 = 0 the FP64
+ * defect), 2 the V-cycle's restriction from the finest level (with the
+ * residual where the two are one kernel), 3 the V-cycle's prolongation to
+ * the finest level;
+ * fp32 selects the FP32 inner hierarchy. The solve replays no graphs while
+ * timed. Returns mps_solve's status. */
+int mps_time_in_solve(mps_ctx *ctx, int kernel, int fp32, double *soln, int *ncycles, double *avg_ms, int *launches);
 
 /* Measurement hook (no reference counterpart): the HBM rate a hand-written
  * streaming kernel reaches on the current device, 16-byte accesses per lane,

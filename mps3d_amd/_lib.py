@@ -31,6 +31,7 @@ EXPORTS = (
     "mps_efield", "mps_set_chemistry", "mps_species_transport", "mps_eenergy_transport",
     "mps_poisson_transport", "mps_electron_temp", "mps_ion_drift_faces",
     "mps_error_string", "mps_last_error", "mps_time_kernel", "mps_grad_velocity", "mps_time_stream",
+    "mps_time_in_solve",
     "mps_get_grid", "mps_gmres_noalloc", "mps_restrict", "mps_prolong1",
     "mps_eden_faces", "mps_neutral_faces", "mps_eenergy_faces", "mps_floor", "mps_set_source",
 )
@@ -120,6 +121,7 @@ def load():
     L.mps_prolong1.argtypes = [_vp, _vp, _vp]
     L.mps_get_grid.argtypes = [_vp, C.POINTER(Grid), C.POINTER(C.c_int)]
     L.mps_time_stream.argtypes = [C.c_int, C.c_longlong, C.c_int, _dp, _dp]
+    L.mps_time_in_solve.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
     L.mps_error_string.argtypes = [C.c_int]
     L.mps_error_string.restype = C.c_char_p
     L.mps_last_error.argtypes = []

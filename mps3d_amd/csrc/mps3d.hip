@@ -150,6 +150,12 @@ struct GmresBufs {
     double *w = nullptr, *bg = nullptr, *dh = nullptr, *dpart = nullptr;
     double *hh = nullptr;  // pinned host mirror of dh
     int nblocks = 0;
+    // device Arnoldi state: H, cs, sn, gv, y, hn, the second Gram-Schmidt
+    // pass's products h2, the subtracted coefficients ny and |w|^2
+    double *darn = nullptr;
+    int *dstate = nullptr;
+    int *hflag = nullptr;                // pinned: per step the done flag, then kk and the NaN flag
+    std::vector<hipEvent_t> ev;          // one per Arnoldi step
 };
 
 }  // namespace
@@ -201,6 +207,12 @@ struct mps_ctx {
     int sweep2 = 1;                      // precisions whose smoothing pairs run in one pass
                                          // (bit 1 FP32, bit 2 FP64; MPS3D_SWEEP2 overrides)
     StepGraph step[2];
+    // in-solve kernel timing (mps_time_in_solve): the kernel and precision
+    // whose launches get an event pair, the events, pairs recorded
+    int kt_kernel = -1;
+    bool kt_fp32 = false;
+    std::vector<hipEvent_t> kt_ev;
+    int kt_n = 0;
     unsigned gen = 0;                    // bumped when set_bc / set_coeffs change what
                                          // captured step graphs bake in (see touch_gen)
     int sig_bc[6] = {-1, -1, -1, -1, -1, -1};
@@ -562,6 +574,26 @@ void launch_colour(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int red, int 
 // z-planes per block of the plane-marching kernels: enough blocks to fill
 // the chip several times over on large grids, short serial z chains (each
 // plane waits on one memory round trip) on small ones
+// In-solve kernel timing: while mps_time_in_solve runs a solve, a KtScope
+// around a finest-level launch of the timed kernel (0 sweep, 1 residual,
+// 2 restriction, 3 prolongation) records an event before and after it on
+// the context's stream.
+constexpr int kKtMax = 256;  // launches recorded per timed solve
+
+struct KtScope {
+    mps_ctx *c;
+    bool on;
+    KtScope(mps_ctx *c_, int kernel, bool fp32)
+        : c(c_), on(c_->kt_kernel == kernel && c_->kt_fp32 == fp32 && c_->kt_n < kKtMax)
+    {
+        if (on) HIPCHK(hipEventRecord(c->kt_ev[2 * c->kt_n], c->stream));
+    }
+    ~KtScope()
+    {
+        if (on && hipEventRecord(c->kt_ev[2 * c->kt_n + 1], c->stream) == hipSuccess) ++c->kt_n;
+    }
+};
+
 int z_chunk(const Geom &g, int tx, int ty, int target_blocks, int kmax)
 {
     const long long cols = (long long)((g.nx + tx - 1) / tx) * ((g.ny + ty - 1) / ty);
@@ -701,6 +733,7 @@ int launch_residual(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *u, 
 {
     Lev<T> v = view(c, L, l);
     const Geom &g = c->geom[l];
+    KtScope kt(c, l == 0 ? 1 : -2, sizeof(T) == 4);
     if (v.fast && op != OP_STENCIL && !c->has_vel) {
         int kz;
         const dim3 gr = residual_planes_grid(g, kz);
@@ -776,9 +809,6 @@ void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *wo
 #ifndef MPS_SWEEP_KZ
 #define MPS_SWEEP_KZ 16
 #endif
-#ifndef MPS_SWEEP_PD
-#define MPS_SWEEP_PD 1
-#endif
 constexpr int kSweepTX = MPS_SWEEP_TX, kSweepTY = MPS_SWEEP_TY, kSweepKZ = MPS_SWEEP_KZ;
 
 // per-face multipliers of the staged ghost coefficient (k_sweep_ref,
@@ -799,6 +829,8 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
 {
     Lev<T> v = view(c, L, l);
     const Geom &g = c->geom[l];
+    // (the zero-guess sweeps, which read no u, are not timed)
+    KtScope kt(c, l == 0 && !zero_in ? 0 : -2, sizeof(T) == 4);
 #ifndef MPS_SWEEP_TARGET
 #define MPS_SWEEP_TARGET 2048
 #endif
@@ -816,15 +848,18 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
                                c->stream, v, G, uin, uout, (const T *)L[l].b, kz);
         return;
     }
+    // z ghost-plane multipliers of the fast kernel's OP_REF rows
+    const GhostD<T> G = ghost_multipliers(c, v);
+    const T zg0 = G.m[BACK], zg1 = G.m[FRONT];
     auto go = [&](auto O, auto R) {
         if (v.fast && zero_in)
             hipLaunchKernelGGL(
-                (k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, MPS_SWEEP_PD, true>), gr,
-                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
+                (k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, true>), gr,
+                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz, zg0, zg1);
         else if (v.fast)
             hipLaunchKernelGGL(
-                (k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, MPS_SWEEP_PD, false>), gr,
-                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
+                (k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, false>), gr,
+                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz, zg0, zg1);
         else
             hipLaunchKernelGGL((k_sweep<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
                                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
@@ -834,11 +869,11 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
         // kernel's arithmetic for them is the per-level kernels' in either
         // arithmetic mode
         if (zero_in)
-            hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, MPS_SWEEP_PD, true>), gr,
-                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
+            hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, true>), gr,
+                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz, zg0, zg1);
         else
-            hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, MPS_SWEEP_PD, false>), gr,
-                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
+            hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, false>), gr,
+                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz, zg0, zg1);
     } else if (op == OP_REF) {
         if (c->has_reac) go(IntC<OP_REF>{}, BoolC<true>{});
         else go(IntC<OP_REF>{}, BoolC<false>{});
@@ -851,12 +886,16 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
 template <typename T>
 bool fused_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op)
 {
-    // (the fused kernels address planes with 32-bit offsets). Explicit rows
+    // (the fused kernels address planes with 32-bit offsets, and the fast
+    // sweep's buffer resources span a z-chunk and two planes either side in
+    // fewer than 2^31 bytes). Explicit rows
     // -- conductance coarse rows, Scharfetter-Gummel rows with or without
     // drift, transposed rows -- carry everything a Gauss-Seidel update
     // needs; rows formed from coefficients are fused only without
     // convection (the fused kernels have no upwind terms)
-    return (op == OP_STENCIL || !c->has_vel) && L[l].u2 && c->geom[l].ng() < (1LL << 31);
+    const Geom &g = c->geom[l];
+    const long long slab = (long long)(kSweepKZ + 4) * (g.nx + 2) * (g.ny + 2) * (long long)sizeof(T);
+    return (op == OP_STENCIL || !c->has_vel) && L[l].u2 && g.ng() < (1LL << 31) && slab < (1LL << 31);
 }
 
 // Two sweeps in one pass (k_sweep2_fast): FP32 fast rows from one
@@ -1033,8 +1072,11 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
     Lev<T> vf = view(c, L, l), vc = view(c, L, l + 1);
     const int opc = level_op(c, L, l + 1);
     if (c->opts.hierarchy == MPS_HIER_REF) {
-        hipLaunchKernelGGL(k_restrict_fas<T>, cells_grid(gc), block(), 0, c->stream, vf, vc, (const T *)L[l].u,
-                           (const T *)L[l].r, L[l + 1].uold, L[l + 1].u, L[l + 1].b);
+        {
+            KtScope kt(c, l == 0 ? 2 : -2, sizeof(T) == 4);
+            hipLaunchKernelGGL(k_restrict_fas<T>, cells_grid(gc), block(), 0, c->stream, vf, vc, (const T *)L[l].u,
+                               (const T *)L[l].r, L[l + 1].uold, L[l + 1].u, L[l + 1].b);
+        }
         with_op(opc, c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
             hipLaunchKernelGGL((k_tau<T, decltype(O)::value, decltype(V)::value, decltype(R)::value>),
                                cells_grid(gc), block(), 0, c->stream, vc, (const T *)L[l + 1].uold, L[l + 1].b);
@@ -1042,12 +1084,17 @@ void vcycle(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, T *bwork, bool homog
         vcycle<T>(c, L, l + 1, bwork, homog);
         FacesT<T> G{};
         for (int f = 0; f < 6; ++f) G.f[f] = homog ? nullptr : L[l + 1].faces[f];
+        KtScope kt(c, l == 0 ? 3 : -2, sizeof(T) == 4);
         hipLaunchKernelGGL(k_prolong1_correct<T>, cells_grid(gc), block(), 0, c->stream, vf, vc, L[l].u,
                            (const T *)L[l + 1].u, (const T *)L[l + 1].uold, G);
     } else {
-        if (fused) launch_resrestrict<T>(c, L, l, op);
-        else launch_restrict_fw<T>(c, L, l);
+        {
+            KtScope kt(c, l == 0 ? 2 : -2, sizeof(T) == 4);
+            if (fused) launch_resrestrict<T>(c, L, l, op);
+            else launch_restrict_fw<T>(c, L, l);
+        }
         vcycle<T>(c, L, l + 1, bwork, homog, true);
+        KtScope kt(c, l == 0 ? 3 : -2, sizeof(T) == 4);
         launch_prolong_tri<T>(c, L, l);
     }
     smooth_level<T>(c, L, l, c->opts.nu2, op, false, swap_odd);
@@ -1770,6 +1817,7 @@ int mps_destroy(mps_ctx *c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+    for (auto e : c->kt_ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MPS_OK;
@@ -2263,14 +2311,12 @@ int mps_efield(mps_ctx *c, double *phi, double *efield, double scale, int limite
         const long long nt = (long long)g.ny * g.nz + (long long)g.nx * g.nz + (long long)g.nx * g.ny;
         const dim3 gg((unsigned)((nt + kBlock - 1) / kBlock));
         hipLaunchKernelGGL(k_ghosts, gg, block(), 0, c->stream, v, u, F);
-        hipLaunchKernelGGL(k_efield, cells_grid(g), block(), 0, c->stream, v, (const double *)u, E, E + ng,
-                           E + 2 * ng, scale, limiter);
-        // zero-gradient ghost faces of each component (update_efield's
-        // efield_bc_codes = 'ZGRD', poissoneq.f90:580-583, 670-678)
-        Lev<double> z = v;
-        for (int f = 0; f < 6; ++f) z.bc[f] = BC_ZGRD;
-        const Faces none{};
-        for (int q = 0; q < 3; ++q) hipLaunchKernelGGL(k_ghosts, gg, block(), 0, c->stream, z, E + q * ng, none);
+        // interior and zero-gradient ghost faces of each component in one
+        // pass over whole rows
+        const long long plane = (long long)(g.nx + 2) * (g.ny + 2);
+        const dim3 ge((unsigned)((plane + kBlock - 1) / kBlock), (unsigned)(g.nz + 2));
+        hipLaunchKernelGGL(k_efield, ge, block(), 0, c->stream, v, (const double *)u, E, E + ng, E + 2 * ng, scale,
+                           limiter);
         HIPCHK(hipGetLastError());
         unbind_solution(c, phi, staged);
         if (!dev_out) {
@@ -3013,20 +3059,29 @@ void multidot(mps_ctx *c, GmresBufs &G, const double *w, double *const *vs, int 
     std::memcpy(h, G.hh, nv * sizeof(double));
 }
 
-// w -= sum_q h[q] v_q; returns |w|^2
-double multiaxpy(mps_ctx *c, GmresBufs &G, double *w, double *const *vs, int nv, const double *h)
+// multidot with the products left on the device in out[0..nv)
+void multidot_dev(mps_ctx *c, GmresBufs &G, const double *w, double *const *vs, int nv, double *out)
 {
     Lev<double> v = view(c, c->L64, 0);
     VecList VL{};
     for (int q = 0; q < nv; ++q) VL.v[q] = vs[q];
-    std::memcpy(G.hh, h, nv * sizeof(double));
-    HIPCHK(hipMemcpyAsync(G.dh, G.hh, nv * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_multiaxpy, red_grid(c->geom[0]), block(), 0, c->stream, v, w, VL, (const double *)G.dh,
-                       nv, c->partials);
+    hipLaunchKernelGGL(k_multidot, red_grid(c->geom[0]), block(), 0, c->stream, v, w, VL, nv, G.dpart);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(nv), block(), 0, c->stream, (const double *)G.dpart, G.nblocks, out);
+}
+
+// w -= sum_q h[q] v_q with device coefficients h (the first min(nv, *nvd)
+// when nvd is given); |w|^2 into *norm2 (device) when norm2 is given
+void multiaxpy_dev(mps_ctx *c, double *w, double *const *vs, int nv, const double *h, const int *nvd,
+                   double *norm2)
+{
+    Lev<double> v = view(c, c->L64, 0);
+    VecList VL{};
+    for (int q = 0; q < nv; ++q) VL.v[q] = vs[q];
     dim3 gr = red_grid(c->geom[0]);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials,
-                       (int)(gr.x * gr.y), c->d_scalar);
-    return read_scalar(c);
+    hipLaunchKernelGGL(k_multiaxpy, gr, block(), 0, c->stream, v, w, VL, h, nv, norm2 ? c->partials : nullptr, nvd);
+    if (norm2)
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), block(), 0, c->stream, (const double *)c->partials,
+                           (int)(gr.x * gr.y), norm2);
 }
 
 // performgmres with the reference's preconditioner on the finest level (at
@@ -3066,9 +3121,36 @@ void free_gmres(GmresBufs &G)
     dfree(G.bg);
     dfree(G.dh);
     dfree(G.dpart);
+    dfree(G.darn);
+    dfree(G.dstate);
     if (G.hh) (void)hipHostFree(G.hh);
+    if (G.hflag) (void)hipHostFree(G.hflag);
+    for (auto e : G.ev) (void)hipEventDestroy(e);
     G = GmresBufs{};
 }
+
+// sizes of the device Arnoldi block (doubles) for m <= kMaxVec - 1
+constexpr int kArnLd = kMaxVec;                    // leading dimension of H (m + 1 <= kMaxVec)
+constexpr size_t kArnH = (size_t)kArnLd * kMaxVec; // H
+constexpr size_t kArnDoubles = kArnH + 7 * (size_t)kMaxVec + 1;
+
+ArnoldiDev arnoldi_view(GmresBufs &G)
+{
+    double *p = G.darn;
+    ArnoldiDev A;
+    A.H = p;
+    A.cs = p + kArnH;
+    A.sn = A.cs + kMaxVec;
+    A.gv = A.sn + kMaxVec;
+    A.y = A.gv + kMaxVec;
+    A.hn = A.y + kMaxVec;   // 1 double (padded to kMaxVec)
+    A.state = G.dstate;
+    return A;
+}
+// the second pass's products, the subtracted coefficients and |w|^2
+double *arn_h2(GmresBufs &G) { return G.darn + kArnH + 5 * (size_t)kMaxVec; }
+double *arn_ny(GmresBufs &G) { return G.darn + kArnH + 6 * (size_t)kMaxVec; }
+double *arn_wn2(GmresBufs &G) { return G.darn + kArnH + 7 * (size_t)kMaxVec; }
 
 // the context's FGMRES(m) vectors, (re)allocated when m grows. Every vector
 // starts with zero ghosts and the Krylov kernels write interiors only, so
@@ -3089,6 +3171,12 @@ GmresBufs &gmres_bufs(mps_ctx *c, int m)
     G.dh = dalloc<double>(kMaxVec);
     G.dpart = dalloc<double>((size_t)G.nblocks * kMaxVec);
     HIPCHK(hipHostMalloc((void **)&G.hh, kMaxVec * sizeof(double), hipHostMallocDefault));
+    G.darn = dalloc<double>(kArnDoubles);
+    G.dstate = dalloc<int>(4);
+    HIPCHK(hipHostMalloc((void **)&G.hflag, (kMaxVec + 4) * sizeof(int), hipHostMallocDefault));
+    G.ev.resize(kMaxVec);
+    for (auto &e : G.ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    fill<double>(c, G.darn, kArnDoubles, 0.0);
     for (auto p : G.V) fill<double>(c, p, g.ng(), 0.0);
     for (auto p : G.Z) fill<double>(c, p, g.ng(), 0.0);
     fill<double>(c, G.w, g.ng(), 0.0);
@@ -3097,12 +3185,11 @@ GmresBufs &gmres_bufs(mps_ctx *c, int m)
     return G;
 }
 
-void scale_into(mps_ctx *c, GmresBufs &G, const double *w, double *v, double s)
+// v = w / s[0] with s on the device (nothing when s[0] <= 0)
+void scale_from(mps_ctx *c, const double *w, double *v, const double *s)
 {
-    G.hh[0] = s;
-    HIPCHK(hipMemcpyAsync(G.dh, G.hh, sizeof(double), hipMemcpyHostToDevice, c->stream));
     Lev<double> vw = view(c, c->L64, 0);
-    hipLaunchKernelGGL(k_scale, cells_grid(c->geom[0]), block(), 0, c->stream, vw, w, v, (const double *)G.dh);
+    hipLaunchKernelGGL(k_scale, cells_grid(c->geom[0]), block(), 0, c->stream, vw, w, v, s);
 }
 
 }  // namespace
@@ -3166,69 +3253,58 @@ int mps_gmres(mps_ctx *c, const double *b, double *x, int m, int max_restarts, d
             beta = 0.0;
             status = MPS_OK;
         }
-        std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), gv(m + 1, 0.0), y(m, 0.0),
-            h(m + 1, 0.0), h2(m + 1, 0.0);
-#define HH(i_, j_) H[(size_t)(i_) + (size_t)(m + 1) * (j_)]
+        // The Gram-Schmidt products stay on the device, k_arnoldi_step
+        // updates H and the Givens rotations there and marks the restart
+        // done; per Arnoldi step the host waits once, for that 4-byte flag
+        // (copied to pinned memory behind an event), before it queues the
+        // next step, and reads the residual once per restart. (Checking the
+        // previous step's flag instead, so that the next step is always
+        // queued, costs an extra preconditioned step per solve, and the
+        // coupled solves converge in 1-3 steps.)
+        ArnoldiDev A = arnoldi_view(G);
+        double *dh2 = arn_h2(G), *dny = arn_ny(G), *dwn2 = arn_wn2(G);
+        const double tolb = tol * bnorm;
         for (int rs = 0; rs < max_restarts && status != MPS_OK; ++rs) {
             if (!std::isfinite(beta)) {
                 status = MPS_ERR_NAN;
                 break;
             }
-            if (beta <= tol * bnorm) {
+            if (beta <= tolb) {
                 status = MPS_OK;
                 break;
             }
-            std::fill(H.begin(), H.end(), 0.0);
-            std::fill(gv.begin(), gv.end(), 0.0);
-            gv[0] = beta;
-            scale_into(c, G, G.w, G.V[0], beta);
-            int kk = 0;
+            // c->d_scalar still holds |b - A x|^2 from true_residual
+            hipLaunchKernelGGL(k_arnoldi_begin, dim3(1), dim3(64), 0, c->stream, A, (const double *)c->d_scalar);
+            scale_from(c, G.w, G.V[0], A.gv);
+            int queued = 0;
             for (int j = 0; j < m; ++j) {
                 precondition(c, G.V[j], G.Z[j]);
                 matvec(c, c->L64, G.Z[j], G.w);
                 // classical Gram-Schmidt with one re-orthogonalisation pass
-                multidot(c, G, G.w, G.V.data(), j + 1, h.data());
-                multiaxpy(c, G, G.w, G.V.data(), j + 1, h.data());
-                multidot(c, G, G.w, G.V.data(), j + 1, h2.data());
-                double wn2 = multiaxpy(c, G, G.w, G.V.data(), j + 1, h2.data());
-                for (int i = 0; i <= j; ++i) HH(i, j) = h[i] + h2[i];
-                const double hn = std::sqrt(wn2);
-                HH(j + 1, j) = hn;
-                if (!std::isfinite(hn)) {
-                    status = MPS_ERR_NAN;
-                    break;
-                }
-                if (hn > 0.0) scale_into(c, G, G.w, G.V[j + 1], hn);
-                for (int i = 0; i < j; ++i) {
-                    const double t = cs[i] * HH(i, j) + sn[i] * HH(i + 1, j);
-                    HH(i + 1, j) = -sn[i] * HH(i, j) + cs[i] * HH(i + 1, j);
-                    HH(i, j) = t;
-                }
-                const double rr = HH(j, j), ss = HH(j + 1, j);
-                const double den = std::hypot(rr, ss);
-                cs[j] = den == 0.0 ? 1.0 : rr / den;
-                sn[j] = den == 0.0 ? 0.0 : ss / den;
-                HH(j, j) = cs[j] * rr + sn[j] * ss;
-                HH(j + 1, j) = 0.0;
-                gv[j + 1] = -sn[j] * gv[j];
-                gv[j] = cs[j] * gv[j];
-                ++total;
-                kk = j + 1;
-                if (std::fabs(gv[j + 1]) <= tol * bnorm || hn <= 1e-300) break;
+                multidot_dev(c, G, G.w, G.V.data(), j + 1, G.dh);
+                multiaxpy_dev(c, G.w, G.V.data(), j + 1, G.dh, nullptr, nullptr);
+                multidot_dev(c, G, G.w, G.V.data(), j + 1, dh2);
+                multiaxpy_dev(c, G.w, G.V.data(), j + 1, dh2, nullptr, dwn2);
+                hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, c->stream, A, j, kArnLd,
+                                   (const double *)G.dh, (const double *)dh2, (const double *)dwn2, tolb);
+                scale_from(c, G.w, G.V[j + 1], A.hn);
+                HIPCHK(hipMemcpyAsync(G.hflag + j, A.state, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipEventRecord(G.ev[j], c->stream));
+                queued = j + 1;
+                HIPCHK(hipEventSynchronize(G.ev[j]));
+                if (G.hflag[j]) break;
             }
-            if (status == MPS_ERR_NAN) break;
-            for (int i = kk - 1; i >= 0; --i) {
-                double s = gv[i];
-                for (int jj = i + 1; jj < kk; ++jj) s -= HH(i, jj) * y[jj];
-                y[i] = s / HH(i, i);
-            }
-            std::vector<double> ny(kk);
-            for (int i = 0; i < kk; ++i) ny[i] = -y[i];
-            multiaxpy(c, G, u, G.Z.data(), kk, ny.data());
+            hipLaunchKernelGGL(k_arnoldi_solve, dim3(1), dim3(64), 0, c->stream, A, kArnLd, dny);
+            multiaxpy_dev(c, u, G.Z.data(), queued, dny, A.state + 1, nullptr);
+            HIPCHK(hipMemcpyAsync(G.hflag + kMaxVec, A.state, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
             beta = true_residual();
-            if (beta <= tol * bnorm) status = MPS_OK;
+            if (G.hflag[kMaxVec + 2]) {
+                status = MPS_ERR_NAN;
+                break;
+            }
+            total += G.hflag[kMaxVec + 1];
+            if (beta <= tolb) status = MPS_OK;
         }
-#undef HH
         if (iters) *iters = total;
         if (relres) *relres = bnorm > 0 ? beta / bnorm : beta;
         unbind_solution(c, x, staged);
@@ -3496,14 +3572,42 @@ int time_kernel_t(mps_ctx *c, std::vector<LevelBuf<T>> &L, int kernel, int reps,
     } else {
         return fail(MPS_ERR_ARG, "unknown kernel %d", kernel);
     }
-    launch();  // warm
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
-    for (int q = 0; q < reps; ++q) launch();
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
-    HIPCHK(hipEventSynchronize(c->ev1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    *avg_ms = ms / reps;
+    // Each timed launch starts with cold caches: 1 GiB read in between (four
+    // times the 256 MB Infinity Cache) evicts what the previous launch left
+    // (its dirty lines are written back during the read, not during the
+    // next timed launch), so a kernel whose working set would fit the cache
+    // still reads it from HBM, as it does inside a solve. Launches are timed
+    // one by one.
+    constexpr long long kEvict4 = (1LL << 30) / 16;
+    const long long per_block = (long long)kStreamBlock * kStreamUnroll;
+    const unsigned nbe = (unsigned)((kEvict4 + per_block - 1) / per_block);
+    float4 *evict = dalloc<float4>((size_t)kEvict4);
+    double *epart = nullptr;
+    double total = 0.0;
+    try {
+        epart = dalloc<double>(nbe);
+        HIPCHK(hipMemsetAsync(evict, 0, (size_t)kEvict4 * 16, c->stream));
+        launch();  // warm
+        for (int q = 0; q < reps; ++q) {
+            hipLaunchKernelGGL(k_stream<0>, dim3(nbe), dim3(kStreamBlock), 0, c->stream, (const float4 *)evict,
+                               (const float4 *)nullptr, (const float4 *)nullptr, (float4 *)nullptr, kEvict4, 0.f,
+                               epart);
+            HIPCHK(hipEventRecord(c->ev0, c->stream));
+            launch();
+            HIPCHK(hipEventRecord(c->ev1, c->stream));
+            HIPCHK(hipEventSynchronize(c->ev1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            total += ms;
+        }
+    } catch (...) {
+        dfree(evict);
+        dfree(epart);
+        throw;
+    }
+    dfree(evict);
+    dfree(epart);
+    *avg_ms = total / reps;
     if (bind64) {
         L[0].u = u;
         L[0].b = b;
@@ -3530,6 +3634,46 @@ extern "C" int mps_time_kernel(mps_ctx *c, int kernel, int fp32, int reps, doubl
         }
         return st;
     } catch (const HipError &e) {
+        return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
+    }
+}
+
+extern "C" int mps_time_in_solve(mps_ctx *c, int kernel, int fp32, double *soln, int *ncycles, double *avg_ms,
+                                 int *launches)
+{
+    int st = guard_mg(c);
+    if (st) return st;
+    if (kernel < 0 || kernel > 3 || !soln || !avg_ms || !launches) return fail(MPS_ERR_ARG, "bad arguments");
+    if (fp32 && c->opts.precision != MPS_PREC_FP32_DC) return fail(MPS_ERR_ARG, "no FP32 hierarchy");
+    try {
+        if (c->kt_ev.empty()) {
+            c->kt_ev.resize(2 * kKtMax);
+            for (auto &e : c->kt_ev) HIPCHK(hipEventCreate(&e));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->kt_kernel = kernel;
+        c->kt_fp32 = fp32 != 0;
+        c->kt_n = 0;
+        // the solve's launches one by one (no graph replay), so that each
+        // timed launch is bracketed by its own events on the stream
+        const bool graphs = c->graphs;
+        c->graphs = false;
+        st = mps_solve(c, soln, ncycles, nullptr, nullptr);
+        c->graphs = graphs;
+        c->kt_kernel = -1;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (st != MPS_OK && st != MPS_NOT_CONVERGED) return st;
+        double total = 0.0;
+        for (int q = 0; q < c->kt_n; ++q) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->kt_ev[2 * q], c->kt_ev[2 * q + 1]));
+            total += ms;
+        }
+        *launches = c->kt_n;
+        *avg_ms = c->kt_n ? total / c->kt_n : 0.0;
+        return st;
+    } catch (const HipError &e) {
+        c->kt_kernel = -1;
         return fail(MPS_ERR_HIP, "%s: %s", e.what, hipGetErrorString(e.e));
     }
 }

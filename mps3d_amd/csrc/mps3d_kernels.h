@@ -46,6 +46,59 @@ __device__ __forceinline__ void xcd_tile(int &bx, int &by, int &bz)
 }
 
 // ------------------------------------------------------------------------
+// raw buffer access
+// ------------------------------------------------------------------------
+// A buffer resource (base, size in bytes) lets a load or store take a
+// per-lane 32-bit byte offset in a vector register and a uniform one in a
+// scalar register, with no 64-bit address arithmetic per access. The
+// hardware range-checks raw buffers: an access whose offset lies at or
+// beyond the resource's size returns 0 (loads) or is dropped (stores), so a
+// per-lane offset of kOob marks an invalid element without a select.
+// Resources here are built over at most 2^31 - 1 bytes, so every valid
+// offset is below kOob.
+constexpr int kBufferWord3 = 0x00020000;  // raw buffer descriptor word 3 (gfx9 family)
+constexpr int kOob = (int)0x80000000u;     // per-lane offset of an invalid element
+
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const T *p, int nbytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, nbytes, kBufferWord3);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void buf_store(T v, __amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ void buf_store<float>(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+template <>
+__device__ __forceinline__ void buf_store<double>(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
+// per-face multipliers of a staged ghost coefficient (fast sweeps): the
+// ghost cell's D is staged as m_f times the adjacent interior cell's, so a
+// face weight (D_c + D_ghost) h_f carries the boundary condition
+template <typename T>
+struct GhostD {
+    T m[6];
+};
+
+// ------------------------------------------------------------------------
 // cell addressing
 // ------------------------------------------------------------------------
 template <typename T>
@@ -881,12 +934,24 @@ __device__ __forceinline__ double minmod_phi(double ntr, double dtr)
     return 1.0;
 }
 
+//
+// One thread per ghosted cell, so every row is written whole: an interior
+// cell gets its own value; a face ghost cell (one index outside the
+// interior) the value of the interior cell next to it, which is the
+// zero-gradient ghost rule update_efield applies to each component
+// (efield_bc_codes = 'ZGRD', :580-583, 670-678); edge and corner ghosts are
+// not written (the ghost rule sets faces only).
 __global__ void __launch_bounds__(kBlock) k_efield(Lev<double> L, const double *__restrict__ phi, double *ex,
                                                   double *ey, double *ez, double scale, int limiter)
 {
-    int i, j, k;
-    long long c;
-    if (!cell_of(L, i, j, k, c)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int W = L.nx + 2;
+    if (t >= W * (L.ny + 2)) return;
+    const int j = t / W, i = t - j * W, k = blockIdx.y;
+    const int ci = min(max(i, 1), L.nx), cj = min(max(j, 1), L.ny), ck = min(max(k, 1), L.nz);
+    if ((ci != i) + (cj != j) + (ck != k) > 1) return;
+    const long long o = (long long)t + L.sz * k;
+    const long long c = (long long)ci + (long long)L.sy * cj + L.sz * ck;
     const long long st[3] = {1, L.sy, L.sz};
     const double d[3] = {L.dx, L.dy, L.dz};
     double *E[3] = {ex, ey, ez};
@@ -895,7 +960,7 @@ __global__ void __launch_bounds__(kBlock) k_efield(Lev<double> L, const double *
         const double a = phi[c + st[q]], b = phi[c - st[q]], p0 = phi[c];
         double e = 0.5 * (a - b) / d[q];
         if (limiter) e = minmod_phi(p0 - b, a - p0) * e;
-        E[q][c] = -e * scale;
+        E[q][o] = -e * scale;
     }
 }
 
@@ -1095,12 +1160,14 @@ __global__ void __launch_bounds__(kBlock) k_reduce_cols(const double *partials, 
     if (threadIdx.x == 0) out[q] = s;
 }
 
-// w = w - sum_q h[q] v_q over the interior; block partial of |w|^2
+// w = w - sum_q h[q] v_q over the interior (q < nv, and q < *nvd when nvd
+// is given); block partial of |w|^2
 __global__ void __launch_bounds__(kBlock) k_multiaxpy(Lev<double> L, double *w, VecList V, const double *h, int nv,
-                                                     double *partials)
+                                                     double *partials, const int *nvd = nullptr)
 {
     int i, j;
     double sq = 0.0;
+    if (nvd) nv = min(nv, *nvd);
     if (cell_ij(L, i, j)) {
         for (int k = blockIdx.y + 1; k <= L.nz; k += gridDim.y) {
             const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
@@ -1128,13 +1195,92 @@ __global__ void __launch_bounds__(kBlock) k_matvec(Lev<T> L, const T *__restrict
     out[c] = dg * u[c] + of;
 }
 
-// v = w * (1/s[0]) over the interior (ghosts of v stay zero)
+// v = w / s[0] over the interior (ghosts of v stay zero); nothing when s[0]
+// is not positive (an Arnoldi breakdown or NaN leaves v as it is)
 __global__ void __launch_bounds__(kBlock) k_scale(Lev<double> L, const double *w, double *v, const double *s)
 {
     int i, j, k;
     long long c;
     if (!cell_of(L, i, j, k, c)) return;
-    v[c] = w[c] / s[0];
+    const double d = s[0];
+    if (!(d > 0.0)) return;
+    v[c] = w[c] / d;
+}
+
+// ------------------------------------------------------------------------
+// FGMRES(m) Hessenberg and Givens on the device (one lane): the host queues
+// the Arnoldi steps and reads the residual once per restart.
+// ------------------------------------------------------------------------
+struct ArnoldiDev {
+    double *H;      // (m + 1) x m Hessenberg, column-major, leading dimension ld
+    double *cs, *sn, *gv, *y;
+    double *hn;     // |w| of the last step (the next basis vector's scale)
+    int *state;     // [0] done, [1] columns kk, [2] 1 after a NaN or Inf
+};
+
+// start of a restart: gv = (beta, 0, ...), beta = sqrt(*beta2)
+__global__ void k_arnoldi_begin(ArnoldiDev A, const double *beta2)
+{
+    if (threadIdx.x | blockIdx.x) return;
+    A.state[0] = 0;
+    A.state[1] = 0;
+    A.state[2] = 0;
+    A.gv[0] = sqrt(beta2[0]);
+}
+
+// Arnoldi step j after both Gram-Schmidt passes: column j of H is h + h2,
+// H(j+1, j) = |w| = sqrt(*wn2); the previous rotations and a new one are
+// applied, and the step marks the restart done when |gv(j+1)| <= tolb or
+// |w| <= 1e-300 (breakdown), or flags a NaN. Nothing once done (steps
+// queued past the end of a restart leave H and gv as they are).
+__global__ void k_arnoldi_step(ArnoldiDev A, int j, int ld, const double *h, const double *h2, const double *wn2,
+                               double tolb)
+{
+    if (threadIdx.x | blockIdx.x) return;
+    if (A.state[0]) return;
+    double *Hj = A.H + (long long)ld * j;
+    for (int i = 0; i <= j; ++i) Hj[i] = h[i] + h2[i];
+    const double hn = sqrt(wn2[0]);
+    Hj[j + 1] = hn;
+    A.hn[0] = hn;
+    if (!isfinite(hn)) {
+        A.state[2] = 1;
+        A.state[0] = 1;
+        return;
+    }
+    for (int i = 0; i < j; ++i) {
+        const double t = A.cs[i] * Hj[i] + A.sn[i] * Hj[i + 1];
+        Hj[i + 1] = -A.sn[i] * Hj[i] + A.cs[i] * Hj[i + 1];
+        Hj[i] = t;
+    }
+    const double rr = Hj[j], ss = Hj[j + 1];
+    const double den = hypot(rr, ss);
+    A.cs[j] = den == 0.0 ? 1.0 : rr / den;
+    A.sn[j] = den == 0.0 ? 0.0 : ss / den;
+    Hj[j] = A.cs[j] * rr + A.sn[j] * ss;
+    Hj[j + 1] = 0.0;
+    A.gv[j + 1] = -A.sn[j] * A.gv[j];
+    A.gv[j] = A.cs[j] * A.gv[j];
+    A.state[1] = j + 1;
+    if (fabs(A.gv[j + 1]) <= tolb || hn <= 1e-300) A.state[0] = 1;
+}
+
+// y = H(0:kk, 0:kk)^-1 gv by back substitution; ny = -y (the coefficients
+// k_multiaxpy subtracts); nothing after a NaN
+__global__ void k_arnoldi_solve(ArnoldiDev A, int ld, double *ny)
+{
+    if (threadIdx.x | blockIdx.x) return;
+    if (A.state[2]) {
+        A.state[1] = 0;
+        return;
+    }
+    const int kk = A.state[1];
+    for (int i = kk - 1; i >= 0; --i) {
+        double s = A.gv[i];
+        for (int jj = i + 1; jj < kk; ++jj) s -= A.H[i + (long long)ld * jj] * A.y[jj];
+        A.y[i] = s / A.H[i + (long long)ld * i];
+    }
+    for (int i = 0; i < kk; ++i) ny[i] = -A.y[i];
 }
 
 }  // namespace mps

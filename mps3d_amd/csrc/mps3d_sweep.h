@@ -240,19 +240,31 @@ using SlotC = std::integral_constant<int, I>;
 // (zero on the physical boundary) and their DIRC wall factors, so a row is
 // w_f = (D_c + D_nb) h_f,  diag = sum w_f + D_c g + 1/dt - reac,
 // u = (b + sum w_f u_nb) / diag.  Values outside the interior are staged as
-// zeros, so masked faces multiply zeros. LDS holds three u planes and five
-// D planes of the tile with a 2-cell halo and three planes of red results;
-// each lane's own b values travel in registers. Ring depths: step s stores
-// u/D plane s+1 before its first barrier, and its red pass writes red plane
-// s after it. The u slot it reuses (plane s-2) was last read by the red pass
-// of step s-1 and the red slot (plane s-3) by the black pass of step s-1,
-// both behind a barrier; the D slot it reuses was last read by the black
-// pass of step s-1 (planes s-3..s-1) with no barrier in between, hence five
-// D planes. Global loads run PD steps ahead in two alternating register
-// sets.
-template <typename T, int OP, bool REAC, int TX, int TY, int PD, bool ZG>
-__global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__restrict__ uin, T *__restrict__ uout,
-                                                      const T *__restrict__ b, int kz)
+// zeros, so masked faces multiply zeros. LDS holds rings of four u planes
+// and eight D planes of the tile with a 2-cell halo and four planes of red
+// results; each lane's own b values travel in registers. Ring depths: step
+// s stores u/D plane s+1 before its first barrier, and its red pass writes
+// red plane s after it. The u slot it reuses (plane s-3) was last read by
+// the red pass of step s-2 and the red slot (plane s-4) by the black pass
+// of step s-2, both behind a barrier; the D slot (plane s-7) was last read
+// by the black pass of step s-6 (the black pass of step s-1, which follows
+// no barrier before step s's store, reads D planes s-3..s-1, so the D ring
+// needs at least five). The depths are powers of two and the march is
+// unrolled by eight, so every ring slot is a compile-time constant and
+// every LDS access a per-lane base plus an immediate offset. Global loads
+// and stores are raw buffer accesses with per-lane offsets fixed for the
+// launch (out-of-interior positions load 0 and drop their stores) and run
+// one step ahead in two alternating register sets.
+// (MPS_SWEEP_WAVES, a build-time tuning knob, sets the waves per SIMD the
+// register allocation aims at)
+#ifdef MPS_SWEEP_WAVES
+#define MPS_SWEEP_OCC __attribute__((amdgpu_waves_per_eu(MPS_SWEEP_WAVES, MPS_SWEEP_WAVES)))
+#else
+#define MPS_SWEEP_OCC
+#endif
+template <typename T, int OP, bool REAC, int TX, int TY, bool ZG>
+__global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, const T *__restrict__ uin, T *__restrict__ uout,
+                                                      const T *__restrict__ b, int kz, T zg0, T zg1)
 {
 #pragma clang fp contract(fast)
     constexpr int W2 = TX + 4, H2 = TY + 4, P2 = W2 * H2;
@@ -266,12 +278,19 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     constexpr int NB = BW * TY;  // black lanes
     static_assert(TX % 2 == 0, "tile width must be even");
     static_assert(NR <= NT, "one red cell per lane");
-    constexpr int DR = 5;
+    constexpr int RU = 4, RD = 8;             // ring depths (powers of two: slots are t & (R - 1))
     constexpr bool STN = OP == OP_STENCIL;  // explicit 7-point rows (no D tiles)
-    __shared__ T su[3][P2];
-    __shared__ T sD[STN ? 1 : DR][STN ? 1 : P2];
-    __shared__ T sq[REAC ? 4 : 1][REAC ? P1 : 1];
-    __shared__ T sr[3][P1];
+    // ZGHOST (OP_REF rows): the z faces of planes 1 and nz take the interior
+    // factor hz like every other plane, and the staged D of the ghost plane
+    // carries the boundary instead: zg * D of the adjacent interior plane,
+    // with zg = -1 on a no-flux face (the face weight (D_c + D_g) hz is 0)
+    // and gz/hz - 1 on a DIRC face (the weight is D_c gz, the wall term of
+    // the diagonal); the ghost plane of u is staged as 0.
+    constexpr bool ZGHOST = OP == OP_REF;
+    __shared__ T su[RU][P2];
+    __shared__ T sD[STN ? 1 : RD][STN ? 1 : P2];
+    __shared__ T sq[REAC ? RU : 1][REAC ? P1 : 1];
+    __shared__ T sr[RU][P1];
 
     const int tid = threadIdx.x;
     int bx, by, bz;
@@ -283,27 +302,43 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     const long long sz = L.sz;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
 
-    // staging offsets (interior positions only; -1 elsewhere). Loads are
-    // issued for every lane from a valid address (the offset clamped to 0,
-    // the plane clamped to the ghost planes) and the value selected away,
-    // so the staging has no divergent branches.
-    int o2[N2], o1[N1];
+    // Every global access goes through buffer resources over the planes
+    // this block touches, pl0..pl1 (its chunk and two planes either side,
+    // clamped to the ghost planes; the host keeps that slab below 2^31
+    // bytes): a per-lane in-plane byte offset fixed for the whole march
+    // (kOob outside the interior: the load returns 0, the store is dropped)
+    // and a uniform plane offset, so loads and stores need no address
+    // arithmetic or selects per step.
+    const int pl0 = max(kb - 2, 0), pl1 = min(ke + 2, nz + 1);
+    const int szb = (int)sz * (int)sizeof(T);
+    const int nrec = (pl1 - pl0 + 1) * szb;
+    auto rsrc = [&](const T *p) { return buf_rsrc(p + (long long)pl0 * sz, nrec); };
+    const __amdgpu_buffer_rsrc_t Ru = rsrc(uin), Ro = rsrc(uout), Rb = rsrc(b);
+    const __amdgpu_buffer_rsrc_t Rd = rsrc(STN ? b : L.D), Rq = rsrc(REAC ? L.reac : b);
+    __amdgpu_buffer_rsrc_t Rs[STN ? ST_N : 1];
+#pragma unroll
+    for (int q = 0; q < (STN ? ST_N : 1); ++q) Rs[q] = rsrc(STN ? L.st[q] : b);
+    // byte offset of plane kp in the resources (kp clamped to pl0..pl1)
+    auto pofs = [&](int kp) { return (min(max(kp, pl0), pl1) - pl0) * szb; };
+
+    // staging offsets: in-plane bytes of interior positions, kOob elsewhere
+    int v2[N2], v1[N1];
 #pragma unroll
     for (int r = 0; r < N2; ++r) {
         const int q = tid + r * NT;
-        o2[r] = -1;
+        v2[r] = kOob;
         if (q < P2) {
             const int gi = i0 - 2 + q % W2, gj = j0 - 2 + q / W2;
-            if (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny) o2[r] = gi + sy * gj;
+            if (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny) v2[r] = (gi + sy * gj) * (int)sizeof(T);
         }
     }
 #pragma unroll
     for (int r = 0; r < N1; ++r) {
         const int q = tid + r * NT;
-        o1[r] = -1;
+        v1[r] = kOob;
         if (REAC && q < P1) {
             const int gi = i0 - 1 + q % W1, gj = j0 - 1 + q / W1;
-            if (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny) o1[r] = gi + sy * gj;
+            if (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny) v1[r] = (gi + sy * gj) * (int)sizeof(T);
         }
     }
 
@@ -386,16 +421,17 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     // (rpar + kb - 1 + y) & 1, its black cell (bpar + kb - 2 + y) & 1.
     struct Cell {
         int q, e, o;    // LDS index (ring plane), LDS index (halo-2 plane), field offset (-1: none)
-        bool tile;      // red cell inside this block's tile (written out)
-        int i;          // x index (black cells)
+        int vb, vs;     // in-plane bytes of the cell for loads and for the store (kOob: none)
         T fxm, fxp, g;  // x face factors and x/y DIRC factor
     };
+    auto vbytes = [](int o) { return o >= 0 ? o * (int)sizeof(T) : kOob; };
     auto red_cell = [&](int pc) {
-        return pc ? Cell{rq1, re1, ro1, rt1, i_r1, rfxm1, rfxp1, rgxy1} : Cell{rq0, re0, ro0, rt0, i_r0, rfxm0, rfxp0, rgxy0};
+        return pc ? Cell{rq1, re1, ro1, vbytes(ro1), rt1 ? vbytes(ro1) : kOob, rfxm1, rfxp1, rgxy1}
+                  : Cell{rq0, re0, ro0, vbytes(ro0), rt0 ? vbytes(ro0) : kOob, rfxm0, rfxp0, rgxy0};
     };
     auto blk_cell = [&](int pc) {
-        return pc ? Cell{bq1, be1, cb1, false, bi1, bfxm1, bfxp1, bgxy1}
-                  : Cell{bq0, be0, cb0, false, bi0, bfxm0, bfxp0, bgxy0};
+        return pc ? Cell{bq1, be1, cb1, vbytes(cb1), vbytes(cb1), bfxm1, bfxp1, bgxy1}
+                  : Cell{bq0, be0, cb0, vbytes(cb0), vbytes(cb0), bfxm0, bfxp0, bgxy0};
     };
     const int pr = (rpar + kb - 1) & 1, pbk = (bpar + kb - 2) & 1;
     const Cell R0 = red_cell(pr), R1 = red_cell(pr ^ 1);
@@ -404,49 +440,42 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
     // gload(kp, X): into register set X, the u and D tiles of plane kp (for
     // LDS), the red-cell b of plane kp - 1 and the black-cell b of plane
     // kp - 2: everything step kp - 1 consumes. Step s consumes gload(s + 1)
-    // and issues gload(s + 1 + PD), so loads have PD steps to arrive; the
-    // two sets alternate with the step parity (the loop is unrolled by two,
-    // so no set is copied while its loads are in flight). Plane offsets are
-    // 32-bit (the host keeps ghosted fields below 2^31 cells).
-    const int szi = (int)sz;
-    auto plane = [&](int kp) { return min(max(kp, 0), nz + 1) * szi; };
-    bool on2[N2], on1[N1];
-#pragma unroll
-    for (int r = 0; r < N2; ++r) on2[r] = o2[r] >= 0;
-#pragma unroll
-    for (int r = 0; r < N1; ++r) on1[r] = o1[r] >= 0;
-    // raw loads; the validity flags travel beside them and are applied
-    // where the values are consumed (a select right after a load would wait
-    // for it and void the prefetch)
+    // and issues gload(s + 2), so loads have a step to arrive; the two sets
+    // alternate with the step parity (the march is unrolled, so no set is
+    // copied while its loads are in flight). Positions outside
+    // the interior load 0; b of a cell outside the interior or of a plane
+    // outside 1..nz is never consumed.
     T pu[2][N2], pD[2][N2], pq[2][N1], nbr0, nbr1, nbb0, nbb1;
     // stencil rows: the 7 coefficients of the slot's red and black cell
     T rs0[STN ? ST_N : 1], rs1[STN ? ST_N : 1], bs0[STN ? ST_N : 1], bs1[STN ? ST_N : 1];
-    bool okp0 = false, okp1 = false, rok0 = false, rok1 = false, bok0 = false, bok1 = false;
+    bool okp0 = false, okp1 = false;
+    T dm0 = T(0), dm1 = T(0);
     auto gload = [&](int kp, auto X) {
         constexpr int x = decltype(X)::value;
-        const int pk = plane(kp);
+        const int pk = pofs(kp);
+        // OP_REF: the z ghost planes of D are staged as zg0 / zg1 times the
+        // first / last interior plane (see ZGHOST below)
+        const int pkd = ZGHOST ? pofs(min(max(kp, 1), nz)) : pk;
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
-            const int o = pk + (on2[r] ? o2[r] : 0);
-            pu[x][r] = ZG ? T(0) : uin[o];  // ZG: sweep from a zero guess (uin unread)
-            if (!STN) pD[x][r] = L.D[o];
+            pu[x][r] = ZG ? T(0) : buf_load<T>(Ru, v2[r], pk);  // ZG: sweep from a zero guess (uin unread)
+            if (!STN) pD[x][r] = buf_load<T>(Rd, v2[r], pkd);
         }
         if (REAC && !STN) {
 #pragma unroll
-            for (int r = 0; r < N1; ++r) pq[x][r] = L.reac[pk + (on1[r] ? o1[r] : 0)];
+            for (int r = 0; r < N1; ++r) pq[x][r] = buf_load<T>(Rq, v1[r], pk);
         }
         // (gload(kp) fills set (kp - kb) & 1, the slot of step kp - 1 that
         // consumes it; outside that pattern the b values are not consumed)
-        const int kr = kp - 1, kk = kp - 2;
-        const int orr = x ? R1.o : R0.o;
-        const int obb = x ? B1.o : B0.o;
-        const int ar = plane(kr) + (orr >= 0 ? orr : 0), ab = plane(kk) + (obb >= 0 ? obb : 0);
-        const T vr = b[ar];
-        const T vb = b[ab];
+        const int pr = pofs(kp - 1), pb = pofs(kp - 2);
+        const int vr_ = x ? R1.vb : R0.vb;
+        const int vb_ = x ? B1.vb : B0.vb;
+        const T vr = buf_load<T>(Rb, vr_, pr);
+        const T vb = buf_load<T>(Rb, vb_, pb);
         if constexpr (STN) {
 #pragma unroll
             for (int q = 0; q < ST_N; ++q) {
-                const T sr_ = L.st[q][ar], sb_ = L.st[q][ab];
+                const T sr_ = buf_load<T>(Rs[q], vr_, pr), sb_ = buf_load<T>(Rs[q], vb_, pb);
                 if (x) {
                     rs1[q] = sr_;
                     bs1[q] = sb_;
@@ -457,32 +486,31 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
             }
         }
         const bool ok = kp >= 1 && kp <= nz;
-        const bool rok = kr >= 1 && kr <= nz && orr >= 0;
-        const bool bok = kk >= 1 && kk <= nz && obb >= 0;
+        const T dm = ok ? T(1) : (ZGHOST && kp == 0) ? zg0 : (ZGHOST && kp == nz + 1) ? zg1 : T(0);
         if (x) {
-            nbr1 = vr, nbb1 = vb, okp1 = ok, rok1 = rok, bok1 = bok;
+            nbr1 = vr, nbb1 = vb, okp1 = ok, dm1 = dm;
         } else {
-            nbr0 = vr, nbb0 = vb, okp0 = ok, rok0 = rok, bok0 = bok;
+            nbr0 = vr, nbb0 = vb, okp0 = ok, dm0 = dm;
         }
     };
     // stores plane kp (u into ring slot s3, D into s5, reac into s4)
     auto lstore = [&](int s3, int s5, int s4, auto X) {
         constexpr int x = decltype(X)::value;
         const bool ok = x ? okp1 : okp0;
+        const T dm = x ? dm1 : dm0;
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
             const int q = tid + r * NT;
             if (r * NT + NT <= P2 || q < P2) {
-                const bool on = ok && on2[r];
-                su[s3][q] = on ? pu[x][r] : T(0);
-                if (!STN) sD[s5][q] = on ? pD[x][r] : T(0);
+                su[s3][q] = ok ? pu[x][r] : T(0);
+                if (!STN) sD[s5][q] = pD[x][r] * dm;
             }
         }
         if (REAC && !STN) {
 #pragma unroll
             for (int r = 0; r < N1; ++r) {
                 const int q = tid + r * NT;
-                if (q < P1) sq[s4][q] = (ok && on1[r]) ? pq[x][r] : T(0);
+                if (q < P1) sq[s4][q] = ok ? pq[x][r] : T(0);
             }
         }
     };
@@ -516,12 +544,6 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
         o = o + st[ST_ZP] * uzp;
         return gs_update(L, bv, o, st[ST_C]);
     };
-    // Ring slots advance by one per step: u3/d5/q4 are the slots of plane s
-    // in the u and red rings (3), the D ring (5) and the reac ring (4).
-    auto next3 = [](int a) { return a == 2 ? 0 : a + 1; };
-    auto prev3 = [](int a) { return a == 0 ? 2 : a - 1; };
-    auto next5 = [](int a) { return a == 4 ? 0 : a + 1; };
-    auto prev5 = [](int a) { return a == 0 ? 4 : a - 1; };
     // z face factors of interior planes; planes 1 and nz get their own
     T fzm_i, fzp_i, gz_i;
     z_factors(2, fzm_i, fzp_i, gz_i);
@@ -533,33 +555,43 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
             fzm = fzm_i, fzp = fzp_i, gz = gz_i;
         }
     };
-    // slots of plane kb - 2 (the first one staged)
-    int u3 = 0, d5 = 0, q4 = 0;
+    // Ring slots are compile-time: the march is unrolled by eight steps and
+    // the rings hold RU = 4 planes (u, red results, reac) and RD = 8 (D), so
+    // step t of a group (plane s, t = (s - kb + 1) mod 8) finds plane s in
+    // u/red/reac slot (t + 1) & 3 and D slot (t + 1) & 7, and every LDS
+    // access is a per-lane base plus an immediate offset. Plane kb - 2 is
+    // staged into slots 0, plane kb - 1 into slots 1.
     gload(kb - 2, SlotC<0>{});
-    lstore(u3, d5, q4, SlotC<0>{});
+    lstore(0, 0, 0, SlotC<0>{});
     gload(kb - 1, SlotC<0>{});
-    lstore(next3(u3), next5(d5), (q4 + 1) & 3, SlotC<0>{});
-    // now plane s = kb - 1 has slots next3(0), next5(0), 1
-    u3 = 1, d5 = 1, q4 = 1;
+    lstore(1, 1, 1, SlotC<0>{});
     // set of gload(kp): (kp - kb) & 1
     gload(kb, SlotC<0>{});
-    if (PD == 2) gload(kb + 1, SlotC<1>{});
-    auto step = [&](int s, auto X) {
-        constexpr int x = decltype(X)::value;
-        const int u3p = next3(u3), u3m = prev3(u3);
-        const int d5p = next5(d5), d5m = prev5(d5), d5mm = prev5(d5m);
-        lstore(u3p, d5p, (q4 + 1) & 3, X);
-        const T br = (x ? rok1 : rok0) ? (x ? nbr1 : nbr0) : T(0);
-        const T bk = (x ? bok1 : bok0) ? (x ? nbb1 : nbb0) : T(0);
+    auto step = [&](int s, auto Tc) {
+        constexpr int t = decltype(Tc)::value;
+        constexpr int x = t & 1;
+        // u / red / reac slots of planes s + 1, s, s - 1, s - 2
+        constexpr int up_ = (t + 2) & (RU - 1), u0_ = (t + 1) & (RU - 1), um_ = t & (RU - 1),
+                      umm_ = (t + 3) & (RU - 1);
+        // D slots of planes s + 1, s, s - 1, s - 2
+        constexpr int dp_ = (t + 2) & (RD - 1), d0_ = (t + 1) & (RD - 1), dm_ = t & (RD - 1),
+                      dmm_ = (t + 7) & (RD - 1);
+        lstore(up_, dp_, up_, SlotC<x>{});
+        const T br = x ? nbr1 : nbr0;
+        const T bk = x ? nbb1 : nbb0;
+        // z face factors of the red plane s and the black plane s - 1
+        // (uniform: formed outside the lanes' branches; with ZGHOST the
+        // interior ones on every plane)
+        T rfzm = L.hz, rfzp = L.hz, rgz = T(0), bfzm = L.hz, bfzp = L.hz, bgz = T(0);
+        if constexpr (!ZGHOST) {
+            zf(s, rfzm, rfzp, rgz);
+            zf(s - 1, bfzm, bfzp, bgz);
+        }
         __syncthreads();
         // the prefetch is issued on every step (past the last plane this
         // block needs it re-reads plane ke + 2, an L2 hit), so the loop body
         // is straight-line and the compiler keeps the register sets in place
-        if (PD == 2) {
-            gload(min(s + 3, ke + 2), X);
-        } else {
-            gload(min(s + 2, ke + 2), SlotC<1 - x>{});
-        }
+        gload(min(s + 2, ke + 2), SlotC<1 - x>{});
         // red cells of plane s; red cells outside the interior (or planes
         // outside 1..nz) hold 0 in the red ring, so the black pass reads its
         // neighbours without masking them (their face factors are zero)
@@ -567,53 +599,51 @@ __global__ void __launch_bounds__(TX *TY) k_sweep_fast(Lev<T> L, const T *__rest
             const Cell &C = x ? R1 : R0;
             T val = T(0);
             if (s >= 1 && s <= nz && C.o >= 0) {
-                T fzm, fzp, gz;
-                zf(s, fzm, fzp, gz);
                 const int e = C.e;
-                const T *um = su[u3m], *u0 = su[u3], *up = su[u3p];
+                const T *um = su[um_], *u0 = su[u0_], *up = su[up_];
                 if constexpr (STN) {
                     val = stencil_row(x ? rs1 : rs0, u0[e - 1], u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], br);
                 } else {
-                    const T *Dm = sD[d5m], *D0 = sD[d5], *Dp = sD[d5p];
-                    const T rc = REAC ? sq[REAC ? q4 : 0][C.q] : T(0);
+                    const T *Dm = sD[dm_], *D0 = sD[d0_], *Dp = sD[dp_];
+                    const T rc = REAC ? sq[REAC ? u0_ : 0][C.q] : T(0);
                     val = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], u0[e - 1],
-                                   u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], C.fxm, C.fxp, rfym, rfyp, fzm, fzp,
-                                   C.g + gz, rc, br);
+                                   u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], C.fxm, C.fxp, rfym, rfyp, rfzm,
+                                   rfzp, ZGHOST ? C.g : C.g + rgz, rc, br);
                 }
-                if (C.tile && s >= kb && s <= ke) uout[C.o + szi * s] = val;
+                if (s >= kb && s <= ke) buf_store<T>(val, Ro, C.vs, pofs(s));
             }
-            sr[u3][C.q] = val;
+            sr[u0_][C.q] = val;
         }
         __syncthreads();
         const int k = s - 1;
         const Cell &Bc = x ? B1 : B0;
         if (k >= kb && k <= ke && Bc.o >= 0) {
             const int q = Bc.q, e = Bc.e;
-            T fzm, fzp, gz;
-            zf(k, fzm, fzp, gz);
-            // red planes k-1, k, k+1 = s-2, s-1, s: slots u3p, u3m, u3
-            const T *rm = sr[u3p], *r0 = sr[u3m], *rp = sr[u3];
+            // red planes k-1, k, k+1 = s-2, s-1, s
+            const T *rm = sr[umm_], *r0 = sr[um_], *rp = sr[u0_];
+            T v;
             if constexpr (STN) {
-                uout[Bc.o + szi * k] = stencil_row(x ? bs1 : bs0, r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q],
-                                                   rp[q], bk);
+                v = stencil_row(x ? bs1 : bs0, r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q], rp[q], bk);
             } else {
-                const T *Dm = sD[d5mm], *D0 = sD[d5m], *Dp = sD[d5];
-                const T rc = REAC ? sq[REAC ? ((q4 + 3) & 3) : 0][q] : T(0);
-                uout[Bc.o + szi * k] = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e],
-                                                r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q], rp[q], Bc.fxm,
-                                                Bc.fxp, bfym, bfyp, fzm, fzp, Bc.g + gz, rc, bk);
+                const T *Dm = sD[dmm_], *D0 = sD[dm_], *Dp = sD[d0_];
+                const T rc = REAC ? sq[REAC ? um_ : 0][q] : T(0);
+                v = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], r0[q - 1], r0[q + 1],
+                             r0[q - W1], r0[q + W1], rm[q], rp[q], Bc.fxm, Bc.fxp, bfym, bfyp, bfzm, bfzp,
+                             ZGHOST ? Bc.g : Bc.g + bgz, rc, bk);
             }
+            buf_store<T>(v, Ro, Bc.vs, pofs(k));
         }
-        u3 = u3p;
-        d5 = d5p;
-        q4 = (q4 + 1) & 3;
     };
-    int s = kb - 1;
-    for (; s + 1 <= ke + 1; s += 2) {
+    for (int s = kb - 1; s <= ke + 1; s += 8) {
         step(s, SlotC<0>{});
-        step(s + 1, SlotC<1>{});
+        if (s + 1 <= ke + 1) step(s + 1, SlotC<1>{});
+        if (s + 2 <= ke + 1) step(s + 2, SlotC<2>{});
+        if (s + 3 <= ke + 1) step(s + 3, SlotC<3>{});
+        if (s + 4 <= ke + 1) step(s + 4, SlotC<4>{});
+        if (s + 5 <= ke + 1) step(s + 5, SlotC<5>{});
+        if (s + 6 <= ke + 1) step(s + 6, SlotC<6>{});
+        if (s + 7 <= ke + 1) step(s + 7, SlotC<7>{});
     }
-    if (s <= ke + 1) step(s, SlotC<0>{});
 }
 
 }  // namespace mps

@@ -40,38 +40,6 @@ namespace mps {
 
 constexpr int kSweep2Threads = 512;
 
-template <typename T>
-struct GhostD {
-    T m[6];   // per-face multiplier of the staged ghost coefficient
-};
-
-constexpr int kBufferWord3 = 0x00020000;  // raw buffer descriptor word 3 (gfx9 family)
-
-template <typename T>
-__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
-template <>
-__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-template <>
-__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
-{
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-template <typename T>
-__device__ __forceinline__ void buf_store(T v, __amdgpu_buffer_rsrc_t r, int voff, int soff);
-template <>
-__device__ __forceinline__ void buf_store<float>(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
-}
-template <>
-__device__ __forceinline__ void buf_store<double>(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
-{
-    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
-}
 
 template <typename T, int TX, int TY, bool ZG>
 __global__ void __launch_bounds__(kSweep2Threads) k_sweep2_fast(Lev<T> L, GhostD<T> G, const T *__restrict__ uin,

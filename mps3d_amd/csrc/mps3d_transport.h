@@ -103,12 +103,18 @@ __host__ __device__ __forceinline__ double powi_sq(double x, int m)
 // efm, as getspecproduction takes it: getarrhrate (ion_elec_chem.f90:260-274)
 // A T^alpha exp(-Ea/T), or getcustomrate: the fit of O2_chem.f90:413-448,
 // C0 exp(c1/T + c2/T**2 + c3/T**3 + c4/T**4) (coefficients signed, the
-// terms summed left to right, T**n by repeated squaring as Fortran's
-// real**integer), or alpha |E| exp(-E0/|E|) of H2_chem.f90:423-437
+// terms summed left to right), or alpha |E| exp(-E0/|E|) of
+// H2_chem.f90:423-437. An Arrhenius rate with alpha = 0 or Ea = 0 skips
+// the pow or the exp, which would return exactly 1 (T > 0).
 __device__ __forceinline__ double reaction_rate(const ChemDev &C, int r, double T, double efm)
 {
     const int kind = C.rkind[r];
-    if (kind == 0) return C.A[r] * pow(T, C.alpha[r]) * exp(-C.Ea[r] / T);
+    if (kind == 0) {
+        const double a = C.alpha[r], ea = C.Ea[r];
+        const double tp = a == 0.0 ? 1.0 : pow(T, a);
+        const double ex = ea == 0.0 ? 1.0 : exp(-ea / T);
+        return C.A[r] * tp * ex;
+    }
     const double *q = C.rco[r];
     if (kind == 1) {
         // T**2, T**3, T**4 as the reference build forms them: a chain of

@@ -540,6 +540,20 @@ class FieldSolver:
                                          C.byref(by), name, 64))
         return ms.value, by.value, name.value.decode()
 
+    def time_kernel_in_solve(self, soln, kernel=0, fp32=False):
+        """Runs solve(soln) with every finest-level launch of one hot kernel
+        (0 sweep, 1 residual or FP64 defect, 2 restriction, 3 prolongation)
+        timed by its own HIP events; returns (mean ms per launch, launches,
+        V-cycles)."""
+        ms, nl, nc = C.c_double(), C.c_int(), C.c_int()
+        try:
+            st = self.lib.mps_time_in_solve(self._h, int(kernel), int(bool(fp32)), self._ptr(soln, "g", True),
+                                            C.byref(nc), C.byref(ms), C.byref(nl))
+        finally:
+            self._release()
+        M.check(st, allow=(M.OK, M.NOT_CONVERGED))
+        return ms.value, nl.value, nc.value
+
     def find_AX(self, x):
         ax, dg, of = (np.zeros(self.n, order="F") for _ in range(3))
         try:
