@@ -634,7 +634,7 @@ dim3 residual_planes_grid(const Geom &g, int &kz)
 #ifndef MPS_PROL_TY
 #define MPS_PROL_TY 4
 #endif
-constexpr int kRestCX = 32, kRestCY = 8, kProlTX = MPS_PROL_TX, kProlTY = MPS_PROL_TY;
+constexpr int kRestCX = 32, kRestCY = 8, kProlTX = MPS_PROL_TX, kProlTY = MPS_PROL_TY, kProlKZ = 16;
 
 template <typename T>
 void launch_restrict_fw(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *src = nullptr)
@@ -709,11 +709,12 @@ void launch_prolong_tri(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l)
 {
     const Geom &g = c->geom[l];
     Lev<T> vf = view(c, L, l), vc = view(c, L, l + 1);
-    const int kz = std::max(2, z_chunk(g, kProlTX, kProlTY, 4096, 16) & ~1);
+    // kz even and at most the kernel's chunk (kProlKZ planes)
+    const int kz = std::max(2, z_chunk(g, kProlTX, kProlTY, 4096, kProlKZ) & ~1);
     dim3 gr((unsigned)((g.nx + kProlTX - 1) / kProlTX), (unsigned)((g.ny + kProlTY - 1) / kProlTY),
             (unsigned)((g.nz + kz - 1) / kz));
-    hipLaunchKernelGGL((k_prolong_tri_planes<T, kProlTX, kProlTY>), gr, dim3(kProlTX * kProlTY), 0, c->stream, vf,
-                       vc, L[l].u, (const T *)L[l + 1].u, kz);
+    hipLaunchKernelGGL((k_prolong_tri_planes<T, kProlTX, kProlTY, kProlKZ>), gr, dim3(kProlTX * kProlTY), 0,
+                       c->stream, vf, vc, L[l].u, (const T *)L[l + 1].u, kz);
 }
 
 // most block partial sums any residual launch on level 0 writes
@@ -851,14 +852,18 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
     // z ghost-plane multipliers of the fast kernel's OP_REF rows
     const GhostD<T> G = ghost_multipliers(c, v);
     const T zg0 = G.m[BACK], zg1 = G.m[FRONT];
+    // k_sweep_fast: on k TX + 1 columns (rows) k tiles, the last extended
+    // by the remaining one
+    auto tiles = [](int n, int t) { return (n > t && n % t == 1) ? n / t : (n + t - 1) / t; };
+    const dim3 grf((unsigned)tiles(g.nx, kSweepTX), (unsigned)tiles(g.ny, kSweepTY), gr.z);
     auto go = [&](auto O, auto R) {
         if (v.fast && zero_in)
             hipLaunchKernelGGL(
-                (k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, true>), gr,
+                (k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, true>), grf,
                 dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz, zg0, zg1);
         else if (v.fast)
             hipLaunchKernelGGL(
-                (k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, false>), gr,
+                (k_sweep_fast<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY, false>), grf,
                 dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz, zg0, zg1);
         else
             hipLaunchKernelGGL((k_sweep<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
@@ -869,10 +874,10 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
         // kernel's arithmetic for them is the per-level kernels' in either
         // arithmetic mode
         if (zero_in)
-            hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, true>), gr,
+            hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, true>), grf,
                                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz, zg0, zg1);
         else
-            hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, false>), gr,
+            hipLaunchKernelGGL((k_sweep_fast<T, OP_STENCIL, false, kSweepTX, kSweepTY, false>), grf,
                                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz, zg0, zg1);
     } else if (op == OP_REF) {
         if (c->has_reac) go(IntC<OP_REF>{}, BoolC<true>{});

@@ -575,11 +575,12 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_inverse(Lev<double> L, 
         // the pivot column of the other rows, before the update
         for (int r = tid; r < n; r += nt) col[r] = M[(long long)r * n + kk];
         __syncthreads();
-        for (long long q = tid; q < nn; q += nt) {
-            const int r = (int)(q / n), cidx = (int)(q - (long long)r * n);
+        // rows over the waves, columns over the lanes (no index division)
+        for (int r = tid >> 6; r < n; r += nt >> 6) {
             if (r == kk) continue;
             const double f = col[r];
-            M[q] = (cidx == kk ? 0.0 : M[q]) - f * rk[cidx];
+            double *row = M + (long long)r * n;
+            for (int cidx = tid & 63; cidx < n; cidx += 64) row[cidx] = (cidx == kk ? 0.0 : row[cidx]) - f * rk[cidx];
         }
         __syncthreads();
     }

@@ -274,10 +274,9 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     constexpr int N1 = (P1 + NT - 1) / NT;
     constexpr int RW = W1 / 2;   // red cells per ring row
     constexpr int NR = RW * H1;  // red lanes
-    constexpr int BW = TX / 2;   // black cells per tile row
-    constexpr int NB = BW * TY;  // black lanes
     static_assert(TX % 2 == 0, "tile width must be even");
     static_assert(NR <= NT, "one red cell per lane");
+    static_assert((TX / 2 + 1) * (TY + 1) <= NT, "one black cell per lane, extended tile included");
     constexpr int RU = 4, RD = 8;             // ring depths (powers of two: slots are t & (R - 1))
     constexpr bool STN = OP == OP_STENCIL;  // explicit 7-point rows (no D tiles)
     // ZGHOST (OP_REF rows): the z faces of planes 1 and nz take the interior
@@ -290,7 +289,7 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     __shared__ T su[RU][P2];
     __shared__ T sD[STN ? 1 : RD][STN ? 1 : P2];
     __shared__ T sq[REAC ? RU : 1][REAC ? P1 : 1];
-    __shared__ T sr[RU][P1];
+    __shared__ T sr[RU][P1 + W1];   // (one spare row: an extended tile's last black row reads one below)
 
     const int tid = threadIdx.x;
     int bx, by, bz;
@@ -301,6 +300,20 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     const int sy = L.sy;
     const long long sz = L.sz;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
+    // Extended tiles: on a grid of k TX + 1 columns (the 2^m + 1 levels of
+    // the aligned hierarchy) the host launches k tiles in x and the last one
+    // also owns column nx, which is its ring column: its red cells there are
+    // written out, and one more black pair per row covers it. Likewise the
+    // last row. (The fields beyond nx and ny are outside the interior: zero
+    // face factors, staged zeros.)
+    const bool xext = bx == (int)gridDim.x - 1 && i0 + TX == nx;
+    const bool yext = by == (int)gridDim.y - 1 && j0 + TY == ny;
+    const int txe = TX + (xext ? 1 : 0), tye = TY + (yext ? 1 : 0);  // owned columns and rows
+    // The red ring is zeroed once: its black-parity positions are never
+    // written (each slot holds planes of one parity), and an extended
+    // tile's black cells on column nx or row ny read one position past
+    // their row or plane there, with a zero face factor.
+    for (int q = tid; q < RU * (P1 + W1); q += NT) (&sr[0][0])[q] = T(0);
 
     // Every global access goes through buffer resources over the planes
     // this block touches, pl0..pl1 (its chunk and two planes either side,
@@ -391,16 +404,17 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     const int ro0 = (jin && i_r0 >= 1 && i_r0 <= nx) ? i_r0 + sy * jrg : -1;
     const int ro1 = (jin && i_r1 >= 1 && i_r1 <= nx) ? i_r1 + sy * jrg : -1;
     // the red cell lies in this block's tile (it is written out by this lane)
-    const bool rowin = jr >= 1 && jr <= TY;
-    const bool rt0 = ro0 >= 0 && rowin && ii0 >= 1 && ii0 <= TX;
-    const bool rt1 = ro1 >= 0 && rowin && ii1 >= 1 && ii1 <= TX;
+    const bool rowin = jr >= 1 && jr <= tye;
+    const bool rt0 = ro0 >= 0 && rowin && ii0 >= 1 && ii0 <= txe;
+    const bool rt1 = ro1 >= 0 && rowin && ii1 >= 1 && ii1 <= txe;
     T rfxm0, rfxp0, rgxy0, rfxm1, rfxp1, rgxy1, rfym, rfyp;
     xy_factors(i_r0, jrg, rfxm0, rfxp0, rfym, rfyp, rgxy0);
     xy_factors(i_r1, jrg, rfxm1, rfxp1, rfym, rfyp, rgxy1);
     // black lane: tile row yb, pair mb; its cell sits at tile column
     // 2mb + pb, pb = (bpar + k) & 1 on plane k
-    const bool blk_lane = tid < NB;
-    const int yb = tid / BW, mb = tid - yb * BW;
+    const int bw = (txe + 1) / 2;  // black pairs per row
+    const bool blk_lane = tid < bw * tye;
+    const int yb = tid / bw, mb = tid - yb * bw;
     const int bj = j0 + yb;
     const int bpar = (i0 + bj) & 1;
     const int tb0 = 2 * mb, tb1 = 2 * mb + 1;
@@ -1164,13 +1178,16 @@ __global__ void __launch_bounds__(CX *CY *M) k_resrestrict_planes(Lev<T> F, Lev<
     }
 }
 
-// Trilinear prolongation + correction on the aligned hierarchy, marching
-// fine z-planes in odd/even pairs: each lane interpolates the coarse grid
-// in x and y once per coarse plane (fine cell f takes coarse (f+1)/2 when f
-// is odd, the mean of f/2 and f/2+1 when even) and reuses it for the fine
-// planes on either side. Same arithmetic as k_prolong_tri_correct. kz must
-// be even, so every chunk starts on an odd fine plane.
-template <typename T, int TX, int TY>
+// Trilinear prolongation + correction on the aligned hierarchy: each lane
+// owns a fine column of its block's z-chunk (at most KZM planes, kz even,
+// so the chunk starts on an odd fine plane). It interpolates the coarse
+// grid in x and y once per coarse plane the chunk touches (fine cell f
+// takes coarse (f+1)/2 when f is odd, the mean of f/2 and f/2+1 when even)
+// and loads all of its fine values before the first store, so every load
+// of the chunk is in flight at once; then fine plane k gets the coarse
+// plane's value (k odd) or the mean of the two around it (k even). Same
+// arithmetic as k_prolong_tri_correct.
+template <typename T, int TX, int TY, int KZM>
 __global__ void __launch_bounds__(TX *TY) k_prolong_tri_planes(Lev<T> F, Lev<T> C, T *__restrict__ uf,
                                                               const T *__restrict__ uc, int kz)
 {
@@ -1193,19 +1210,24 @@ __global__ void __launch_bounds__(TX *TY) k_prolong_tri_planes(Lev<T> F, Lev<T> 
         const T r1 = T(0.5) * (p[oab] + p[obb]);
         return T(0.5) * (r0 + r1);
     };
-    int K = (kb + 1) >> 1;
-    T e0 = xy(K);
-    T e1 = K + 1 <= C.nz ? xy(K + 1) : T(0);
-    T *col = uf + i + (long long)F.sy * j;
-    for (int k = kb; k <= ke; k += 2) {
-        const bool pair = k + 1 <= ke;
-        const T a = col[F.sz * k];
-        const T b2 = pair ? col[F.sz * (k + 1)] : T(0);
-        col[F.sz * k] = a + e0;
-        if (pair) col[F.sz * (k + 1)] = b2 + T(0.5) * (e0 + e1);
-        e0 = e1;
-        ++K;
-        e1 = K + 1 <= C.nz ? xy(K + 1) : T(0);
+    constexpr int KC = KZM / 2 + 1;  // coarse planes K0 .. K0 + KC - 1
+    const int K0 = (kb + 1) >> 1, nk = ke - kb + 1;
+    T ec[KC], fv[KZM];
+    T *col = uf + i + (long long)F.sy * j + F.sz * kb;
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+        // fine planes kb .. ke use coarse planes K0 .. (ke + 2) / 2 (zero
+        // past the last coarse plane)
+        const int K = K0 + q;
+        ec[q] = (2 * q <= nk && K <= C.nz) ? xy(K) : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < KZM; ++q) fv[q] = q < nk ? col[F.sz * q] : T(0);
+#pragma unroll
+    for (int q = 0; q < KZM; ++q) {
+        if (q >= nk) break;
+        const T e = (q & 1) ? T(0.5) * (ec[q >> 1] + ec[(q >> 1) + 1]) : ec[q >> 1];
+        col[F.sz * q] = fv[q] + e;
     }
 }
 
