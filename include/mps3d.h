@@ -128,8 +128,13 @@ typedef struct {
     int arith;                  /* MPS_ARITH_* for the solver's own rows       */
     int start;                  /* MPS_START_*                                 */
     int dc_inner;               /* FP32 V-cycles per FP64 defect-correction
-                                   step under MPS_PREC_FP32_DC, 1..4
-                                   (default 2)                                 */
+                                   step under MPS_PREC_FP32_DC, 1..4, or 0
+                                   (default): chosen per step, the fewest
+                                   that reach tol at the contraction the
+                                   previous step showed, at most 4 and no
+                                   more than keep a step's reduction above
+                                   1e-5; the first step (and one that did
+                                   not contract) takes 2                       */
 } mps_opts;
 
 typedef struct mps_ctx mps_ctx;
@@ -418,6 +423,13 @@ int mps_get_grid(mps_ctx *ctx, mps_grid *grid, int *nspecies);
 /* Timing of the last mps_solve / mps_solve_transpose on the context's stream
  * (HIP events): total milliseconds and per-cycle milliseconds. */
 int mps_last_timing(mps_ctx *ctx, double *total_ms, int *ncycles);
+
+/* V-cycles of the last solve on the context (no reference counterpart, whose
+ * solves count V-cycles only): under MPS_PREC_FP32_DC the FP32 V-cycles of
+ * all its defect-correction steps, which differ from step to step when
+ * opts.dc_inner is 0; otherwise the V-cycles. A full-multigrid start counts
+ * as one. For callers that report work per solve. */
+int mps_last_vcycles(mps_ctx *ctx, int *nvcycles);
 
 /* Measurement hook (no reference counterpart): times `reps` launches of one
  * hot kernel of the current hierarchy with HIP events on the context's

@@ -27,7 +27,7 @@ ARITH_EXACT, ARITH_FAST = 0, 1
 EXPORTS = (
     "mps_default_opts", "mps_create", "mps_destroy", "mps_set_bc", "mps_set_bc_str", "mps_set_coeffs",
     "mps_vcycle", "mps_solve", "mps_apply", "mps_apply_transpose", "mps_bvec", "mps_smooth",
-    "mps_fill_ghosts", "mps_gmres", "mps_solve_transpose", "mps_last_timing", "mps_stream", "mps_solve_rhs",
+    "mps_fill_ghosts", "mps_gmres", "mps_solve_transpose", "mps_last_timing", "mps_last_vcycles", "mps_stream", "mps_solve_rhs",
     "mps_efield", "mps_set_chemistry", "mps_species_transport", "mps_eenergy_transport",
     "mps_poisson_transport", "mps_electron_temp", "mps_ion_drift_faces",
     "mps_error_string", "mps_last_error", "mps_time_kernel", "mps_grad_velocity", "mps_time_stream",
@@ -107,6 +107,7 @@ def load():
     L.mps_gmres.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), _dp]
     L.mps_solve_transpose.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_int), _dp]
     L.mps_last_timing.argtypes = [_vp, _dp, C.POINTER(C.c_int)]
+    L.mps_last_vcycles.argtypes = [_vp, C.POINTER(C.c_int)]
     L.mps_stream.argtypes = [_vp]
     L.mps_stream.restype = _vp
     L.mps_grad_velocity.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]

@@ -125,6 +125,7 @@ struct LevelBuf {
     T *u = nullptr, *b = nullptr, *r = nullptr, *uold = nullptr;
     T *D = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *reac = nullptr;
     T *st[ST_N] = {};
+    T *fc[4] = {};                // face form of st (fc[0] aliases st[ST_C]; see Lev::fc)
     T *faces[6] = {};
     double *w[3] = {};
     T *u2 = nullptr;              // ping-pong partner of u for fused sweeps
@@ -141,6 +142,10 @@ struct StepGraph {
     int kind = -1;
     unsigned gen = ~0u;
 };
+
+// FP32 V-cycles per FP64 defect-correction step: opts.dc_inner when set
+// (1..kDcInnerMax), else chosen per step from the observed contraction
+constexpr int kDcInnerMax = 4;
 
 // FGMRES(m) work vectors of the finest level, kept by the context between
 // calls (allocated on the first call with a given m or more)
@@ -200,13 +205,15 @@ struct mps_ctx {
     double *fmg_r = nullptr, *fmg_e = nullptr;  // FP64 defect and correction of a full-multigrid start
     bool exact_now = false;              // reference-mirroring call in progress
     // outer solve steps captured as HIP graphs and replayed (see
-    // outer_step): [0] the V-cycle step, [1] the full-multigrid first step
+    // outer_step): [0] the FP64 V-cycle step, [1] the full-multigrid first
+    // step, [1 + k] the defect-correction step of k FP32 V-cycles
     bool graphs = true;                  // MPS3D_NO_GRAPHS=1 turns replay off
     bool fused_rr = true;                // MPS3D_NO_FUSED_RESTRICT=1: residual and restriction apart
     bool sweep_ref = false;              // k_sweep_ref for OP_REF rows (MPS3D_SWEEP_REF=1: on)
     int sweep2 = 1;                      // precisions whose smoothing pairs run in one pass
                                          // (bit 1 FP32, bit 2 FP64; MPS3D_SWEEP2 overrides)
-    StepGraph step[2];
+    StepGraph step[2 + kDcInnerMax];
+    int last_vcycles = 0;                // V-cycles of the last solve (mps_solve_vcycles)
     // in-solve kernel timing (mps_time_in_solve): the kernel and precision
     // whose launches get an event pair, the events, pairs recorded
     int kt_kernel = -1;
@@ -259,6 +266,7 @@ Lev<T> view(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op_o
     v.vz = B.vz;
     v.reac = B.reac;
     for (int q = 0; q < ST_N; ++q) v.st[q] = B.st[q];
+    for (int q = 0; q < 4; ++q) v.fc[q] = B.fc[q];
     v.dx = (T)g.dx;
     v.dy = (T)g.dy;
     v.dz = (T)g.dz;
@@ -517,6 +525,7 @@ void free_levels(std::vector<LevelBuf<T>> &L)
         dfree(B.vz);
         dfree(B.reac);
         for (auto p : B.st) dfree(p);
+        for (int q = 1; q < 4; ++q) dfree(B.fc[q]);
         for (auto p : B.faces) dfree(p);
         for (auto p : B.w) dfree(p);
     }
@@ -795,8 +804,14 @@ void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *wo
     const T tol = (T)(1e-11 * (double)0.0001f);
     const int m = (int)c->geom[l].n();
     with_op(op, c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
-        hipLaunchKernelGGL((k_bottom<T, decltype(O)::value, decltype(V)::value, decltype(R)::value>), dim3(1),
-                           block(), 0, c->stream, v, L[l].u, (const T *)L[l].b, work, m, 1, tol, c->d_status);
+        if (m <= kBottomLdsM)
+            hipLaunchKernelGGL((k_bottom<T, decltype(O)::value, decltype(V)::value, decltype(R)::value, true>),
+                               dim3(1), block(), 0, c->stream, v, L[l].u, (const T *)L[l].b, work, m, 1, tol,
+                               c->d_status);
+        else
+            hipLaunchKernelGGL((k_bottom<T, decltype(O)::value, decltype(V)::value, decltype(R)::value, false>),
+                               dim3(1), block(), 0, c->stream, v, L[l].u, (const T *)L[l].b, work, m, 1, tol,
+                               c->d_status);
     });
 }
 
@@ -869,7 +884,15 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
             hipLaunchKernelGGL((k_sweep<T, decltype(O)::value, decltype(R)::value, kSweepTX, kSweepTY>), gr,
                                dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz);
     };
-    if (op == OP_STENCIL) {
+    if (op == OP_STENCIL && L[l].fc[1]) {
+        // the same rows from their face form (conductance coarse levels)
+        if (zero_in)
+            hipLaunchKernelGGL((k_sweep_fast<T, OP_FACES, false, kSweepTX, kSweepTY, true>), grf,
+                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz, zg0, zg1);
+        else
+            hipLaunchKernelGGL((k_sweep_fast<T, OP_FACES, false, kSweepTX, kSweepTY, false>), grf,
+                               dim3(kSweepTX * kSweepTY), 0, c->stream, v, uin, uout, (const T *)L[l].b, kz, zg0, zg1);
+    } else if (op == OP_STENCIL) {
         // explicit rows carry their own reaction and time terms; the fast
         // kernel's arithmetic for them is the per-level kernels' in either
         // arithmetic mode
@@ -1195,6 +1218,16 @@ void conductance_coarse_levels(mps_ctx *c)
         L[l].op = on ? OP_STENCIL : -1;
         if (l < (int)c->L32.size()) c->L32[l].op = L[l].op;
     }
+    // the face form exists only while the levels hold these rows
+    for (int l = 0; l < c->nlev(); ++l) {
+        for (int q = 1; q < 4; ++q) {
+            if (!on || l == 0) {
+                dfree(L[l].fc[q]);
+                L[l].fc[q] = nullptr;
+            }
+        }
+        if (!on || l == 0) L[l].fc[0] = nullptr;
+    }
     if (!on) return;
     std::vector<double *> f(3 * (size_t)c->nlev(), nullptr);
     try {
@@ -1217,9 +1250,17 @@ void conductance_coarse_levels(mps_ctx *c)
                 if (!B.st[q]) B.st[q] = dalloc<double>(g.ng());
                 fill<double>(c, B.st[q], g.ng(), 0.0);
             }
+            // face form: conductances (ghost entries stay 0: the boundary
+            // faces) and the diagonal of st
+            for (int q = 1; q < 4; ++q) {
+                if (!B.fc[q]) B.fc[q] = dalloc<double>(g.ng());
+                fill<double>(c, B.fc[q], g.ng(), 0.0);
+            }
+            B.fc[0] = B.st[ST_C];
             hipLaunchKernelGGL(k_faces_stencil, cells_grid(g), block(), 0, c->stream, view(c, L, l),
                                (const double *)f[3 * l], (const double *)f[3 * l + 1], (const double *)f[3 * l + 2],
-                               B.st[0], B.st[1], B.st[2], B.st[3], B.st[4], B.st[5], B.st[6]);
+                               B.st[0], B.st[1], B.st[2], B.st[3], B.st[4], B.st[5], B.st[6], B.fc[1], B.fc[2],
+                               B.fc[3]);
         }
         HIPCHK(hipStreamSynchronize(c->stream));
     } catch (...) {
@@ -1279,6 +1320,24 @@ void build_fp32(mps_ctx *c)
             if (!L[l].st[q]) continue;
             if (!F[l].st[q]) F[l].st[q] = dalloc<float>(ng);
             convert<double, float>(c, L[l].st[q], F[l].st[q], ng);
+        }
+        // face form: FP32 conductances, the FP32 diagonal, and the FP32
+        // off-diagonal rows rebuilt from them in FP32 (what the face-form
+        // sweep forms), so all FP32 kernels of the level apply one operator
+        if (L[l].fc[1]) {
+            for (int q = 1; q < 4; ++q) {
+                if (!F[l].fc[q]) F[l].fc[q] = dalloc<float>(ng);
+                convert<double, float>(c, L[l].fc[q], F[l].fc[q], ng);
+            }
+            F[l].fc[0] = F[l].st[ST_C];
+            hipLaunchKernelGGL(k_faces_rows<float>, cells_grid(c->geom[l]), block(), 0, c->stream, view(c, F, l),
+                               F[l].st[1], F[l].st[2], F[l].st[3], F[l].st[4], F[l].st[5], F[l].st[6]);
+        } else {
+            for (int q = 1; q < 4; ++q) {
+                dfree(F[l].fc[q]);
+                F[l].fc[q] = nullptr;
+            }
+            F[l].fc[0] = nullptr;
         }
     }
 }
@@ -1455,14 +1514,15 @@ void b_sumsq(mps_ctx *c, const double *b)
 // HIP graph and replayed, which takes the host launch cost off the critical
 // path between the per-cycle convergence checks. Every node is a kernel
 // (fills included), so replays do not depend on memset nodes.
-void outer_step_body(mps_ctx *c, double *u, double *b, bool homog, bool dc)
+void outer_step_body(mps_ctx *c, double *u, double *b, bool homog, bool dc, int inner)
 {
     if (dc) {
         vcycle<float>(c, c->L32, 0, c->bottom32, true, true);
-        // further FP32 cycles on the same defect from the current correction:
-        // the FP32 cycle's error reduction (~0.08 at 257^3) stays far above
-        // FP32 rounding, so two per FP64 defect halve the FP64 work
-        for (int q = 1; q < c->opts.dc_inner; ++q) vcycle<float>(c, c->L32, 0, c->bottom32, true, false);
+        // further FP32 cycles on the same defect from the current correction
+        // (inner in all): the FP32 cycle's error reduction (~0.08 at 257^3)
+        // stays far above FP32 rounding for a few cycles, so each FP64
+        // defect carries several
+        for (int q = 1; q < inner; ++q) vcycle<float>(c, c->L32, 0, c->bottom32, true, false);
         Lev<double> v64 = view(c, c->L64, 0);
         hipLaunchKernelGGL((k_add_interior<double, float>), cells_grid(c->geom[0]), block(), 0, c->stream, v64, u,
                            (const float *)c->L32[0].u, (const double *)c->d_dcscale);
@@ -1513,20 +1573,21 @@ void fmg_buffers(mps_ctx *c)
     }
 }
 
-// One outer step (fmg_first: the full-multigrid first step), replayed from
-// a HIP graph captured on first use.
-void outer_step(mps_ctx *c, double *u, double *b, bool homog, bool dc, bool fmg_first = false)
+// One outer step (fmg_first: the full-multigrid first step; inner: FP32
+// V-cycles of a defect-correction step), replayed from a HIP graph captured
+// on first use.
+void outer_step(mps_ctx *c, double *u, double *b, bool homog, bool dc, bool fmg_first = false, int inner = 1)
 {
     if (fmg_first && !dc) fmg_buffers(c);
     auto body = [&]() {
         if (fmg_first) fmg_step_body(c, u, b, dc);
-        else outer_step_body(c, u, b, homog, dc);
+        else outer_step_body(c, u, b, homog, dc, inner);
     };
     if (!c->graphs) {
         body();
         return;
     }
-    StepGraph &G = c->step[fmg_first ? 1 : 0];
+    StepGraph &G = c->step[fmg_first ? 1 : dc ? 1 + inner : 0];
     const int kind = (dc ? 2 : 0) + (homog ? 1 : 0) + (c->exact_now ? 4 : 0);
     if (!G.exec || G.u != u || G.b != b || G.kind != kind || G.gen != c->gen) {
         if (G.exec) {
@@ -1557,9 +1618,29 @@ void outer_step(mps_ctx *c, double *u, double *b, bool homog, bool dc, bool fmg_
     HIPCHK(hipGraphLaunch(G.exec, c->stream));
 }
 
+// FP32 V-cycles for the next defect-correction step after one of k cycles
+// took the measure from prev to now: the fewest that reach tol at the
+// step's per-cycle contraction rho = (now / prev)^(1/k), but no more than
+// keep the step's reduction above 1e-5 (FP32 rounding bounds what one FP32
+// correction of the defect can deliver), within 1..kDcInnerMax; two when
+// the step did not contract.
+int dc_inner_next(double prev, double now, int k, double tol)
+{
+    if (!(now > 0.0 && prev > now && tol > 0.0)) return 2;
+    const double lr = std::log(now / prev) / k;   // log rho < 0
+    if (!(lr < 0.0)) return 2;
+    const double need = std::log(tol / now) / lr;  // cycles to reach tol
+    const double room = std::log(1e-5) / lr;      // cycles before the FP32 limit
+    int q = (int)std::ceil(std::min(need, kDcInnerMax + 1.0));
+    q = std::min(q, std::max(1, (int)std::floor(std::min(room, kDcInnerMax + 1.0))));
+    return std::max(1, std::min(q, kDcInnerMax));
+}
+
 // The solve loop shared by the forward and the self-adjoint solves: V-cycles
 // (FP64) or FP32 defect-correction steps on u for right-hand side b until
-// the opts.tol_kind measure <= opts.tol. Returns the cycles run.
+// the opts.tol_kind measure <= opts.tol. Returns the cycles run (outer
+// steps); c->last_vcycles gets the V-cycles (a full-multigrid pass counts
+// as one).
 int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, double &meas)
 {
     double bn2 = 0.0;
@@ -1577,25 +1658,33 @@ int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, do
     }
     int it = 0;
     meas = 0.0;
+    c->last_vcycles = 0;
     if (dc) {
         // defect correction: residual of the current iterate first
         defect_step(c, u, false, b);
         meas = measure(c, read_scalar(c), bn2);
+        const bool adapt = c->opts.dc_inner == 0;
+        int inner = adapt ? 2 : c->opts.dc_inner;
         if (!(meas <= c->opts.tol)) {
             for (it = 0; it < c->opts.max_cycles;) {
-                // the FP32 cycle on the defect, the FP64 correction, then the
+                // the FP32 cycles on the defect, the FP64 correction, then the
                 // FP64 residual of the corrected iterate (the next defect)
-                outer_step(c, u, b, homog, true, it == 0 && c->opts.start == MPS_START_FMG);
+                const bool fmg_first = it == 0 && c->opts.start == MPS_START_FMG;
+                outer_step(c, u, b, homog, true, fmg_first, inner);
+                c->last_vcycles += fmg_first ? 1 : inner;
+                const double prev = meas;
                 meas = measure(c, read_scalar(c), bn2);
                 if (history) history[it] = meas;
                 ++it;
                 if (!std::isfinite(meas)) break;
                 if (meas <= c->opts.tol) break;
+                if (adapt && !fmg_first) inner = dc_inner_next(prev, meas, inner, c->opts.tol);
             }
         }
     } else {
         for (it = 0; it < c->opts.max_cycles;) {
             outer_step(c, u, b, homog, false, it == 0 && c->opts.start == MPS_START_FMG && !c->geom[0].bottom);
+            ++c->last_vcycles;
             meas = c->geom[0].bottom ? 0.0 : measure(c, read_scalar(c), bn2);
             if (history) history[it] = meas;
             ++it;
@@ -1764,7 +1853,7 @@ void mps_default_opts(mps_opts *o)
     o->tol_kind = MPS_TOL_RMS;
     o->arith = MPS_ARITH_EXACT;
     o->start = MPS_START_GUESS;
-    o->dc_inner = 2;
+    o->dc_inner = 0;
 }
 
 const char *mps_error_string(int s)
@@ -1848,9 +1937,9 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
         delete c;
         return fail(MPS_ERR_ARG, "bad options");
     }
-    if (c->opts.dc_inner < 1 || c->opts.dc_inner > 4) {
+    if (c->opts.dc_inner < 0 || c->opts.dc_inner > kDcInnerMax) {
         delete c;
-        return fail(MPS_ERR_ARG, "dc_inner must be 1..4");
+        return fail(MPS_ERR_ARG, "dc_inner must be 0 (chosen per step) or 1..4");
     }
     if (c->opts.start != MPS_START_GUESS &&
         (c->opts.start != MPS_START_FMG || c->opts.hierarchy != MPS_HIER_ALIGNED)) {
@@ -2968,6 +3057,13 @@ int mps_last_timing(mps_ctx *c, double *total_ms, int *ncycles)
     return MPS_OK;
 }
 
+int mps_last_vcycles(mps_ctx *c, int *nvcycles)
+{
+    if (!c || !nvcycles) return fail(MPS_ERR_ARG, "null argument");
+    *nvcycles = c->last_vcycles;
+    return MPS_OK;
+}
+
 }  // extern "C"
 
 // ======================================================= Krylov and adjoint
@@ -3102,8 +3198,14 @@ void gmres_ref_launch(mps_ctx *c, const double *bd, double *u, int m, int max_re
     Lev<double> v = view(c, c->L64, 0);
     hipLaunchKernelGGL(k_interior_in<double>, cells_grid(g), block(), 0, c->stream, v, bd, bg);
     with_op(level_op(c, 0), c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
-        hipLaunchKernelGGL((k_bottom<double, decltype(O)::value, decltype(V)::value, decltype(R)::value>), dim3(1),
-                           block(), 0, c->stream, v, u, (const double *)bg, work, m, max_restarts, tol, c->d_status);
+        if (m <= kBottomLdsM)
+            hipLaunchKernelGGL((k_bottom<double, decltype(O)::value, decltype(V)::value, decltype(R)::value, true>),
+                               dim3(1), block(), 0, c->stream, v, u, (const double *)bg, work, m, max_restarts, tol,
+                               c->d_status);
+        else
+            hipLaunchKernelGGL((k_bottom<double, decltype(O)::value, decltype(V)::value, decltype(R)::value, false>),
+                               dim3(1), block(), 0, c->stream, v, u, (const double *)bg, work, m, max_restarts, tol,
+                               c->d_status);
     });
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_scalar + 1, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));

@@ -38,7 +38,11 @@ __host__ __device__ inline long long bottom_work_elems(int m, long long ng)
 
 // performgmres with nrestarts restarts of GMRES(m); the coarse solve uses
 // m = n, nrestarts = 1 and tol = ERRTOLR*0.0001 (mgridsteps.f90:305, :342-343).
-template <typename T, int OP, bool VEL, bool REAC>
+// LM: m <= kBottomLdsM, the Hessenberg and Givens arrays in LDS. A compile-
+// time choice, so they are addressed as LDS (a pointer chosen at run time
+// between LDS and global memory is a generic one, and every access through it
+// a flat instruction).
+template <typename T, int OP, bool VEL, bool REAC, bool LM>
 __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__restrict__ b, T *work, int m,
                                                   int nrestarts, T tol, int *status)
 {
@@ -56,9 +60,9 @@ __global__ void __launch_bounds__(kBlock) k_bottom(Lev<T> L, T *u, const T *__re
     T *mb = wj + ng;
     __shared__ T sH[(kBottomLdsM + 1) * kBottomLdsM];
     __shared__ T sg[4 * (kBottomLdsM + 1)];
-    const bool lds = m <= kBottomLdsM;
-    T *H = lds ? sH : mb + ng;
-    T *cs = lds ? sg : H + (long long)(m + 1) * m;
+    const bool lds = LM;
+    T *H = LM ? sH : mb + ng;
+    T *cs = LM ? sg : H + (long long)(m + 1) * m;
     T *sn = cs + (m + 1);
     T *be = sn + (m + 1);
     T *y = be + (m + 1);
@@ -514,24 +518,101 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_inverse(Lev<double> L, 
     __shared__ int piv;
     const int nx = L.nx, ny = L.ny, n = L.nx * L.ny * L.nz;
     const int tid = threadIdx.x, nt = blockDim.x;
-    double *M = n <= kDenseLdsN ? sM : gwork;
     const long long nn = (long long)n * n;
-    for (long long q = tid; q < nn; q += nt) M[q] = 0.0;
-    __syncthreads();
-    for (int r = tid; r < n; r += nt) {
-        const int i = r % nx + 1, j = (r / nx) % ny + 1, k = r / (nx * ny) + 1;
-        const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
-        double *row = M + (long long)r * n;
-        row[r] = st0[c];
-        if (i > 1) row[r - 1] = st1[c];
-        if (i < nx) row[r + 1] = st2[c];
-        if (j > 1) row[r - nx] = st3[c];
-        if (j < ny) row[r + nx] = st4[c];
-        if (k > 1) row[r - nx * ny] = st5[c];
-        if (k < L.nz) row[r + nx * ny] = st6[c];
+    // A into the matrix: called with the LDS array or the global work array
+    // itself, so each call addresses one memory kind
+    auto assemble = [&](double *M) {
+        for (long long q = tid; q < nn; q += nt) M[q] = 0.0;
+        __syncthreads();
+        for (int r = tid; r < n; r += nt) {
+            const int i = r % nx + 1, j = (r / nx) % ny + 1, k = r / (nx * ny) + 1;
+            const long long c = (long long)i + (long long)L.sy * j + L.sz * k;
+            double *row = M + (long long)r * n;
+            row[r] = st0[c];
+            if (i > 1) row[r - 1] = st1[c];
+            if (i < nx) row[r + 1] = st2[c];
+            if (j > 1) row[r - nx] = st3[c];
+            if (j < ny) row[r + nx] = st4[c];
+            if (k > 1) row[r - nx * ny] = st5[c];
+            if (k < L.nz) row[r + nx * ny] = st6[c];
+        }
+        if (tid == 0 && status) *status = 0;
+        __syncthreads();
+    };
+    if (n <= kDenseLdsN) {
+        assemble(sM);
+        // LDS-resident matrix, one column per lane. Wave 0 alone finds the
+        // pivot (a butterfly over the candidate rows, one per lane), swaps
+        // the rows and scales the pivot row, with no barrier between those
+        // steps; then every wave updates its rows (wave w: rows w + 16 m),
+        // each lane loading all its values before it stores. Two barriers
+        // per pivot. The row swaps are recorded as the original row index
+        // at each position (pos, one per lane of wave 0) and undone at the
+        // end by scattering each row's columns to their original places,
+        // which moves the same values as swapping the columns back, last
+        // swap first. Launched with kDenseThreads threads (16 waves, four
+        // rows each).
+        constexpr int kW = kDenseThreads / 64, kRows = kDenseLdsN / kW;
+        const int lane = tid & 63, wv = tid >> 6;
+        int pos = lane;
+        for (int kk = 0; kk < n; ++kk) {
+            if (wv == 0) {
+                const int r = kk + lane;
+                double v = r < n ? fabs(sM[r * n + kk]) : -1.0;
+                if (!(v >= 0.0)) v = -1.0;   // NaN never wins
+                int bi = r;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ov = __shfl_xor(v, off, 64);
+                    const int oi = __shfl_xor(bi, off, 64);
+                    if (ov > v || (ov == v && oi < bi)) v = ov, bi = oi;
+                }
+                const int p = bi;   // the same in every lane
+                if (lane == 0 && !(v > 0.0) && status) *status = -1;
+                const int pk = __shfl(pos, kk, 64), pp = __shfl(pos, p, 64);
+                if (lane == kk)
+                    pos = pp;
+                else if (lane == p)
+                    pos = pk;
+                double a = 0.0;
+                if (lane < n) {
+                    a = sM[kk * n + lane];
+                    if (p != kk) {
+                        const double b = sM[p * n + lane];
+                        sM[p * n + lane] = a;
+                        a = b;
+                    }
+                }
+                const double inv = 1.0 / __shfl(a, kk, 64);
+                if (lane < n) sM[kk * n + lane] = (lane == kk ? 1.0 : a) * inv;
+            }
+            __syncthreads();
+            const double rkc = lane < n ? sM[kk * n + lane] : 0.0;
+            double f[kRows], x[kRows];
+#pragma unroll
+            for (int m = 0; m < kRows; ++m) {
+                const int r = wv + m * kW;
+                f[m] = r < n ? sM[r * n + kk] : 0.0;
+                x[m] = (r < n && lane < n) ? sM[r * n + lane] : 0.0;
+            }
+#pragma unroll
+            for (int m = 0; m < kRows; ++m) {
+                const int r = wv + m * kW;
+                if (r < n && r != kk && lane < n) sM[r * n + lane] = (lane == kk ? 0.0 : x[m]) - f[m] * rkc;
+            }
+            __syncthreads();
+        }
+        if (wv == 0 && lane < n) perm[lane] = pos;
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < kRows; ++m) {
+            const int r = wv + m * kW;
+            if (r < n && lane < n) Ainv[r * n + perm[lane]] = sM[r * n + lane];
+        }
+        return;
     }
-    if (tid == 0 && status) *status = 0;
-    __syncthreads();
+    double *const M = gwork;
+    assemble(M);
     for (int kk = 0; kk < n; ++kk) {
         // pivot: the largest |M[r][kk]|, r >= kk, lowest r on ties
         double best = -1.0;

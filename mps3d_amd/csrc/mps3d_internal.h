@@ -22,7 +22,10 @@ enum BcCode { BC_NONE = 0, BC_DIRC = 1, BC_ZGRD = 2, BC_FLUX = 3 };
 //              (H + h0)/2 and sit h0/2 from the wall.
 //  OP_STENCIL  explicit 7-point rows: c u + xm u(i-1) + xp u(i+1) + ...
 //              (Scharfetter-Gummel operators and transposed operators).
-enum OpKind { OP_REF = 0, OP_ALIGNED = 1, OP_STENCIL = 2 };
+//  OP_FACES    (a kernel variant, not a level's form) OP_STENCIL rows of
+//              a symmetric operator read from their face form (Lev::fc):
+//              four arrays instead of seven.
+enum OpKind { OP_REF = 0, OP_ALIGNED = 1, OP_STENCIL = 2, OP_FACES = 3 };
 // flux schemes as the kernels see them (values of MPS_FLUX_*)
 enum { MPS_FLUX_UPWIND_K = 0, MPS_FLUX_SG_K = 1 };
 
@@ -39,6 +42,12 @@ struct Lev {
     long long sz;
     const T *D, *vx, *vy, *vz, *reac;   // cell coefficients (nullable)
     const T *st[ST_N];                  // explicit stencil rows (OP_STENCIL)
+    // face form of the rows (nullable; conductance-coarsened levels): the
+    // diagonal and the + face conductance of each cell per direction, G_d;
+    // row c's x- entry is -(G_x[c - 1] * a) and its x+ entry -(G_x[c] * a),
+    // a = axb on the first and last column, else ax (y, z alike); the
+    // faces on the boundary hold 0
+    const T *fc[4];
     T dx, dy, dz;                       // centre spacing
     T ax, ay, az;                       // 1/width of interior cells
     T axb, ayb, azb;                    // 1/width of boundary cells (OP_ALIGNED)

@@ -1067,14 +1067,20 @@ __global__ void __launch_bounds__(kBlock) k_coarsen_faces(Lev<double> F, Lev<dou
 // 7-point rows of an aligned coarse level from its face coefficients: face
 // weight (f / d) * abv, with abv the reciprocal width of the row's cell;
 // DIRC walls D_c * idist * abv (the cell coefficient at the wall); then
-// -reac and +1/dt, as the cell-coefficient rows of the aligned grids
+// -reac and +1/dt, as the cell-coefficient rows of the aligned grids. Also
+// the rows' face form: the face conductances f / d per direction (gx, gy,
+// gz; the diagonal is st0)
 __global__ void __launch_bounds__(kBlock) k_faces_stencil(Lev<double> L, const double *fx, const double *fy,
                                                          const double *fz, double *st0, double *st1, double *st2,
-                                                         double *st3, double *st4, double *st5, double *st6)
+                                                         double *st3, double *st4, double *st5, double *st6,
+                                                         double *gx, double *gy, double *gz)
 {
     int i, j, k;
     long long c;
     if (!cell_of(L, i, j, k, c)) return;
+    gx[c] = fx[c] / L.dx;
+    gy[c] = fy[c] / L.dy;
+    gz[c] = fz[c] / L.dz;
     double d = 0.0, w;
     const double Dc = L.D[c];
     {
@@ -1115,6 +1121,29 @@ __global__ void __launch_bounds__(kBlock) k_faces_stencil(Lev<double> L, const d
     if (L.reac) d -= L.reac[c];
     if (L.timederiv) d += 1.0 / L.dt;
     st0[c] = d;
+}
+
+// The off-diagonal rows of a level from their face form (L.fc), in T's
+// arithmetic: x- entry -(G_x[c - 1] * a), x+ entry -(G_x[c] * a), with a the
+// row's reciprocal cell width (y, z alike). For FP64 these are k_faces_stencil's
+// values; for the FP32 copy of a level, the values the face-form sweep
+// (k_sweep_fast<T, OP_FACES>) forms, so every kernel on the level applies the
+// same FP32 operator.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_faces_rows(Lev<T> L, T *st1, T *st2, T *st3, T *st4, T *st5, T *st6)
+{
+    int i, j, k;
+    long long c;
+    if (!cell_of(L, i, j, k, c)) return;
+    const T ax = (i == 1 || i == L.nx) ? L.axb : L.ax;
+    const T ay = (j == 1 || j == L.ny) ? L.ayb : L.ay;
+    const T az = (k == 1 || k == L.nz) ? L.azb : L.az;
+    st1[c] = -(L.fc[1][c - 1] * ax);
+    st2[c] = -(L.fc[1][c] * ax);
+    st3[c] = -(L.fc[2][c - L.sy] * ay);
+    st4[c] = -(L.fc[2][c] * ay);
+    st5[c] = -(L.fc[3][c - L.sz] * az);
+    st6[c] = -(L.fc[3][c] * az);
 }
 
 // ------------------------------------------------------------------------

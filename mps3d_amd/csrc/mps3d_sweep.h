@@ -278,7 +278,13 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     static_assert(NR <= NT, "one red cell per lane");
     static_assert((TX / 2 + 1) * (TY + 1) <= NT, "one black cell per lane, extended tile included");
     constexpr int RU = 4, RD = 8;             // ring depths (powers of two: slots are t & (R - 1))
-    constexpr bool STN = OP == OP_STENCIL;  // explicit 7-point rows (no D tiles)
+    constexpr bool STN = OP == OP_STENCIL || OP == OP_FACES;  // explicit 7-point rows (no D tiles)
+    // OP_FACES: the rows' face form (L.fc): per cell the diagonal and the
+    // conductances of its six faces (the -, + faces of each direction, the
+    // - one read at the neighbour's offset), so four arrays are streamed
+    // instead of seven; each coefficient is formed as -(G * a), which is
+    // the value k_faces_rows stores for the level's other kernels
+    constexpr bool FCS = OP == OP_FACES;
     // ZGHOST (OP_REF rows): the z faces of planes 1 and nz take the interior
     // factor hz like every other plane, and the staged D of the ghost plane
     // carries the boundary instead: zg * D of the adjacent interior plane,
@@ -330,7 +336,7 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     const __amdgpu_buffer_rsrc_t Rd = rsrc(STN ? b : L.D), Rq = rsrc(REAC ? L.reac : b);
     __amdgpu_buffer_rsrc_t Rs[STN ? ST_N : 1];
 #pragma unroll
-    for (int q = 0; q < (STN ? ST_N : 1); ++q) Rs[q] = rsrc(STN ? L.st[q] : b);
+    for (int q = 0; q < (FCS ? 4 : STN ? ST_N : 1); ++q) Rs[q] = rsrc(FCS ? L.fc[q] : STN ? L.st[q] : b);
     // byte offset of plane kp in the resources (kp clamped to pl0..pl1)
     auto pofs = [&](int kp) { return (min(max(kp, pl0), pl1) - pl0) * szb; };
 
@@ -437,15 +443,22 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         int q, e, o;    // LDS index (ring plane), LDS index (halo-2 plane), field offset (-1: none)
         int vb, vs;     // in-plane bytes of the cell for loads and for the store (kOob: none)
         T fxm, fxp, g;  // x face factors and x/y DIRC factor
+        int vxm, vym;   // OP_FACES: in-plane bytes of the x- and y- neighbours (kOob: none)
+        T ax, ay;       // OP_FACES: the row's reciprocal cell widths in x and y
     };
     auto vbytes = [](int o) { return o >= 0 ? o * (int)sizeof(T) : kOob; };
+    auto cell = [&](int q, int e, int o, bool own, T fxm, T fxp, T g, int i, int j) {
+        const T ax = (i == 1 || i == nx) ? L.axb : L.ax, ay = (j == 1 || j == ny) ? L.ayb : L.ay;
+        return Cell{q, e, o, vbytes(o), own ? vbytes(o) : kOob, fxm, fxp, g, o >= 0 ? vbytes(o - 1) : kOob,
+                    o >= 0 ? vbytes(o - sy) : kOob, ax, ay};
+    };
     auto red_cell = [&](int pc) {
-        return pc ? Cell{rq1, re1, ro1, vbytes(ro1), rt1 ? vbytes(ro1) : kOob, rfxm1, rfxp1, rgxy1}
-                  : Cell{rq0, re0, ro0, vbytes(ro0), rt0 ? vbytes(ro0) : kOob, rfxm0, rfxp0, rgxy0};
+        return pc ? cell(rq1, re1, ro1, rt1, rfxm1, rfxp1, rgxy1, i_r1, jrg)
+                  : cell(rq0, re0, ro0, rt0, rfxm0, rfxp0, rgxy0, i_r0, jrg);
     };
     auto blk_cell = [&](int pc) {
-        return pc ? Cell{bq1, be1, cb1, vbytes(cb1), vbytes(cb1), bfxm1, bfxp1, bgxy1}
-                  : Cell{bq0, be0, cb0, vbytes(cb0), vbytes(cb0), bfxm0, bfxp0, bgxy0};
+        return pc ? cell(bq1, be1, cb1, true, bfxm1, bfxp1, bgxy1, bi1, bj)
+                  : cell(bq0, be0, cb0, true, bfxm0, bfxp0, bgxy0, bi0, bj);
     };
     const int pr = (rpar + kb - 1) & 1, pbk = (bpar + kb - 2) & 1;
     const Cell R0 = red_cell(pr), R1 = red_cell(pr ^ 1);
@@ -486,7 +499,26 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         const int vb_ = x ? B1.vb : B0.vb;
         const T vr = buf_load<T>(Rb, vr_, pr);
         const T vb = buf_load<T>(Rb, vb_, pb);
-        if constexpr (STN) {
+        if constexpr (FCS) {
+            // the diagonal and the six face conductances, in ST_* order
+            const Cell &CR = x ? R1 : R0, &CB = x ? B1 : B0;
+            const int pr2 = pofs(kp - 2), pb2 = pofs(kp - 3);
+            T *ra = x ? rs1 : rs0, *ba = x ? bs1 : bs0;
+            ra[ST_C] = buf_load<T>(Rs[0], vr_, pr);
+            ra[ST_XM] = buf_load<T>(Rs[1], CR.vxm, pr);
+            ra[ST_XP] = buf_load<T>(Rs[1], vr_, pr);
+            ra[ST_YM] = buf_load<T>(Rs[2], CR.vym, pr);
+            ra[ST_YP] = buf_load<T>(Rs[2], vr_, pr);
+            ra[ST_ZM] = buf_load<T>(Rs[3], vr_, pr2);
+            ra[ST_ZP] = buf_load<T>(Rs[3], vr_, pr);
+            ba[ST_C] = buf_load<T>(Rs[0], vb_, pb);
+            ba[ST_XM] = buf_load<T>(Rs[1], CB.vxm, pb);
+            ba[ST_XP] = buf_load<T>(Rs[1], vb_, pb);
+            ba[ST_YM] = buf_load<T>(Rs[2], CB.vym, pb);
+            ba[ST_YP] = buf_load<T>(Rs[2], vb_, pb);
+            ba[ST_ZM] = buf_load<T>(Rs[3], vb_, pb2);
+            ba[ST_ZP] = buf_load<T>(Rs[3], vb_, pb);
+        } else if constexpr (STN) {
 #pragma unroll
             for (int q = 0; q < ST_N; ++q) {
                 const T sr_ = buf_load<T>(Rs[q], vr_, pr), sb_ = buf_load<T>(Rs[q], vb_, pb);
@@ -558,6 +590,18 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         o = o + st[ST_ZP] * uzp;
         return gs_update(L, bv, o, st[ST_C]);
     };
+    // the same row from its face form: g holds the diagonal and the face
+    // conductances in ST_* order, each coefficient is -(G * a)
+    auto faces_row = [&](const T *g, T ax, T ay, T az, T uxm, T uxp, T uym, T uyp, T uzm, T uzp, T bv) {
+        T o = T(0);
+        o = o + (-(g[ST_XM] * ax)) * uxm;
+        o = o + (-(g[ST_XP] * ax)) * uxp;
+        o = o + (-(g[ST_YM] * ay)) * uym;
+        o = o + (-(g[ST_YP] * ay)) * uyp;
+        o = o + (-(g[ST_ZM] * az)) * uzm;
+        o = o + (-(g[ST_ZP] * az)) * uzp;
+        return gs_update(L, bv, o, g[ST_C]);
+    };
     // z face factors of interior planes; planes 1 and nz get their own
     T fzm_i, fzp_i, gz_i;
     z_factors(2, fzm_i, fzp_i, gz_i);
@@ -615,7 +659,11 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
             if (s >= 1 && s <= nz && C.o >= 0) {
                 const int e = C.e;
                 const T *um = su[um_], *u0 = su[u0_], *up = su[up_];
-                if constexpr (STN) {
+                if constexpr (FCS) {
+                    const T az = (s == 1 || s == nz) ? L.azb : L.az;
+                    val = faces_row(x ? rs1 : rs0, C.ax, C.ay, az, u0[e - 1], u0[e + 1], u0[e - W2], u0[e + W2], um[e],
+                                    up[e], br);
+                } else if constexpr (STN) {
                     val = stencil_row(x ? rs1 : rs0, u0[e - 1], u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], br);
                 } else {
                     const T *Dm = sD[dm_], *D0 = sD[d0_], *Dp = sD[dp_];
@@ -636,7 +684,11 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
             // red planes k-1, k, k+1 = s-2, s-1, s
             const T *rm = sr[umm_], *r0 = sr[um_], *rp = sr[u0_];
             T v;
-            if constexpr (STN) {
+            if constexpr (FCS) {
+                const T az = (k == 1 || k == nz) ? L.azb : L.az;
+                v = faces_row(x ? bs1 : bs0, Bc.ax, Bc.ay, az, r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q],
+                              rp[q], bk);
+            } else if constexpr (STN) {
                 v = stencil_row(x ? bs1 : bs0, r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q], rp[q], bk);
             } else {
                 const T *Dm = sD[dmm_], *D0 = sD[dm_], *Dp = sD[d0_];

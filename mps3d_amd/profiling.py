@@ -155,6 +155,40 @@ def traffic_table(fetch_csv, write_csv, kernels):
     return "\n".join(out)
 
 
+def counter_table(csvs, kernels, grid=None):
+    """Markdown table of the mean value per dispatch of every counter in the
+    given --pmc CSVs, for each kernel name filter (dispatches of one grid
+    size when given, else of the largest grid the kernel was launched
+    with)."""
+    vals = defaultdict(lambda: defaultdict(dict))  # kernel filter -> counter -> {dispatch: value}
+    grids = defaultdict(dict)
+    names = []
+    for path in csvs:
+        for r in csv.DictReader(open(path)):
+            n, c = r["Kernel_Name"], r["Counter_Name"]
+            for k in kernels:
+                if k in n:
+                    d = int(r["Dispatch_Id"])
+                    vals[k][c][d] = vals[k][c].get(d, 0.0) + float(r["Counter_Value"])
+                    grids[k][d] = _grid_total(r)
+            if c not in names:
+                names.append(c)
+    out = ["| kernel | grid (threads) | dispatches | " + " | ".join(names) + " |",
+           "|---|---|---|" + "---|" * len(names)]
+    for k in kernels:
+        if k not in grids:
+            continue
+        g = grid if grid is not None else max(grids[k].values())
+        cells = []
+        nd = 0
+        for c in names:
+            ds = [v for d, v in vals[k][c].items() if grids[k][d] == g]
+            nd = max(nd, len(ds))
+            cells.append("%.3g" % (sum(ds) / len(ds)) if ds else "-")
+        out.append("| `%s` | %d | %d | %s |" % (k, g, nd, " | ".join(cells)))
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m mps3d_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -174,8 +208,14 @@ def main(argv=None):
     tt.add_argument("fetch_csv")
     tt.add_argument("write_csv")
     tt.add_argument("kernels", nargs="+", help="kernel name filters")
+    ct = sub.add_parser("counters")
+    ct.add_argument("--csv", action="append", required=True, help="a --pmc counter_collection CSV (repeatable)")
+    ct.add_argument("--grid", type=int, default=None)
+    ct.add_argument("kernels", nargs="+", help="kernel name filters")
     a = ap.parse_args(argv)
-    if a.cmd == "table":
+    if a.cmd == "counters":
+        print(counter_table(a.csv, a.kernels, a.grid))
+    elif a.cmd == "table":
         print(traffic_table(a.fetch_csv, a.write_csv, a.kernels))
     elif a.cmd == "stats":
         print(stats_table(a.stats_csv, a.trace_csv))

@@ -531,6 +531,13 @@ class FieldSolver:
         M.check(self.lib.mps_last_timing(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def last_vcycles(self):
+        """V-cycles of the last solve (FP32 ones under fp32dc; a full-multigrid
+        start counts as one)."""
+        n = C.c_int()
+        M.check(self.lib.mps_last_vcycles(self._h, C.byref(n)))
+        return n.value
+
     def time_kernel(self, kernel=0, fp32=False, reps=20):
         """Mean launch time (ms), algorithmic bytes per launch and symbol name of
         one hot kernel (0 colour pass, 1 residual, 2 restriction, 3 prolongation)."""

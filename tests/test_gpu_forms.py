@@ -75,12 +75,13 @@ def test_finest_operator_is_the_reference_operator_in_aligned_mode(gpu):
 @pytest.mark.parametrize("hier,n", [("aligned", 65), ("ref", 64)])
 def test_fp32_defect_correction_matches_fp64(gpu, hier, n):
     """FP32 smoothing under FP64 defect correction reaches the same 1e-10
-    relative residual and the same solution as the plain FP64 solve."""
+    relative residual and the same solution as the plain FP64 solve, with
+    two FP32 cycles per defect step and with the count chosen per step."""
     c = W.varcoef_poisson(n)
     sol = {}
-    for prec in ("fp64", "fp32dc"):
+    for prec, inner in (("fp64", None), ("fp32dc", 2), ("fp32dc", 0)):
         with FieldSolver(c.n, c.length, hierarchy=hier, precision=prec, max_cycles=60, tol=1e-10,
-                         tol_kind="rel") as s:
+                         tol_kind="rel", dc_inner=inner) as s:
             s.set_case(c)
             u = np.zeros(s.shape, order="F")
             k, r, h, st = s.solve(u, raise_on_failure=True)
@@ -88,12 +89,22 @@ def test_fp32_defect_correction_matches_fp64(gpu, hier, n):
             ax, _, _ = s.find_AX(u)
             b = s.find_bvec(None, True)
             assert np.linalg.norm(b - ax) / np.linalg.norm(b) <= 1e-10  # the FP64 residual, recomputed
-            sol[prec] = (u[1:-1, 1:-1, 1:-1].copy(), k)
-    a, b = sol["fp32dc"][0], sol["fp64"][0]
-    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-9
-    # two FP32 cycles per FP64 defect step (the default): as many V-cycles
-    # as the FP64 solve, within a step
-    assert abs(2 * sol["fp32dc"][1] - sol["fp64"][1]) <= 2
+            sol[prec, inner] = (u[1:-1, 1:-1, 1:-1].copy(), k, s.last_vcycles())
+    ref = sol["fp64", None]
+    assert ref[2] == ref[1]                        # FP64: one V-cycle per cycle
+    for inner in (2, 0):
+        a = sol["fp32dc", inner]
+        assert np.linalg.norm(a[0] - ref[0]) / np.linalg.norm(ref[0]) < 1e-9
+    # two FP32 cycles per FP64 defect step: as many V-cycles as the FP64
+    # solve, within a step
+    two = sol["fp32dc", 2]
+    assert two[2] == 2 * two[1]
+    assert abs(two[2] - ref[1]) <= 2
+    # chosen per step: no more FP64 defect steps than with two cycles each,
+    # and at most one FP32 cycle more in all
+    ad = sol["fp32dc", 0]
+    assert ad[1] <= two[1]
+    assert ad[2] <= two[2] + 1
 
 
 @pytest.mark.parametrize("scale", [1e-46, 1e-30, 1e30, 1e45])
