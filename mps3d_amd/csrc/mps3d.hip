@@ -556,6 +556,49 @@ void stage_in(mps_ctx *c, const double *src, double *dst, long long n)
         HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
 }
 
+// up to six same-length device-to-device copies (a NULL source: zero fill)
+// in one launch, blockIdx.y picking the array
+struct CopySet {
+    const double *src[6];
+    double *dst[6];
+};
+__global__ void __launch_bounds__(kBlock) k_copy_set(CopySet S, long long n)
+{
+    const double *s = S.src[blockIdx.y];
+    double *d = S.dst[blockIdx.y];
+    for (long long t = blockIdx.x * (long long)kBlock + threadIdx.x; t < n; t += (long long)gridDim.x * kBlock)
+        d[t] = s ? s[t] : 0.0;
+}
+
+// collects stage_in's device copies (and zero fills) of one length into one
+// k_copy_set launch; host sources are copied at once
+struct StageSet {
+    mps_ctx *c;
+    long long n;
+    CopySet S{};
+    int k = 0;
+    StageSet(mps_ctx *c_, long long n_) : c(c_), n(n_) {}
+    void add(const double *src, double *dst)
+    {
+        if (src && !is_device_ptr(src)) {
+            HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            return;
+        }
+        if (k == 6) flush();
+        S.src[k] = src;
+        S.dst[k] = dst;
+        ++k;
+    }
+    void flush()
+    {
+        if (k == 0) return;
+        const long long blocks = std::min<long long>((n + kBlock - 1) / kBlock, 4096);
+        hipLaunchKernelGGL(k_copy_set, dim3((unsigned)blocks, (unsigned)k), dim3(kBlock), 0, c->stream, S, n);
+        HIPCHK(hipGetLastError());
+        k = 0;
+    }
+};
+
 void stage_out(mps_ctx *c, const double *src, double *dst, long long n)
 {
     if (is_device_ptr(dst))
@@ -803,10 +846,11 @@ void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *wo
     // err_tol = ERRTOLR*0.0001 with a default-real literal (mgridsteps.f90:305)
     const T tol = (T)(1e-11 * (double)0.0001f);
     const int m = (int)c->geom[l].n();
+    const dim3 bb = block();
     with_op(op, c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
         if (m <= kBottomLdsM)
             hipLaunchKernelGGL((k_bottom<T, decltype(O)::value, decltype(V)::value, decltype(R)::value, true>),
-                               dim3(1), block(), 0, c->stream, v, L[l].u, (const T *)L[l].b, work, m, 1, tol,
+                               dim3(1), bb, 0, c->stream, v, L[l].u, (const T *)L[l].b, work, m, 1, tol,
                                c->d_status);
         else
             hipLaunchKernelGGL((k_bottom<T, decltype(O)::value, decltype(V)::value, decltype(R)::value, false>),
@@ -1000,6 +1044,16 @@ void smooth_level(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int nu, int op
             std::swap(L[l].u, L[l].u2);
             ++s;
         }
+    }
+    const Geom &gs = c->geom[l];
+    if (s < nu && gs.n() <= kSmallSmooth) {
+        // the remaining sweeps of a small level in one launch
+        Lev<T> v = view(c, L, l);
+        with_op(op, c->has_vel, c->has_reac, [&](auto O, auto V, auto R) {
+            hipLaunchKernelGGL((k_colour_small<T, decltype(O)::value, decltype(V)::value, decltype(R)::value>),
+                               dim3(1), dim3(kSmallBlock), 0, c->stream, v, L[l].u, (const T *)L[l].b, nu - s);
+        });
+        s = nu;
     }
     for (; s < nu; ++s) {
         launch_colour<T>(c, L, l, 1, op);
@@ -1721,7 +1775,11 @@ void build_bottom_inverse_device(mps_ctx *c)
                            cells_grid(g), block(), 0, c->stream, v, c->bst[0], c->bst[1], c->bst[2], c->bst[3],
                            c->bst[4], c->bst[5], c->bst[6], 0);
     });
-    hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(kDenseThreads), 0, c->stream, v, (const double *)c->bst[0],
+#ifndef MPS_DENSE_LDS_THREADS
+#define MPS_DENSE_LDS_THREADS 1024
+#endif
+    const unsigned nth = n <= kDenseLdsN ? MPS_DENSE_LDS_THREADS : kDenseThreads;
+    hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(nth), 0, c->stream, v, (const double *)c->bst[0],
                        (const double *)c->bst[1], (const double *)c->bst[2], (const double *)c->bst[3],
                        (const double *)c->bst[4], (const double *)c->bst[5], (const double *)c->bst[6], c->dwork,
                        c->dinv, (int *)nullptr);
@@ -2017,6 +2075,44 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
     return MPS_OK;
 }
 
+// mps_set_bc's device work in one launch, one workgroup per face: the
+// caller's face (NULL: zeros) into the context's copy, then restrict_boundary
+// (mgridsteps.f90:55-128) level by level, the 4-child mean in the host
+// loop's order (as k_restrict_face), with a block barrier between levels
+constexpr int kFaceLev = 12;
+struct FaceSet {
+    const double *src[6];
+    double *d0[6];
+    int n0[6];
+    int nl;
+    int fa[kFaceLev][6], ca[kFaceLev][6], cb[kFaceLev][6];
+    double *c64[kFaceLev][6];
+    float *c32[kFaceLev][6];
+};
+__global__ void __launch_bounds__(1024) k_set_faces(FaceSet S)
+{
+    const int f = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const double *src = S.src[f];
+    double *d0 = S.d0[f];
+    for (int t = tid; t < S.n0[f]; t += nt) d0[t] = src ? src[t] : 0.0;
+    for (int l = 1; l < S.nl; ++l) {
+        __syncthreads();
+        const double *F = l == 1 ? d0 : S.c64[l - 1][f];
+        const int fa = S.fa[l][f], ca = S.ca[l][f], cb = S.cb[l][f];
+        double *C = S.c64[l][f];
+        float *c32 = S.c32[l][f];
+        for (int t = tid; t < ca * cb; t += nt) {
+            const int p = t % ca + 1, q = t / ca + 1;
+            const double v = 0.25 * (F[(2 * p - 2) + (size_t)fa * (2 * q - 2)] +
+                                     F[(2 * p - 1) + (size_t)fa * (2 * q - 2)] +
+                                     F[(2 * p - 2) + (size_t)fa * (2 * q - 1)] +
+                                     F[(2 * p - 1) + (size_t)fa * (2 * q - 1)]);
+            C[t] = v;
+            if (c32) c32[t] = (float)v;
+        }
+    }
+}
+
 // restrict_boundary (mgridsteps.f90:55-128) of one face on the device: the
 // 4-child mean of an (fa, fb) face onto (ca, cb), in the host loop's order;
 // c32 (nullable) gets the FP32 copy
@@ -2040,6 +2136,43 @@ void set_bc_device(mps_ctx *c, const double *const faces[6])
     const Geom &g0 = c->geom[0];
     const long long fsz[6] = {(long long)g0.ny * g0.nz, (long long)g0.ny * g0.nz, (long long)g0.nx * g0.nz,
                               (long long)g0.nx * g0.nz, (long long)g0.nx * g0.ny, (long long)g0.nx * g0.ny};
+    const bool ref = c->opts.hierarchy == MPS_HIER_REF;
+    if (nl <= kFaceLev && fsz[0] < (1LL << 31) && fsz[2] < (1LL << 31) && fsz[4] < (1LL << 31)) {
+        // one launch: copies and every coarse level's faces
+        FaceSet S{};
+        S.nl = ref ? nl : 1;
+        for (int f = 0; f < 6; ++f) {
+            if (!c->faces0[f]) c->faces0[f] = dalloc<double>(fsz[f]);
+            S.src[f] = faces ? faces[f] : nullptr;
+            S.d0[f] = c->faces0[f];
+            S.n0[f] = (int)fsz[f];
+        }
+        for (int l = 1; l < S.nl; ++l) {
+            const Geom &gf = c->geom[l - 1], &gc = c->geom[l];
+            const int fa[6] = {gf.ny, gf.ny, gf.nx, gf.nx, gf.nx, gf.nx};
+            const int ca[6] = {gc.ny, gc.ny, gc.nx, gc.nx, gc.nx, gc.nx};
+            const int cb[6] = {gc.nz, gc.nz, gc.nz, gc.nz, gc.ny, gc.ny};
+            for (int f = 0; f < 6; ++f) {
+                const size_t nc = (size_t)ca[f] * cb[f];
+                auto &B64 = c->L64[l];
+                if (!B64.faces[f]) B64.faces[f] = dalloc<double>(nc);
+                float *c32 = nullptr;
+                if (c->opts.precision == MPS_PREC_FP32_DC) {
+                    auto &B32 = c->L32[l];
+                    if (!B32.faces[f]) B32.faces[f] = dalloc<float>(nc);
+                    c32 = B32.faces[f];
+                }
+                S.fa[l][f] = fa[f];
+                S.ca[l][f] = ca[f];
+                S.cb[l][f] = cb[f];
+                S.c64[l][f] = B64.faces[f];
+                S.c32[l][f] = c32;
+            }
+        }
+        hipLaunchKernelGGL(k_set_faces, dim3(6), dim3(1024), 0, c->stream, S);
+        HIPCHK(hipGetLastError());
+        return;
+    }
     for (int f = 0; f < 6; ++f) {
         if (!c->faces0[f]) c->faces0[f] = dalloc<double>(fsz[f]);
         if (faces && faces[f])
@@ -2180,8 +2313,6 @@ int mps_set_coeffs(mps_ctx *c, const double *dcoeff, const double *velx, const d
         c->has_reac = reac != nullptr;
         c->timederiv = timederiv != 0;
         c->dt = dt;
-        if (dcoeff) stage_in(c, dcoeff, L[0].D, ng);
-        else fill<double>(c, L[0].D, ng, 0.0);
         const double *vsrc[3] = {velx, vely, velz};
         for (int l = 0; l < c->nlev(); ++l) {
             double **vd[4] = {&L[l].vx, &L[l].vy, &L[l].vz, &L[l].reac};
@@ -2202,15 +2333,19 @@ int mps_set_coeffs(mps_ctx *c, const double *dcoeff, const double *velx, const d
                 }
             }
         }
-        if (c->has_vel)
-            for (int d = 0; d < 3; ++d) {
-                double *dst = d == 0 ? L[0].vx : d == 1 ? L[0].vy : L[0].vz;
-                if (vsrc[d]) stage_in(c, vsrc[d], dst, ng);
-                else fill<double>(c, dst, ng, 0.0);
+        {
+            // the finest level's arrays (absent ones zero) in one launch
+            StageSet S(c, ng);
+            S.add(dcoeff, L[0].D);
+            if (c->has_vel) {
+                S.add(vsrc[0], L[0].vx);
+                S.add(vsrc[1], L[0].vy);
+                S.add(vsrc[2], L[0].vz);
             }
-        if (c->has_reac) stage_in(c, reac, L[0].reac, ng);
-        if (source) stage_in(c, source, c->source, ng);
-        else fill<double>(c, c->source, ng, 0.0);
+            if (c->has_reac) S.add(reac, L[0].reac);
+            S.add(source, c->source);
+            S.flush();
+        }
         derive_coarse(c);
         conductance_coarse_levels(c);
         for (int l = 0; l < c->nlev(); ++l) {

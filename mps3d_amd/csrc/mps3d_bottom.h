@@ -544,16 +544,16 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_inverse(Lev<double> L, 
         // LDS-resident matrix, one column per lane. Wave 0 alone finds the
         // pivot (a butterfly over the candidate rows, one per lane), swaps
         // the rows and scales the pivot row, with no barrier between those
-        // steps; then every wave updates its rows (wave w: rows w + 16 m),
-        // each lane loading all its values before it stores. Two barriers
+        // steps; then every wave updates its rows (wave w of nw: rows
+        // w + nw m), each lane loading a batch of rows before it stores them
+        // (any whole number of waves: the host picks it). Two barriers
         // per pivot. The row swaps are recorded as the original row index
         // at each position (pos, one per lane of wave 0) and undone at the
         // end by scattering each row's columns to their original places,
         // which moves the same values as swapping the columns back, last
-        // swap first. Launched with kDenseThreads threads (16 waves, four
-        // rows each).
-        constexpr int kW = kDenseThreads / 64, kRows = kDenseLdsN / kW;
-        const int lane = tid & 63, wv = tid >> 6;
+        // swap first.
+        const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+        constexpr int kRows = 4;   // rows per wave and batch: all loads of a batch before its stores
         int pos = lane;
         for (int kk = 0; kk < n; ++kk) {
             if (wv == 0) {
@@ -588,27 +588,26 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_inverse(Lev<double> L, 
             }
             __syncthreads();
             const double rkc = lane < n ? sM[kk * n + lane] : 0.0;
-            double f[kRows], x[kRows];
+            for (int r0 = wv; r0 < n; r0 += kRows * nw) {
+                double f[kRows], x[kRows];
 #pragma unroll
-            for (int m = 0; m < kRows; ++m) {
-                const int r = wv + m * kW;
-                f[m] = r < n ? sM[r * n + kk] : 0.0;
-                x[m] = (r < n && lane < n) ? sM[r * n + lane] : 0.0;
-            }
+                for (int m = 0; m < kRows; ++m) {
+                    const int r = r0 + m * nw;
+                    f[m] = r < n ? sM[r * n + kk] : 0.0;
+                    x[m] = (r < n && lane < n) ? sM[r * n + lane] : 0.0;
+                }
 #pragma unroll
-            for (int m = 0; m < kRows; ++m) {
-                const int r = wv + m * kW;
-                if (r < n && r != kk && lane < n) sM[r * n + lane] = (lane == kk ? 0.0 : x[m]) - f[m] * rkc;
+                for (int m = 0; m < kRows; ++m) {
+                    const int r = r0 + m * nw;
+                    if (r < n && r != kk && lane < n) sM[r * n + lane] = (lane == kk ? 0.0 : x[m]) - f[m] * rkc;
+                }
             }
             __syncthreads();
         }
         if (wv == 0 && lane < n) perm[lane] = pos;
         __syncthreads();
-#pragma unroll
-        for (int m = 0; m < kRows; ++m) {
-            const int r = wv + m * kW;
-            if (r < n && lane < n) Ainv[r * n + perm[lane]] = sM[r * n + lane];
-        }
+        for (int r = wv; r < n; r += nw)
+            if (lane < n) Ainv[r * n + perm[lane]] = sM[r * n + lane];
         return;
     }
     double *const M = gwork;

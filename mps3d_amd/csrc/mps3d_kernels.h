@@ -460,6 +460,48 @@ __global__ void __launch_bounds__(kBlock) k_colour(Lev<T> L, T *__restrict__ u, 
     u[c] = gs_update(L, b[c], of, dg);
 }
 
+// nu red-black sweeps of a small level in one workgroup: k_colour's passes
+// (red, then black, nu times) with block barriers between them instead of
+// launches, each cell updated by the same arithmetic. Each lane takes the
+// pass's colour cells (k_colour's compact numbering) two at a time and forms
+// both values before it stores either, so their loads are in flight
+// together. For levels of at most kSmallSmooth cells, where a pass is a few
+// microseconds of launch.
+constexpr int kSmallSmooth = 8192;
+constexpr int kSmallBlock = 1024;
+template <typename T, int OP, bool VEL, bool REAC>
+__global__ void __launch_bounds__(kSmallBlock) k_colour_small(Lev<T> L, T *__restrict__ u, const T *__restrict__ b,
+                                                             int nu)
+{
+    const int hx = (L.nx + 1) / 2, ncol = hx * L.ny * L.nz;
+    // the new value of colour cell t (c = -1: none)
+    auto cell = [&](int t, int red, long long &c, T &v) {
+        c = -1;
+        if (t >= ncol) return;
+        const int rw = t / hx;   // (j - 1) + ny (k - 1)
+        const int k = rw / L.ny + 1, j = rw - (k - 1) * L.ny + 1;
+        const int i = (((1 + j + k) & 1) == red ? 1 : 2) + 2 * (t - rw * hx);
+        if (i > L.nx) return;
+        c = (long long)i + (long long)L.sy * j + L.sz * k;
+        T dg, of;
+        row<T, OP, VEL, REAC>(L, u, i, j, k, c, dg, of);
+        v = gs_update(L, b[c], of, dg);
+    };
+    for (int s = 0; s < nu; ++s) {
+        for (int red = 1; red >= 0; --red) {
+            for (int t = threadIdx.x; t < ncol; t += 2 * kSmallBlock) {
+                long long c0, c1;
+                T v0 = T(0), v1 = T(0);
+                cell(t, red, c0, v0);
+                cell(t + kSmallBlock, red, c1, v1);
+                if (c0 >= 0) u[c0] = v0;
+                if (c1 >= 0) u[c1] = v1;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // r = b - (diag*u + off); writes r (type R, ghosted layout, nullable) and the
 // block's partial sum of r^2 (nullable). When R is narrower than T, the
 // stored r is scaled by rscale[0] (a power of two, nullable = 1) so that
