@@ -135,6 +135,14 @@ typedef struct {
                                    more than keep a step's reduction above
                                    1e-5; the first step (and one that did
                                    not contract) takes 2                       */
+    double omega;               /* over-relaxation of the red-black sweeps of
+                                   mps_solve's V-cycles in MPS_ARITH_FAST:
+                                   u = u_old + omega (u_GS - u_old); 0
+                                   (default) = 1.15 on MPS_HIER_ALIGNED, 1 on
+                                   MPS_HIER_REF; else 0 < omega < 2. Exact
+                                   arithmetic, mps_smooth and mps_vcycle
+                                   always sweep with 1 (the reference's
+                                   Gauss-Seidel)                               */
 } mps_opts;
 
 typedef struct mps_ctx mps_ctx;

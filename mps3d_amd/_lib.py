@@ -61,7 +61,7 @@ class Opts(C.Structure):
     _fields_ = [("hierarchy", C.c_int), ("precision", C.c_int), ("flux", C.c_int),
                 ("nu1", C.c_int), ("nu2", C.c_int), ("max_cycles", C.c_int),
                 ("tol", C.c_double), ("tol_kind", C.c_int), ("arith", C.c_int), ("start", C.c_int),
-                ("dc_inner", C.c_int)]
+                ("dc_inner", C.c_int), ("omega", C.c_double)]
 
 
 _lib = None

@@ -210,6 +210,9 @@ struct mps_ctx {
     bool graphs = true;                  // MPS3D_NO_GRAPHS=1 turns replay off
     bool fused_rr = true;                // MPS3D_NO_FUSED_RESTRICT=1: residual and restriction apart
     bool sweep_ref = false;              // k_sweep_ref for OP_REF rows (MPS3D_SWEEP_REF=1: on)
+    double omega = 1.0;                  // fast-sweep over-relaxation of solves (opts.omega;
+                                         // MPS3D_OMEGA overrides)
+    bool relax_now = false;              // a solve (solve_core) is running: omega applies
     int sweep2 = 1;                      // precisions whose smoothing pairs run in one pass
                                          // (bit 1 FP32, bit 2 FP64; MPS3D_SWEEP2 overrides)
     StepGraph step[2 + kDcInnerMax];
@@ -298,6 +301,7 @@ Lev<T> view(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int op_o
     v.gyb = (T)(g.iy0 * g.ayb);
     v.gzb = (T)(g.iz0 * g.azb);
     v.dtinv = c->timederiv ? (T)(1.0 / c->dt) : T(0);
+    v.omega = (c->relax_now && !c->exact_now && c->opts.arith == MPS_ARITH_FAST) ? (T)c->omega : T(1);
     return v;
 }
 
@@ -897,7 +901,8 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
     const int kz = z_chunk(g, kSweepTX, kSweepTY, MPS_SWEEP_TARGET, kSweepKZ);
     dim3 gr((unsigned)((g.nx + kSweepTX - 1) / kSweepTX), (unsigned)((g.ny + kSweepTY - 1) / kSweepTY),
             (unsigned)((g.nz + kz - 1) / kz));
-    if (v.fast && op == OP_REF && !c->has_reac && c->sweep_ref && g.ng() * (long long)sizeof(T) < (1LL << 31)) {
+    if (v.fast && op == OP_REF && !c->has_reac && c->sweep_ref && v.omega == T(1) &&
+        g.ng() * (long long)sizeof(T) < (1LL << 31)) {
         // uniform face factors: boundary rows through staged ghost coefficients
         GhostD<T> G = ghost_multipliers(c, v);
         if (zero_in)
@@ -996,7 +1001,9 @@ bool sweep2_ok(const mps_ctx *c, const std::vector<LevelBuf<T>> &L, int l, int o
     if (!(c->sweep2 & (sizeof(T) == 4 ? 1 : 2)) || op != OP_REF || c->has_reac || c->has_vel || !L[l].u2)
         return false;
     if (c->geom[l].ng() * (long long)sizeof(T) >= (1LL << 31)) return false;
-    return view(c, L, l).fast != 0;
+    // (the pair kernel sweeps with plain Gauss-Seidel only)
+    const Lev<T> v = view(c, L, l);
+    return v.fast != 0 && v.omega == T(1);
 }
 
 template <typename T>
@@ -1697,6 +1704,11 @@ int dc_inner_next(double prev, double now, int k, double tol)
 // as one).
 int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, double &meas)
 {
+    struct RelaxScope {
+        mps_ctx *c;
+        explicit RelaxScope(mps_ctx *c_) : c(c_) { c->relax_now = true; }
+        ~RelaxScope() { c->relax_now = false; }
+    } relax(c);
     double bn2 = 0.0;
     const bool dc = c->opts.precision == MPS_PREC_FP32_DC;
     if (c->opts.tol_kind == MPS_TOL_REL_L2 || dc) {
@@ -1912,6 +1924,7 @@ void mps_default_opts(mps_opts *o)
     o->arith = MPS_ARITH_EXACT;
     o->start = MPS_START_GUESS;
     o->dc_inner = 0;
+    o->omega = 0.0;
 }
 
 const char *mps_error_string(int s)
@@ -1999,6 +2012,16 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
         delete c;
         return fail(MPS_ERR_ARG, "dc_inner must be 0 (chosen per step) or 1..4");
     }
+    if (!(c->opts.omega >= 0.0 && c->opts.omega < 2.0)) {
+        delete c;
+        return fail(MPS_ERR_ARG, "omega must be 0 (default) or in (0, 2)");
+    }
+    // over-relaxed red-black Gauss-Seidel for the solves' fast sweeps (red-
+    // black SOR smooths better than plain Gauss-Seidel for omega a little
+    // above 1: I. Yavneh, SIAM J. Sci. Comput. 17 (1996) 180-192); on the
+    // 257^3 var-coef Poisson 1.1, 1.15, 1.2, 1.3 took 7 -> 6, 5, 5, 5
+    // V-cycles to 1e-10
+    c->omega = c->opts.omega > 0.0 ? c->opts.omega : (c->opts.hierarchy == MPS_HIER_ALIGNED ? 1.15 : 1.0);
     if (c->opts.start != MPS_START_GUESS &&
         (c->opts.start != MPS_START_FMG || c->opts.hierarchy != MPS_HIER_ALIGNED)) {
         delete c;
@@ -2019,6 +2042,11 @@ int mps_create(mps_ctx **out, const mps_grid *grid, const mps_opts *opts)
             c->fused_rr = !(nf && nf[0] == '1');
             const char *sr = std::getenv("MPS3D_SWEEP_REF");
             c->sweep_ref = sr && sr[0] == '1';
+            const char *om = std::getenv("MPS3D_OMEGA");
+            if (om) {
+                const double w = std::atof(om);
+                if (w > 0.0 && w < 2.0) c->omega = w;
+            }
             const char *n2 = std::getenv("MPS3D_SWEEP2");
             c->sweep2 = (n2 && n2[0] >= '0' && n2[0] <= '3') ? n2[0] - '0' : MPS_SWEEP2_PREC;
         }

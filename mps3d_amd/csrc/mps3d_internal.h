@@ -61,6 +61,9 @@ struct Lev {
     T hx, hy, hz, hxb, hyb, hzb;        // interior / boundary-cell face factors
     T gx, gy, gz, gxb, gyb, gzb;        // DIRC wall factors (interior unused)
     T dtinv;
+    // over-relaxation of the fast sweeps (k_sweep_fast): u = u_old +
+    // omega (u_GS - u_old); 1 = plain Gauss-Seidel
+    T omega;
 };
 
 // Per-level, per-dimension restriction weights of the aligned hierarchy:

@@ -672,6 +672,7 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
                                    u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], C.fxm, C.fxp, rfym, rfyp, rfzm,
                                    rfzp, ZGHOST ? C.g : C.g + rgz, rc, br);
                 }
+                if (L.omega != T(1)) val = u0[e] + L.omega * (val - u0[e]);
                 if (s >= kb && s <= ke) buf_store<T>(val, Ro, C.vs, pofs(s));
             }
             sr[u0_][C.q] = val;
@@ -697,6 +698,7 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
                              r0[q - W1], r0[q + W1], rm[q], rp[q], Bc.fxm, Bc.fxp, bfym, bfyp, bfzm, bfzp,
                              ZGHOST ? Bc.g : Bc.g + bgz, rc, bk);
             }
+            if (L.omega != T(1)) v = su[um_][e] + L.omega * (v - su[um_][e]);
             buf_store<T>(v, Ro, Bc.vs, pofs(k));
         }
     };

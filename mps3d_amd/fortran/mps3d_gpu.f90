@@ -42,6 +42,7 @@ module mps3d_gpu
         integer(c_int) :: hierarchy, precision, flux, nu1, nu2, max_cycles
         real(c_double) :: tol
         integer(c_int) :: tol_kind, arith, start, dc_inner
+        real(c_double) :: omega
     end type mps_opts
 
     interface

@@ -90,7 +90,7 @@ class FieldSolver:
 
     def __init__(self, n, length=(1.0, 1.0, 1.0), p=(1, 1, 1), hierarchy="ref", precision="fp64",
                  flux="upwind", nu1=2, nu2=2, max_cycles=20, tol=1e-11, tol_kind="rms", arith="exact",
-                 start="guess", dc_inner=None):
+                 start="guess", dc_inner=None, omega=None):
         self.lib = M.load()
         self.n = tuple(int(v) for v in (n if not np.isscalar(n) else (n, n, n)))
         self.length = tuple(float(v) for v in (length if not np.isscalar(length) else (length,) * 3))
@@ -108,6 +108,8 @@ class FieldSolver:
         o.start = {"guess": 0, "fmg": 1}[start] if isinstance(start, str) else int(start)
         if dc_inner is not None:
             o.dc_inner = int(dc_inner)
+        if omega is not None:
+            o.omega = float(omega)
         self.opts = o
         self.max_cycles = int(max_cycles)
         h = C.c_void_p()

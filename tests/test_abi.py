@@ -38,6 +38,7 @@ def test_default_options_are_the_reference_constants():
     assert (o.hierarchy, o.precision, o.flux, o.tol_kind) == (M.HIER_REF, M.PREC_FP64, M.FLUX_UPWIND, M.TOL_RMS)
     assert (o.arith, o.start) == (0, 0)  # exact rows, V-cycles from the caller's guess
     assert o.dc_inner == 0                # FP32 cycles per FP64 defect step: chosen per step
+    assert o.omega == 0.0                 # fast solves' sweep relaxation: the library's default
 
 
 def test_error_strings():

@@ -76,7 +76,7 @@ def test_fp32dc_with_one_pass_pairs_follows_single_sweeps(gpu, monkeypatch, n):
     out = {}
     for mask in (0, 1):
         s = _solver(monkeypatch, mask, c.n, c.length, hierarchy="aligned", precision="fp32dc", max_cycles=40,
-                    tol=1e-10, tol_kind="rel", arith="fast", start="fmg", dc_inner=2)
+                    tol=1e-10, tol_kind="rel", arith="fast", start="fmg", dc_inner=2, omega=1.0)
         try:
             s.set_case(c)
             u = np.zeros(s.shape, order="F")
