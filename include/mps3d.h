@@ -133,8 +133,9 @@ typedef struct {
                                    that reach tol at the contraction the
                                    previous step showed, at most 4 and no
                                    more than keep a step's reduction above
-                                   1e-5; the first step (and one that did
-                                   not contract) takes 2                       */
+                                   1e-5; the first step assumes 0.1 per
+                                   cycle, a step after one that did not
+                                   contract takes 2                            */
     double omega;               /* over-relaxation of the red-black sweeps of
                                    mps_solve's V-cycles in MPS_ARITH_FAST:
                                    u = u_old + omega (u_GS - u_old); 0

@@ -1697,6 +1697,17 @@ int dc_inner_next(double prev, double now, int k, double tol)
     return std::max(1, std::min(q, kDcInnerMax));
 }
 
+// FP32 V-cycles for the first defect-correction step (no contraction seen
+// yet): the fewest that would reach tol at a nominal 0.1 per cycle (a
+// multigrid V-cycle's textbook contraction; the steps after it use the
+// measured one), within 1..kDcInnerMax.
+int dc_inner_first(double now, double tol)
+{
+    if (!(now > tol && tol > 0.0)) return 2;
+    const double need = std::log(tol / now) / std::log(0.1);
+    return std::max(1, std::min((int)std::ceil(std::min(need, kDcInnerMax + 1.0)), kDcInnerMax));
+}
+
 // The solve loop shared by the forward and the self-adjoint solves: V-cycles
 // (FP64) or FP32 defect-correction steps on u for right-hand side b until
 // the opts.tol_kind measure <= opts.tol. Returns the cycles run (outer
@@ -1731,11 +1742,13 @@ int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, do
         meas = measure(c, read_scalar(c), bn2);
         const bool adapt = c->opts.dc_inner == 0;
         int inner = adapt ? 2 : c->opts.dc_inner;
+        bool contracted = false;   // a defect step's contraction has been seen
         if (!(meas <= c->opts.tol)) {
             for (it = 0; it < c->opts.max_cycles;) {
                 // the FP32 cycles on the defect, the FP64 correction, then the
                 // FP64 residual of the corrected iterate (the next defect)
                 const bool fmg_first = it == 0 && c->opts.start == MPS_START_FMG;
+                if (adapt && !fmg_first && !contracted) inner = dc_inner_first(meas, c->opts.tol);
                 outer_step(c, u, b, homog, true, fmg_first, inner);
                 c->last_vcycles += fmg_first ? 1 : inner;
                 const double prev = meas;
@@ -1744,7 +1757,10 @@ int solve_core(mps_ctx *c, double *u, double *b, bool homog, double *history, do
                 ++it;
                 if (!std::isfinite(meas)) break;
                 if (meas <= c->opts.tol) break;
-                if (adapt && !fmg_first) inner = dc_inner_next(prev, meas, inner, c->opts.tol);
+                if (adapt && !fmg_first) {
+                    inner = dc_inner_next(prev, meas, inner, c->opts.tol);
+                    contracted = true;
+                }
             }
         }
     } else {
