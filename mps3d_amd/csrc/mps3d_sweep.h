@@ -23,6 +23,10 @@
 
 #include "mps3d_kernels.h"
 
+#ifndef MPS_SWEEP_ONEBAR
+#define MPS_SWEEP_ONEBAR 0
+#endif
+
 namespace mps {
 
 template <typename T, int OP, bool REAC, int TX, int TY>
@@ -277,7 +281,13 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     static_assert(TX % 2 == 0, "tile width must be even");
     static_assert(NR <= NT, "one red cell per lane");
     static_assert((TX / 2 + 1) * (TY + 1) <= NT, "one black cell per lane, extended tile included");
-    constexpr int RU = 4, RD = 8;             // ring depths (powers of two: slots are t & (R - 1))
+    // OB (FP32, MPS_SWEEP_ONEBAR): one barrier per z-step, the black pass two
+    // planes behind the red one in the same barrier interval (so the u and
+    // reac rings hold 8 planes: the next plane is staged while other waves
+    // still read planes s - 4 .. s); else two barriers, black one plane
+    // behind
+    constexpr bool OB = MPS_SWEEP_ONEBAR && sizeof(T) == 4;
+    constexpr int RU = OB ? 8 : 4, RD = 8, RR = 4;   // ring depths (powers of two: slots are t & (R - 1))
     constexpr bool STN = OP == OP_STENCIL || OP == OP_FACES;  // explicit 7-point rows (no D tiles)
     // OP_FACES: the rows' face form (L.fc): per cell the diagonal and the
     // conductances of its six faces (the -, + faces of each direction, the
@@ -295,7 +305,7 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     __shared__ T su[RU][P2];
     __shared__ T sD[STN ? 1 : RD][STN ? 1 : P2];
     __shared__ T sq[REAC ? RU : 1][REAC ? P1 : 1];
-    __shared__ T sr[RU][P1 + W1];   // (one spare row: an extended tile's last black row reads one below)
+    __shared__ T sr[RR][P1 + W1];   // (one spare row: an extended tile's last black row reads one below)
 
     const int tid = threadIdx.x;
     int bx, by, bz;
@@ -319,7 +329,7 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     // written (each slot holds planes of one parity), and an extended
     // tile's black cells on column nx or row ny read one position past
     // their row or plane there, with a zero face factor.
-    for (int q = tid; q < RU * (P1 + W1); q += NT) (&sr[0][0])[q] = T(0);
+    for (int q = tid; q < RR * (P1 + W1); q += NT) (&sr[0][0])[q] = T(0);
 
     // Every global access goes through buffer resources over the planes
     // this block touches, pl0..pl1 (its chunk and two planes either side,
@@ -460,7 +470,7 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         return pc ? cell(bq1, be1, cb1, true, bfxm1, bfxp1, bgxy1, bi1, bj)
                   : cell(bq0, be0, cb0, true, bfxm0, bfxp0, bgxy0, bi0, bj);
     };
-    const int pr = (rpar + kb - 1) & 1, pbk = (bpar + kb - 2) & 1;
+    const int pr = (rpar + kb - 1) & 1, pbk = (bpar + kb - 2 - (OB ? 1 : 0)) & 1;
     const Cell R0 = red_cell(pr), R1 = red_cell(pr ^ 1);
     const Cell B0 = blk_cell(pbk), B1 = blk_cell(pbk ^ 1);
 
@@ -494,7 +504,8 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         }
         // (gload(kp) fills set (kp - kb) & 1, the slot of step kp - 1 that
         // consumes it; outside that pattern the b values are not consumed)
-        const int pr = pofs(kp - 1), pb = pofs(kp - 2);
+        // (the black plane of step kp - 1 is kp - 2, or kp - 3 with OB)
+        const int pr = pofs(kp - 1), pb = pofs(kp - 2 - (OB ? 1 : 0));
         const int vr_ = x ? R1.vb : R0.vb;
         const int vb_ = x ? B1.vb : B0.vb;
         const T vr = buf_load<T>(Rb, vr_, pr);
@@ -502,7 +513,7 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         if constexpr (FCS) {
             // the diagonal and the six face conductances, in ST_* order
             const Cell &CR = x ? R1 : R0, &CB = x ? B1 : B0;
-            const int pr2 = pofs(kp - 2), pb2 = pofs(kp - 3);
+            const int pr2 = pofs(kp - 2), pb2 = pofs(kp - 3 - (OB ? 1 : 0));
             T *ra = x ? rs1 : rs0, *ba = x ? bs1 : bs0;
             ra[ST_C] = buf_load<T>(Rs[0], vr_, pr);
             ra[ST_XM] = buf_load<T>(Rs[1], CR.vxm, pr);
@@ -628,12 +639,19 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     auto step = [&](int s, auto Tc) {
         constexpr int t = decltype(Tc)::value;
         constexpr int x = t & 1;
-        // u / red / reac slots of planes s + 1, s, s - 1, s - 2
-        constexpr int up_ = (t + 2) & (RU - 1), u0_ = (t + 1) & (RU - 1), um_ = t & (RU - 1),
-                      umm_ = (t + 3) & (RU - 1);
-        // D slots of planes s + 1, s, s - 1, s - 2
+        // u / reac slots of planes s + 1, s, s - 1
+        constexpr int up_ = (t + 2) & (RU - 1), u0_ = (t + 1) & (RU - 1), um_ = t & (RU - 1);
+        // red-result slots of planes s, s - 1, s - 2, s - 3
+        constexpr int r0_ = (t + 1) & (RR - 1), rm_ = t & (RR - 1), rmm_ = (t + 3) & (RR - 1),
+                      rmmm_ = (t + 2) & (RR - 1);
+        // D slots of planes s + 1, s, s - 1, s - 2, s - 3
         constexpr int dp_ = (t + 2) & (RD - 1), d0_ = (t + 1) & (RD - 1), dm_ = t & (RD - 1),
-                      dmm_ = (t + 7) & (RD - 1);
+                      dmm_ = (t + 7) & (RD - 1), dmmm_ = (t + 6) & (RD - 1);
+        // the black pass's plane k = s - 1 (OB: s - 2): its red neighbours'
+        // planes k - 1, k, k + 1, its D planes, and its own u / reac slot
+        constexpr int bq_m = OB ? rmmm_ : rmm_, bq_0 = OB ? rmm_ : rm_, bq_p = OB ? rm_ : r0_;
+        constexpr int bd_m = OB ? dmmm_ : dmm_, bd_0 = OB ? dmm_ : dm_, bd_p = OB ? dm_ : d0_;
+        constexpr int bu_ = OB ? (t + 7) & (RU - 1) : um_;
         lstore(up_, dp_, up_, SlotC<x>{});
         const T br = x ? nbr1 : nbr0;
         const T bk = x ? nbb1 : nbb0;
@@ -643,13 +661,15 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         T rfzm = L.hz, rfzp = L.hz, rgz = T(0), bfzm = L.hz, bfzp = L.hz, bgz = T(0);
         if constexpr (!ZGHOST) {
             zf(s, rfzm, rfzp, rgz);
-            zf(s - 1, bfzm, bfzp, bgz);
+            zf(s - 1 - (OB ? 1 : 0), bfzm, bfzp, bgz);
         }
         __syncthreads();
         // the prefetch is issued on every step (past the last plane this
-        // block needs it re-reads plane ke + 2, an L2 hit), so the loop body
-        // is straight-line and the compiler keeps the register sets in place
-        gload(min(s + 2, ke + 2), SlotC<1 - x>{});
+        // block needs it re-reads the last planes, L2 hits: pofs clamps them
+        // to the slab), so the loop body is straight-line and the compiler
+        // keeps the register sets in place. With OB the last step's black
+        // plane ke takes its b from gload(ke + 3).
+        gload(min(s + 2, ke + 2 + (OB ? 1 : 0)), SlotC<1 - x>{});
         // red cells of plane s; red cells outside the interior (or planes
         // outside 1..nz) hold 0 in the red ring, so the black pass reads its
         // neighbours without masking them (their face factors are zero)
@@ -675,15 +695,15 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
                 if (L.omega != T(1)) val = u0[e] + L.omega * (val - u0[e]);
                 if (s >= kb && s <= ke) buf_store<T>(val, Ro, C.vs, pofs(s));
             }
-            sr[u0_][C.q] = val;
+            sr[r0_][C.q] = val;
         }
-        __syncthreads();
-        const int k = s - 1;
+        if constexpr (!OB) __syncthreads();
+        const int k = s - 1 - (OB ? 1 : 0);
         const Cell &Bc = x ? B1 : B0;
         if (k >= kb && k <= ke && Bc.o >= 0) {
             const int q = Bc.q, e = Bc.e;
-            // red planes k-1, k, k+1 = s-2, s-1, s
-            const T *rm = sr[umm_], *r0 = sr[um_], *rp = sr[u0_];
+            // red planes k-1, k, k+1
+            const T *rm = sr[bq_m], *r0 = sr[bq_0], *rp = sr[bq_p];
             T v;
             if constexpr (FCS) {
                 const T az = (k == 1 || k == nz) ? L.azb : L.az;
@@ -692,25 +712,27 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
             } else if constexpr (STN) {
                 v = stencil_row(x ? bs1 : bs0, r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q], rp[q], bk);
             } else {
-                const T *Dm = sD[dmm_], *D0 = sD[dm_], *Dp = sD[d0_];
-                const T rc = REAC ? sq[REAC ? um_ : 0][q] : T(0);
+                const T *Dm = sD[bd_m], *D0 = sD[bd_0], *Dp = sD[bd_p];
+                const T rc = REAC ? sq[REAC ? bu_ : 0][q] : T(0);
                 v = fast_row(D0[e], D0[e - 1], D0[e + 1], D0[e - W2], D0[e + W2], Dm[e], Dp[e], r0[q - 1], r0[q + 1],
                              r0[q - W1], r0[q + W1], rm[q], rp[q], Bc.fxm, Bc.fxp, bfym, bfyp, bfzm, bfzp,
                              ZGHOST ? Bc.g : Bc.g + bgz, rc, bk);
             }
-            if (L.omega != T(1)) v = su[um_][e] + L.omega * (v - su[um_][e]);
+            if (L.omega != T(1)) v = su[bu_][e] + L.omega * (v - su[bu_][e]);
             buf_store<T>(v, Ro, Bc.vs, pofs(k));
         }
     };
-    for (int s = kb - 1; s <= ke + 1; s += 8) {
+    // (OB: one step more, for the last black plane)
+    const int se = ke + 1 + (OB ? 1 : 0);
+    for (int s = kb - 1; s <= se; s += 8) {
         step(s, SlotC<0>{});
-        if (s + 1 <= ke + 1) step(s + 1, SlotC<1>{});
-        if (s + 2 <= ke + 1) step(s + 2, SlotC<2>{});
-        if (s + 3 <= ke + 1) step(s + 3, SlotC<3>{});
-        if (s + 4 <= ke + 1) step(s + 4, SlotC<4>{});
-        if (s + 5 <= ke + 1) step(s + 5, SlotC<5>{});
-        if (s + 6 <= ke + 1) step(s + 6, SlotC<6>{});
-        if (s + 7 <= ke + 1) step(s + 7, SlotC<7>{});
+        if (s + 1 <= se) step(s + 1, SlotC<1>{});
+        if (s + 2 <= se) step(s + 2, SlotC<2>{});
+        if (s + 3 <= se) step(s + 3, SlotC<3>{});
+        if (s + 4 <= se) step(s + 4, SlotC<4>{});
+        if (s + 5 <= se) step(s + 5, SlotC<5>{});
+        if (s + 6 <= se) step(s + 6, SlotC<6>{});
+        if (s + 7 <= se) step(s + 7, SlotC<7>{});
     }
 }
 
