@@ -1267,36 +1267,51 @@ template <typename T, int TX, int TY, int KZM>
 __global__ void __launch_bounds__(TX *TY) k_prolong_tri_planes(Lev<T> F, Lev<T> C, T *__restrict__ uf,
                                                               const T *__restrict__ uc, int kz)
 {
+    // the block's coarse values first: for the tile's fine columns i0 ..
+    // i0 + TX - 1 (i0 odd) the coarse columns I0 .. I0 + TX/2, rows likewise,
+    // planes K0 .. K0 + KC - 1, staged in LDS with one coalesced load per
+    // value (each is read by up to eight lanes); the interpolation below then
+    // reads LDS with the arithmetic of the per-lane form
+    static_assert(TX % 2 == 0 && TY % 2 == 0, "even tiles");
+    constexpr int KC = KZM / 2 + 1;                  // coarse planes K0 .. K0 + KC - 1
+    constexpr int CW = TX / 2 + 1, CH = TY / 2 + 1;  // coarse tile
+    __shared__ T sc[KC * CH * CW];
     const int tid = threadIdx.x;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
-    const int i = bx * TX + tid % TX + 1, j = by * TY + tid / TX + 1;
-    if (i > F.nx || j > F.ny) return;
+    const int i0 = bx * TX + 1, j0 = by * TY + 1;
+    const int I0 = (i0 + 1) >> 1, J0 = (j0 + 1) >> 1;
     const int kb = bz * kz + 1, ke = min(kb + kz - 1, F.nz);
+    const int K0 = (kb + 1) >> 1, nk = ke - kb + 1;
+    for (int t = tid; t < KC * CH * CW; t += TX * TY) {
+        const int q = t / (CH * CW), r = t - q * (CH * CW);
+        const int J = J0 + r / CW, I = I0 + r % CW, K = K0 + q;
+        // fine planes kb .. ke use coarse planes K0 .. (ke + 2) / 2 (zero
+        // past the last coarse plane); columns and rows past the ghost layer
+        // serve no lane
+        const bool ok = 2 * q <= nk && K <= C.nz && I <= C.nx + 1 && J <= C.ny + 1;
+        sc[t] = ok ? uc[I + (long long)C.sy * J + C.sz * K] : T(0);
+    }
+    __syncthreads();
+    const int i = i0 + tid % TX, j = j0 + tid / TX;
+    if (i > F.nx || j > F.ny) return;
     const int ia = (i + 1) >> 1, ja = (j + 1) >> 1;
     const int ib = (i & 1) ? ia : ia + 1, jb = (j & 1) ? ja : ja + 1;
     const bool xo = i & 1, yo = j & 1;
-    const long long sy = C.sy;
-    const long long oaa = ia + sy * ja, oba = ib + sy * ja, oab = ia + sy * jb, obb = ib + sy * jb;
-    auto xy = [&](int K) {
-        const T *p = uc + C.sz * K;
+    const int oaa = (ia - I0) + CW * (ja - J0), oba = (ib - I0) + CW * (ja - J0);
+    const int oab = (ia - I0) + CW * (jb - J0), obb = (ib - I0) + CW * (jb - J0);
+    auto xy = [&](int q) {
+        const T *p = sc + q * (CH * CW);
         if (xo) return yo ? p[oaa] : T(0.5) * (p[oaa] + p[oab]);
         const T r0 = T(0.5) * (p[oaa] + p[oba]);
         if (yo) return r0;
         const T r1 = T(0.5) * (p[oab] + p[obb]);
         return T(0.5) * (r0 + r1);
     };
-    constexpr int KC = KZM / 2 + 1;  // coarse planes K0 .. K0 + KC - 1
-    const int K0 = (kb + 1) >> 1, nk = ke - kb + 1;
     T ec[KC], fv[KZM];
     T *col = uf + i + (long long)F.sy * j + F.sz * kb;
 #pragma unroll
-    for (int q = 0; q < KC; ++q) {
-        // fine planes kb .. ke use coarse planes K0 .. (ke + 2) / 2 (zero
-        // past the last coarse plane)
-        const int K = K0 + q;
-        ec[q] = (2 * q <= nk && K <= C.nz) ? xy(K) : T(0);
-    }
+    for (int q = 0; q < KC; ++q) ec[q] = (2 * q <= nk && K0 + q <= C.nz) ? xy(q) : T(0);
 #pragma unroll
     for (int q = 0; q < KZM; ++q) fv[q] = q < nk ? col[F.sz * q] : T(0);
 #pragma unroll
