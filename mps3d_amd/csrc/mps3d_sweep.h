@@ -290,10 +290,12 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     constexpr int RU = OB ? 8 : 4, RD = 8, RR = 4;   // ring depths (powers of two: slots are t & (R - 1))
     constexpr bool STN = OP == OP_STENCIL || OP == OP_FACES;  // explicit 7-point rows (no D tiles)
     // OP_FACES: the rows' face form (L.fc): per cell the diagonal and the
-    // conductances of its six faces (the -, + faces of each direction, the
-    // - one read at the neighbour's offset), so four arrays are streamed
-    // instead of seven; each coefficient is formed as -(G * a), which is
-    // the value k_faces_rows stores for the level's other kernels
+    // conductances of its six faces (the - face of a direction is the +
+    // face of the neighbour), so four arrays are streamed instead of seven;
+    // G_x and G_y are staged in LDS with each plane (a cell's - faces are
+    // its LDS neighbours), the diagonal and G_z are read per cell. Each
+    // coefficient is formed as -(G * a), which is the value k_faces_rows
+    // stores for the level's other kernels
     constexpr bool FCS = OP == OP_FACES;
     // ZGHOST (OP_REF rows): the z faces of planes 1 and nz take the interior
     // factor hz like every other plane, and the staged D of the ghost plane
@@ -303,7 +305,11 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     // the diagonal); the ghost plane of u is staged as 0.
     constexpr bool ZGHOST = OP == OP_REF;
     __shared__ T su[RU][P2];
-    __shared__ T sD[STN ? 1 : RD][STN ? 1 : P2];
+    // (OP_FACES stages its x and y face conductances with u: G_x in sD,
+    // G_y in sGy, in u's ring slots, so a cell's - faces are its LDS
+    // neighbours)
+    __shared__ T sD[FCS ? RU : (STN ? 1 : RD)][(STN && !FCS) ? 1 : P2];
+    __shared__ T sGy[FCS ? RU : 1][FCS ? P2 : 1];
     __shared__ T sq[REAC ? RU : 1][REAC ? P1 : 1];
     __shared__ T sr[RR][P1 + W1];   // (one spare row: an extended tile's last black row reads one below)
 
@@ -453,14 +459,12 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         int q, e, o;    // LDS index (ring plane), LDS index (halo-2 plane), field offset (-1: none)
         int vb, vs;     // in-plane bytes of the cell for loads and for the store (kOob: none)
         T fxm, fxp, g;  // x face factors and x/y DIRC factor
-        int vxm, vym;   // OP_FACES: in-plane bytes of the x- and y- neighbours (kOob: none)
         T ax, ay;       // OP_FACES: the row's reciprocal cell widths in x and y
     };
     auto vbytes = [](int o) { return o >= 0 ? o * (int)sizeof(T) : kOob; };
     auto cell = [&](int q, int e, int o, bool own, T fxm, T fxp, T g, int i, int j) {
         const T ax = (i == 1 || i == nx) ? L.axb : L.ax, ay = (j == 1 || j == ny) ? L.ayb : L.ay;
-        return Cell{q, e, o, vbytes(o), own ? vbytes(o) : kOob, fxm, fxp, g, o >= 0 ? vbytes(o - 1) : kOob,
-                    o >= 0 ? vbytes(o - sy) : kOob, ax, ay};
+        return Cell{q, e, o, vbytes(o), own ? vbytes(o) : kOob, fxm, fxp, g, ax, ay};
     };
     auto red_cell = [&](int pc) {
         return pc ? cell(rq1, re1, ro1, rt1, rfxm1, rfxp1, rgxy1, i_r1, jrg)
@@ -483,6 +487,7 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     // the interior load 0; b of a cell outside the interior or of a plane
     // outside 1..nz is never consumed.
     T pu[2][N2], pD[2][N2], pq[2][N1], nbr0, nbr1, nbb0, nbb1;
+    T pG[FCS ? 2 : 1][FCS ? N2 : 1];
     // stencil rows: the 7 coefficients of the slot's red and black cell
     T rs0[STN ? ST_N : 1], rs1[STN ? ST_N : 1], bs0[STN ? ST_N : 1], bs1[STN ? ST_N : 1];
     bool okp0 = false, okp1 = false;
@@ -496,7 +501,12 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
             pu[x][r] = ZG ? T(0) : buf_load<T>(Ru, v2[r], pk);  // ZG: sweep from a zero guess (uin unread)
-            if (!STN) pD[x][r] = buf_load<T>(Rd, v2[r], pkd);
+            if constexpr (FCS) {
+                pD[x][r] = buf_load<T>(Rs[1], v2[r], pk);
+                pG[x][r] = buf_load<T>(Rs[2], v2[r], pk);
+            } else if (!STN) {
+                pD[x][r] = buf_load<T>(Rd, v2[r], pkd);
+            }
         }
         if (REAC && !STN) {
 #pragma unroll
@@ -511,22 +521,14 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         const T vr = buf_load<T>(Rb, vr_, pr);
         const T vb = buf_load<T>(Rb, vb_, pb);
         if constexpr (FCS) {
-            // the diagonal and the six face conductances, in ST_* order
-            const Cell &CR = x ? R1 : R0, &CB = x ? B1 : B0;
+            // the diagonal and the z face conductances of the cell (the x
+            // and y ones are staged in LDS with the plane)
             const int pr2 = pofs(kp - 2), pb2 = pofs(kp - 3 - (OB ? 1 : 0));
             T *ra = x ? rs1 : rs0, *ba = x ? bs1 : bs0;
             ra[ST_C] = buf_load<T>(Rs[0], vr_, pr);
-            ra[ST_XM] = buf_load<T>(Rs[1], CR.vxm, pr);
-            ra[ST_XP] = buf_load<T>(Rs[1], vr_, pr);
-            ra[ST_YM] = buf_load<T>(Rs[2], CR.vym, pr);
-            ra[ST_YP] = buf_load<T>(Rs[2], vr_, pr);
             ra[ST_ZM] = buf_load<T>(Rs[3], vr_, pr2);
             ra[ST_ZP] = buf_load<T>(Rs[3], vr_, pr);
             ba[ST_C] = buf_load<T>(Rs[0], vb_, pb);
-            ba[ST_XM] = buf_load<T>(Rs[1], CB.vxm, pb);
-            ba[ST_XP] = buf_load<T>(Rs[1], vb_, pb);
-            ba[ST_YM] = buf_load<T>(Rs[2], CB.vym, pb);
-            ba[ST_YP] = buf_load<T>(Rs[2], vb_, pb);
             ba[ST_ZM] = buf_load<T>(Rs[3], vb_, pb2);
             ba[ST_ZP] = buf_load<T>(Rs[3], vb_, pb);
         } else if constexpr (STN) {
@@ -560,7 +562,12 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
             const int q = tid + r * NT;
             if (r * NT + NT <= P2 || q < P2) {
                 su[s3][q] = ok ? pu[x][r] : T(0);
-                if (!STN) sD[s5][q] = pD[x][r] * dm;
+                if constexpr (FCS) {
+                    sD[s5][q] = pD[x][r];
+                    sGy[s5][q] = pG[x][r];
+                } else if (!STN) {
+                    sD[s5][q] = pD[x][r] * dm;
+                }
             }
         }
         if (REAC && !STN) {
@@ -652,7 +659,7 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         constexpr int bq_m = OB ? rmmm_ : rmm_, bq_0 = OB ? rmm_ : rm_, bq_p = OB ? rm_ : r0_;
         constexpr int bd_m = OB ? dmmm_ : dmm_, bd_0 = OB ? dmm_ : dm_, bd_p = OB ? dm_ : d0_;
         constexpr int bu_ = OB ? (t + 7) & (RU - 1) : um_;
-        lstore(up_, dp_, up_, SlotC<x>{});
+        lstore(up_, FCS ? up_ : dp_, up_, SlotC<x>{});
         const T br = x ? nbr1 : nbr0;
         const T bk = x ? nbb1 : nbb0;
         // z face factors of the red plane s and the black plane s - 1
@@ -681,8 +688,9 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
                 const T *um = su[um_], *u0 = su[u0_], *up = su[up_];
                 if constexpr (FCS) {
                     const T az = (s == 1 || s == nz) ? L.azb : L.az;
-                    val = faces_row(x ? rs1 : rs0, C.ax, C.ay, az, u0[e - 1], u0[e + 1], u0[e - W2], u0[e + W2], um[e],
-                                    up[e], br);
+                    const T *gr = x ? rs1 : rs0, *Gx = sD[u0_], *Gy = sGy[u0_];
+                    const T g[ST_N] = {gr[ST_C], Gx[e - 1], Gx[e], Gy[e - W2], Gy[e], gr[ST_ZM], gr[ST_ZP]};
+                    val = faces_row(g, C.ax, C.ay, az, u0[e - 1], u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], br);
                 } else if constexpr (STN) {
                     val = stencil_row(x ? rs1 : rs0, u0[e - 1], u0[e + 1], u0[e - W2], u0[e + W2], um[e], up[e], br);
                 } else {
@@ -707,8 +715,9 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
             T v;
             if constexpr (FCS) {
                 const T az = (k == 1 || k == nz) ? L.azb : L.az;
-                v = faces_row(x ? bs1 : bs0, Bc.ax, Bc.ay, az, r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q],
-                              rp[q], bk);
+                const T *gb = x ? bs1 : bs0, *Gx = sD[bu_], *Gy = sGy[bu_];
+                const T g[ST_N] = {gb[ST_C], Gx[e - 1], Gx[e], Gy[e - W2], Gy[e], gb[ST_ZM], gb[ST_ZP]};
+                v = faces_row(g, Bc.ax, Bc.ay, az, r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q], rp[q], bk);
             } else if constexpr (STN) {
                 v = stencil_row(x ? bs1 : bs0, r0[q - 1], r0[q + 1], r0[q - W1], r0[q + W1], rm[q], rp[q], bk);
             } else {
