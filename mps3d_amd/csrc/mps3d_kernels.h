@@ -65,16 +65,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const T *p, int nbyte
     return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, nbytes, kBufferWord3);
 }
 template <typename T>
-__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
-template <>
-__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-template <>
-__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
-{
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 template <typename T>
 __device__ __forceinline__ void buf_store(T v, __amdgpu_buffer_rsrc_t r, int voff, int soff);
@@ -88,6 +84,22 @@ __device__ __forceinline__ void buf_store<double>(double v, __amdgpu_buffer_rsrc
 {
     typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
+// A 32-bit buffer load into LDS: lane l of the wave writes its word at
+// lds + l (lds is uniform over the wave), out-of-range offsets write 0. The
+// write lands asynchronously and is counted by vmcnt like a register load.
+__device__ __forceinline__ void buf_load_lds(__amdgpu_buffer_rsrc_t r, float *lds, int voff, int soff)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void *)lds, 4, voff, soff, 0, 0);
+}
+// s_waitcnt vmcnt(N) (gfx9 encoding; expcnt and lgkmcnt left at their
+// maxima)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
 }
 
 // per-face multipliers of a staged ghost coefficient (fast sweeps): the

@@ -26,6 +26,9 @@
 #ifndef MPS_SWEEP_ONEBAR
 #define MPS_SWEEP_ONEBAR 0
 #endif
+#ifndef MPS_SWEEP_DMA
+#define MPS_SWEEP_DMA 1  // LDS-direct plane staging: 0 off, 1 face-form rows, 2 all fast rows
+#endif
 
 namespace mps {
 
@@ -304,12 +307,24 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     // and gz/hz - 1 on a DIRC face (the weight is D_c gz, the wall term of
     // the diagonal); the ghost plane of u is staged as 0.
     constexpr bool ZGHOST = OP == OP_REF;
-    __shared__ T su[RU][P2];
+    // DL (FP32 face-form rows; with MPS_SWEEP_DMA=2 also the face-factor
+    // rows, where it measured 2 % slower at 257^3): the staged planes go
+    // from HBM straight into their LDS ring slots (buffer loads
+    // with the LDS flag: lane l of a wave writes word l after the wave's
+    // uniform LDS base), issued one step ahead like the register sets; each
+    // wave waits for its own before the barrier that publishes them. Rows
+    // are padded to whole waves (the padding takes the zeros that lanes past
+    // the halo plane load). Planes outside 1..nz are staged with LDS stores
+    // instead (zeros, or ZGHOST's scaled D).
+    constexpr bool DL = sizeof(T) == 4 && !OB && !REAC && ((MPS_SWEEP_DMA >= 1 && FCS) || (MPS_SWEEP_DMA >= 2 && !STN));
+    constexpr int P2L = DL ? (P2 + 63) / 64 * 64 : P2;
+    static_assert(!DL || NT % 64 == 0, "whole waves");
+    __shared__ T su[RU][P2L];
     // (OP_FACES stages its x and y face conductances with u: G_x in sD,
     // G_y in sGy, in u's ring slots, so a cell's - faces are its LDS
     // neighbours)
-    __shared__ T sD[FCS ? RU : (STN ? 1 : RD)][(STN && !FCS) ? 1 : P2];
-    __shared__ T sGy[FCS ? RU : 1][FCS ? P2 : 1];
+    __shared__ T sD[FCS ? RU : (STN ? 1 : RD)][(STN && !FCS) ? 1 : P2L];
+    __shared__ T sGy[FCS ? RU : 1][FCS ? P2L : 1];
     __shared__ T sq[REAC ? RU : 1][REAC ? P1 : 1];
     __shared__ T sr[RR][P1 + W1];   // (one spare row: an extended tile's last black row reads one below)
 
@@ -492,20 +507,69 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     T rs0[STN ? ST_N : 1], rs1[STN ? ST_N : 1], bs0[STN ? ST_N : 1], bs1[STN ? ST_N : 1];
     bool okp0 = false, okp1 = false;
     T dm0 = T(0), dm1 = T(0);
+    // DL: plane kp of u and D (FCS: u, G_x and G_y) into u slot US and D
+    // slot DS, as direct LDS loads, or, outside 1..nz, as LDS stores of
+    // zeros (ZGHOST: D as zg0 / zg1 times the first / last interior plane,
+    // a register load waited for in place, at most twice per block)
+    auto stage = [&](int kp, auto US, auto DS) {
+        constexpr int us = decltype(US)::value, ds = FCS ? us : decltype(DS)::value;
+        T *const du = su[us];
+        T *const dd = sD[ds];
+        T *const dg = sGy[FCS ? us : 0];
+        if (kp >= 1 && kp <= nz) {
+            const int pk = pofs(kp);
+            const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll
+            for (int r = 0; r < N2; ++r) {
+                const int q0 = r * NT + 64 * w;
+                if (q0 < P2) {
+                    if constexpr (!ZG) buf_load_lds(Ru, (float *)(du + q0), v2[r], pk);
+                    if constexpr (FCS) {
+                        buf_load_lds(Rs[1], (float *)(dd + q0), v2[r], pk);
+                        buf_load_lds(Rs[2], (float *)(dg + q0), v2[r], pk);
+                    } else {
+                        buf_load_lds(Rd, (float *)(dd + q0), v2[r], pk);
+                    }
+                }
+            }
+        } else {
+            const T dmv = (ZGHOST && kp == 0) ? zg0 : (ZGHOST && kp == nz + 1) ? zg1 : T(0);
+            const int pkd = ZGHOST ? pofs(min(max(kp, 1), nz)) : pofs(kp);
+#pragma unroll
+            for (int r = 0; r < N2; ++r) {
+                const int q = tid + r * NT;
+                if (r * NT + NT <= P2L || q < P2L) {
+                    if constexpr (!ZG) du[q] = T(0);
+                    if constexpr (FCS) {
+                        dd[q] = T(0);
+                        dg[q] = T(0);
+                    } else {
+                        dd[q] = ZGHOST ? buf_load<T>(Rd, v2[r], pkd) * dmv : T(0);
+                    }
+                }
+            }
+        }
+    };
+    // DL: the register loads gload issues after a stage (the b values and
+    // the face-form row's diagonal and z faces): a wait for at most that
+    // many outstanding vector-memory operations completes the stage
+    constexpr int KREG = FCS ? 8 : 2;
     auto gload = [&](int kp, auto X) {
         constexpr int x = decltype(X)::value;
         const int pk = pofs(kp);
         // OP_REF: the z ghost planes of D are staged as zg0 / zg1 times the
         // first / last interior plane (see ZGHOST below)
         const int pkd = ZGHOST ? pofs(min(max(kp, 1), nz)) : pk;
+        if constexpr (!DL) {
 #pragma unroll
-        for (int r = 0; r < N2; ++r) {
-            pu[x][r] = ZG ? T(0) : buf_load<T>(Ru, v2[r], pk);  // ZG: sweep from a zero guess (uin unread)
-            if constexpr (FCS) {
-                pD[x][r] = buf_load<T>(Rs[1], v2[r], pk);
-                pG[x][r] = buf_load<T>(Rs[2], v2[r], pk);
-            } else if (!STN) {
-                pD[x][r] = buf_load<T>(Rd, v2[r], pkd);
+            for (int r = 0; r < N2; ++r) {
+                pu[x][r] = ZG ? T(0) : buf_load<T>(Ru, v2[r], pk);  // ZG: sweep from a zero guess (uin unread)
+                if constexpr (FCS) {
+                    pD[x][r] = buf_load<T>(Rs[1], v2[r], pk);
+                    pG[x][r] = buf_load<T>(Rs[2], v2[r], pk);
+                } else if (!STN) {
+                    pD[x][r] = buf_load<T>(Rd, v2[r], pkd);
+                }
             }
         }
         if (REAC && !STN) {
@@ -637,12 +701,26 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     // u/red/reac slot (t + 1) & 3 and D slot (t + 1) & 7, and every LDS
     // access is a per-lane base plus an immediate offset. Plane kb - 2 is
     // staged into slots 0, plane kb - 1 into slots 1.
-    gload(kb - 2, SlotC<0>{});
-    lstore(0, 0, 0, SlotC<0>{});
-    gload(kb - 1, SlotC<0>{});
-    lstore(1, 1, 1, SlotC<0>{});
+    if constexpr (DL) {
+        // (ZG: the u ring stays zero)
+        if constexpr (ZG)
+            for (int q = tid; q < RU * P2L; q += NT) (&su[0][0])[q] = T(0);
+        stage(kb - 2, SlotC<0>{}, SlotC<0>{});
+        stage(kb - 1, SlotC<1>{}, SlotC<1>{});
+        stage(kb, SlotC<2>{}, SlotC<2>{});
+        __builtin_amdgcn_sched_barrier(0);
+    } else {
+        gload(kb - 2, SlotC<0>{});
+        lstore(0, 0, 0, SlotC<0>{});
+        gload(kb - 1, SlotC<0>{});
+        lstore(1, 1, 1, SlotC<0>{});
+    }
     // set of gload(kp): (kp - kb) & 1
     gload(kb, SlotC<0>{});
+    // (DL: the compiler may drop prologue loads whose values the first
+    // steps provably skip, so the steps' count of loads issued after a stage
+    // does not hold here: wait for everything)
+    if constexpr (DL) wait_vmcnt<0>();
     auto step = [&](int s, auto Tc) {
         constexpr int t = decltype(Tc)::value;
         constexpr int x = t & 1;
@@ -659,7 +737,11 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         constexpr int bq_m = OB ? rmmm_ : rmm_, bq_0 = OB ? rmm_ : rm_, bq_p = OB ? rm_ : r0_;
         constexpr int bd_m = OB ? dmmm_ : dmm_, bd_0 = OB ? dmm_ : dm_, bd_p = OB ? dm_ : d0_;
         constexpr int bu_ = OB ? (t + 7) & (RU - 1) : um_;
-        lstore(up_, FCS ? up_ : dp_, up_, SlotC<x>{});
+        if constexpr (DL) {
+            wait_vmcnt<KREG>();  // this wave's stage of plane s + 1 has landed
+        } else {
+            lstore(up_, FCS ? up_ : dp_, up_, SlotC<x>{});
+        }
         const T br = x ? nbr1 : nbr0;
         const T bk = x ? nbb1 : nbb0;
         // z face factors of the red plane s and the black plane s - 1
@@ -676,6 +758,15 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
         // to the slab), so the loop body is straight-line and the compiler
         // keeps the register sets in place. With OB the last step's black
         // plane ke takes its b from gload(ke + 3).
+        // DL: plane s + 2 goes to u slot (t + 3) & 3 and D slot (t + 3) & 7
+        // (those of planes s - 2 and s - 6, whose last readers, the passes
+        // of step s - 1, are behind the barrier above); none is needed past
+        // plane ke + 2, so the last step stages nothing and no LDS load is
+        // left in flight when the block ends
+        if constexpr (DL) {
+            if (s + 2 <= ke + 2) stage(s + 2, SlotC<(t + 3) & (RU - 1)>{}, SlotC<(t + 3) & (RD - 1)>{});
+            __builtin_amdgcn_sched_barrier(0);
+        }
         gload(min(s + 2, ke + 2 + (OB ? 1 : 0)), SlotC<1 - x>{});
         // red cells of plane s; red cells outside the interior (or planes
         // outside 1..nz) hold 0 in the red ring, so the black pass reads its
@@ -705,7 +796,15 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
             }
             sr[r0_][C.q] = val;
         }
-        if constexpr (!OB) __syncthreads();
+        // DL: the barrier after the red pass only publishes the red results'
+        // LDS stores; __syncthreads' fence would also wait for the stage of
+        // plane s + 2 in flight, which no wave reads before the next step's
+        // barrier. The memory clobber keeps LDS accesses on their side.
+        if constexpr (DL) {
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        } else if constexpr (!OB) {
+            __syncthreads();
+        }
         const int k = s - 1 - (OB ? 1 : 0);
         const Cell &Bc = x ? B1 : B0;
         if (k >= kb && k <= ke && Bc.o >= 0) {
