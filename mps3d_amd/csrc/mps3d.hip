@@ -870,8 +870,11 @@ void launch_bottom(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, int op, T *wo
 #ifndef MPS_SWEEP_TY
 #define MPS_SWEEP_TY 8
 #endif
+// (the most planes per z-chunk; k_sweep_fast splits nz into ceil(nz / KZ)
+// even chunks: at 257^3, 15 chunks of 17-18 planes, 3840 blocks, three
+// rounds of the five blocks per CU that the FP32 sweep's LDS and VGPRs allow)
 #ifndef MPS_SWEEP_KZ
-#define MPS_SWEEP_KZ 16
+#define MPS_SWEEP_KZ 18
 #endif
 constexpr int kSweepTX = MPS_SWEEP_TX, kSweepTY = MPS_SWEEP_TY, kSweepKZ = MPS_SWEEP_KZ;
 

@@ -332,8 +332,11 @@ __global__ void __launch_bounds__(TX *TY) MPS_SWEEP_OCC k_sweep_fast(Lev<T> L, c
     int bx, by, bz;
     xcd_tile(bx, by, bz);
     const int i0 = bx * TX + 1, j0 = by * TY + 1;
-    const int kb = bz * kz + 1;
-    const int ke = min(kb + kz - 1, L.nz);
+    // the gridDim.z = ceil(nz / kz) chunks share the planes evenly (lengths
+    // differ by at most one, none above kz), so no chunk is a short tail
+    const int zb = L.nz / (int)gridDim.z, zr = L.nz % (int)gridDim.z;
+    const int kb = bz * zb + min(bz, zr) + 1;
+    const int ke = kb + zb - (bz < zr ? 0 : 1);
     const int sy = L.sy;
     const long long sz = L.sz;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
