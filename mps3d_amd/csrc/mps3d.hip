@@ -898,8 +898,10 @@ void launch_sweep(mps_ctx *c, std::vector<LevelBuf<T>> &L, int l, const T *uin, 
     const Geom &g = c->geom[l];
     // (the zero-guess sweeps, which read no u, are not timed)
     KtScope kt(c, l == 0 && !zero_in ? 0 : -2, sizeof(T) == 4);
+// (blocks a z-chunk length aims at: with 1536 the 129^3 level runs 1216
+// blocks, one round of the resident blocks, where 2048 gave 1664)
 #ifndef MPS_SWEEP_TARGET
-#define MPS_SWEEP_TARGET 2048
+#define MPS_SWEEP_TARGET 1536
 #endif
     const int kz = z_chunk(g, kSweepTX, kSweepTY, MPS_SWEEP_TARGET, kSweepKZ);
     dim3 gr((unsigned)((g.nx + kSweepTX - 1) / kSweepTX), (unsigned)((g.ny + kSweepTY - 1) / kSweepTY),
